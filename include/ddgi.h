@@ -1,0 +1,251 @@
+/*
+ * ddgi.h — C-ABI of the MI355X (gfx950) DDGI irradiance-probe library (libddgi.so).
+ *
+ * The surface mirrors VulkanExp's `class IrradianceProbes`
+ * (src/IrradianceProbes.hpp:16-129, src/IrradianceProbes.cpp:12-624): a handle owns the probe
+ * grid, its atlases, the probe-state buffer and the scene acceleration structure; `ddgi_update`
+ * is one call of `IrradianceProbes::update` (trace -> irradiance/depth blend -> border copy ->
+ * publish).  Every entry point returns a `ddgi_status`; nothing throws across the ABI.
+ *
+ * Conventions
+ *  - Pointers documented as "device" are HIP device pointers (hipMalloc / torch CUDA tensors);
+ *    "host" pointers are ordinary CPU memory.  Streams are `hipStream_t` passed as `void*`
+ *    (NULL = the legacy default stream).
+ *  - A handle is externally synchronised: one thread and one stream at a time.  Independent
+ *    handles may be used concurrently from different threads on different streams.
+ *  - Atlases are kept on the device in a probe-major tiled layout (DDGI_LAYOUT_TILED); the
+ *    reference's 2D image layout (DDGI_LAYOUT_IMAGE: width colorRes*Rx*Ry, height colorRes*Rz)
+ *    is available on read-back.
+ */
+#ifndef DDGI_H
+#define DDGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDGI_ABI_VERSION 1u
+
+typedef enum ddgi_status {
+	DDGI_OK = 0,
+	DDGI_ERROR_INVALID_ARGUMENT = 1, /* null handle/pointer, bad size, bad grid, NaN/Inf input */
+	DDGI_ERROR_OUT_OF_MEMORY = 2,    /* device or host allocation failed */
+	DDGI_ERROR_HIP = 3,              /* a HIP runtime call failed; see ddgi_last_error */
+	DDGI_ERROR_UNSUPPORTED = 4,      /* valid request this build does not implement */
+	DDGI_ERROR_NO_SCENE = 5,         /* update/init/trace before ddgi_set_scene */
+	DDGI_ERROR_INVALID_STATE = 6,    /* call order violated (e.g. saved state of another grid) */
+	DDGI_ERROR_NO_DEVICE = 7         /* no usable gfx950 device */
+} ddgi_status;
+
+/* Grid parameters: 64 bytes, same field order and meaning as the std140 UBO `GridInfo`
+ * (src/IrradianceProbes.hpp:49-60, mirrored by src/shaders/ProbeGrid.glsl:4-15). */
+typedef struct ddgi_grid_params {
+	float extent_min[3];    /* world position of probe (0,0,0) */
+	float depth_sharpness;  /* exponent of the depth-blend weight (reference default 12) */
+	float extent_max[3];    /* world position of probe (Rx-1,Ry-1,Rz-1) */
+	float hysteresis;       /* weight of the previous texel value in the blend */
+	int32_t resolution[3];  /* probes per axis, each >= 2 */
+	uint32_t rays_per_probe;/* 1 .. ddgi_options.max_rays_per_probe */
+	uint32_t color_res;     /* irradiance tile edge incl. 1-texel border; only 8 is supported */
+	uint32_t depth_res;     /* depth tile edge incl. 1-texel border; only 16 is supported */
+	float shadow_bias;      /* sampleProbes position bias (reference default 0.3) */
+	uint32_t padding;
+} ddgi_grid_params;
+
+/* 64-byte vertex record (src/vulkan/Vertex.hpp:8-15, unpacked by src/shaders/Vertex.glsl:16-33). */
+typedef struct ddgi_vertex {
+	float pos[3];
+	float color[3];
+	float normal[3];
+	float tangent[4];
+	float tex_coord[2];
+	uint32_t padding;
+} ddgi_vertex;
+
+/* 48-byte material record (src/vulkan/Material.hpp:16-25, src/shaders/unpackMaterial.glsl:7-19).
+ * Textures are not sampled by this library; texture ids must be 0xFFFFFFFF. */
+typedef struct ddgi_material {
+	float metallic;
+	float roughness;
+	float base_color[3];
+	float emissive[3];
+	uint32_t albedo_texture;
+	uint32_t normal_texture;
+	uint32_t metallic_roughness_texture;
+	uint32_t emissive_texture;
+} ddgi_material;
+
+/* Mesh offset-table entry (src/Renderer.hpp:21-25; built at src/Renderer.cpp:97-126). */
+typedef struct ddgi_offset_entry {
+	uint32_t material_index;
+	uint32_t vertex_offset; /* in vertices */
+	uint32_t index_offset;  /* in indices */
+} ddgi_offset_entry;
+
+/* One TLAS instance (src/Renderer.cpp:532-551): 3x4 row-major object-to-world transform, the
+ * custom index (= offset-table entry of its mesh), the 8-bit visibility mask, flags and the
+ * mesh's triangle count (the BLAS extent, src/Renderer.cpp:323-362). */
+typedef struct ddgi_instance {
+	float transform[12];
+	uint32_t custom_index;
+	uint32_t mask;
+	uint32_t flags;          /* DDGI_INSTANCE_FLIP_FACING */
+	uint32_t triangle_count;
+} ddgi_instance;
+
+#define DDGI_INSTANCE_FLIP_FACING 0x1u
+#define DDGI_MASK_STATIC 0x01u  /* src/shaders/InstanceMasks.glsl:1-3 */
+#define DDGI_MASK_DYNAMIC 0x02u
+
+/* Directional light UBO (src/Light.hpp:6-9): direction points TOWARD the light, color.rgb is
+ * the radiance, color.a the sky brightness of the reference's sky model. */
+typedef struct ddgi_light {
+	float direction[4];
+	float color[4];
+} ddgi_light;
+
+typedef struct ddgi_options {
+	uint32_t max_rays_per_probe; /* capacity of the ray buffers (reference: 256); <= 512; 0 = 512 */
+	uint32_t probes_per_update;  /* reference ProbesPerUpdate (src/IrradianceProbes.hpp:44); 0 = no cap */
+	float target_hysteresis;     /* reference TargetHysteresis (0.98); used with DDGI_UPDATE_HYSTERESIS_RAMP */
+	int32_t device;              /* HIP device ordinal; -1 = current device */
+	uint32_t reserved[4];
+} ddgi_options;
+
+/* ddgi_update flags */
+#define DDGI_UPDATE_ALL_PROBES 0x1u       /* trace+blend every probe of the grid (indices 0..P-1) */
+#define DDGI_UPDATE_BLEND_BF16 0x2u       /* blend on BF16 MFMA (hi/lo compensated), FP32 accumulation */
+#define DDGI_UPDATE_BLEND_FP16 0x4u       /* blend on FP16 MFMA, FP32 accumulation */
+#define DDGI_UPDATE_HYSTERESIS_RAMP 0x8u  /* reference hysteresis ramp (src/IrradianceProbes.cpp:462-476) */
+#define DDGI_UPDATE_NO_TIMING 0x10u       /* skip per-phase HIP events */
+
+/* Atlas kinds / layouts */
+#define DDGI_ATLAS_IRRADIANCE 0u /* float4 texels (rgb, 1) */
+#define DDGI_ATLAS_DEPTH 1u      /* float2 texels (mean, mean of squares) */
+#define DDGI_LAYOUT_TILED 0u     /* [probe][row][col] texels, probe tiles of res x res */
+#define DDGI_LAYOUT_IMAGE 1u     /* reference 2D image, row-major, width res*Rx*Ry */
+
+/* Ray query record for ddgi_trace_rays (a device array). */
+typedef struct ddgi_ray {
+	float origin[3];
+	float tmin;
+	float direction[3];
+	float tmax;
+} ddgi_ray;
+
+/* Hit record written by ddgi_trace_rays: instance = TLAS instance index, primitive = triangle
+ * index within the instance's mesh (gl_PrimitiveID), u/v = barycentric weights of vertices 1
+ * and 2 (the closest-hit `attribs`), flags bit0 = back-facing.  Misses: instance = 0xFFFFFFFF,
+ * t = -1.  For any-hit queries only `instance != 0xFFFFFFFF` is meaningful. */
+typedef struct ddgi_hit {
+	float t;
+	float u;
+	float v;
+	uint32_t instance;
+	uint32_t primitive;
+	uint32_t flags;
+} ddgi_hit;
+
+#define DDGI_TRACE_ANY_HIT 0x1u /* terminate on first hit (shadow-ray semantics) */
+
+/* Per-phase GPU times of the last completed update, in milliseconds (the reference's timestamp
+ * split, src/IrradianceProbes.cpp:441-452). */
+typedef struct ddgi_timings {
+	float select_ms;
+	float trace_ms;   /* primary traversal + shading + shadow rays */
+	float blend_ms;   /* irradiance + depth blend with fused border copy */
+	float border_ms;  /* always 0: the border copy is fused into the blend epilogue */
+	float copy_ms;    /* always 0: the work->exposed copy is a pointer swap */
+	float total_ms;
+	uint32_t probes_updated;
+	uint32_t rays_traced;
+} ddgi_timings;
+
+/* Raw device view of a handle's buffers for zero-copy interop (torch). Valid until the next
+ * ddgi_update / ddgi_set_scene / ddgi_destroy on the handle. */
+typedef struct ddgi_device_view {
+	void* irradiance;        /* exposed irradiance atlas, tiled float4 */
+	void* depth;             /* exposed depth atlas, tiled float2 */
+	void* states;            /* uint32[P] probe states */
+	void* rays;              /* float4[P*max_rays] (rgb, depth) of the last update, slot-major */
+	void* hits;              /* internal hit records of the last update */
+	void* probe_indices;     /* uint32[P] probes traced by the last update */
+	void* ray_directions;    /* float4[max_rays] world directions of the last update */
+	uint64_t probe_count;
+	uint32_t max_rays_per_probe;
+	uint32_t last_update_count; /* probes updated by the last update (host copy; -1 if unknown) */
+} ddgi_device_view;
+
+/* ---- lifetime (IrradianceProbes::init / destroy, src/IrradianceProbes.cpp:12-104,596-624) ---- */
+ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* options, void** out_handle);
+ddgi_status ddgi_destroy(void* handle);
+uint32_t ddgi_abi_version(void);
+const char* ddgi_status_string(ddgi_status status);
+const char* ddgi_last_error(void* handle);
+
+/* ---- configuration (updateUniforms / setLightBuffer / writeDescriptorSet) ----
+ * ddgi_set_params: IrradianceProbes::updateUniforms (src/IrradianceProbes.cpp:106-108); the
+ *   resolution must equal the one given at creation (the reference re-inits to resize).
+ * ddgi_set_light: the light UBO of binding 10 (src/IrradianceProbes.cpp:327-336).
+ * ddgi_set_sky: constant radiance returned by misses (replaces src/shaders/miss.rmiss:17-23). */
+ddgi_status ddgi_set_params(void* handle, const ddgi_grid_params* params);
+ddgi_status ddgi_get_params(void* handle, ddgi_grid_params* out_params);
+ddgi_status ddgi_set_light(void* handle, const ddgi_light* light);
+ddgi_status ddgi_set_sky(void* handle, const float rgb[3]);
+
+/* ---- scene (Renderer::allocateMeshes + createAccelerationStructures + createTLAS,
+ * src/Renderer.cpp:57-95,272-449,525-642).  All arrays are host memory and are copied.
+ * Builds the binned-SAH BVH on the host and uploads it. */
+ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices,
+						   size_t index_count, const ddgi_offset_entry* offsets, size_t offset_count,
+						   const ddgi_instance* instances, size_t instance_count, const ddgi_material* materials,
+						   size_t material_count);
+/* Node / triangle counts and bytes of the uploaded BVH. */
+ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes);
+
+/* ---- per-frame work ----
+ * ddgi_init_probes: IrradianceProbes::initProbes (src/IrradianceProbes.cpp:357-394) with the
+ *   shader src/shaders/probesInit.rgen:31-64; orientation = row-major 3x3 R (rows X,Y,Z of
+ *   genBasis, src/IrradianceProbes.cpp:347-367), ray i = R * sphericalFibonacci(i, n).
+ * ddgi_update: IrradianceProbes::update (src/IrradianceProbes.cpp:426-594).  Asynchronous on
+ *   `stream`; selection of the probes happens on the device. */
+ddgi_status ddgi_init_probes(void* handle, const float orientation[9], void* stream);
+ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags, void* stream);
+
+/* Blend only: feed an externally produced ray buffer `rays` (device float4[count*n], slot-major
+ * (rgb, depth), n = rays_per_probe) for probes `probe_indices` (device uint32[count]) and the
+ * frame orientation, then run blend + border + state update + publish like ddgi_update does. */
+ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_indices, uint32_t count,
+							const float orientation[9], uint32_t flags, void* stream);
+
+/* Trace arbitrary rays against the scene BVH (device arrays).  `mask` is the ray cull mask. */
+ddgi_status ddgi_trace_rays(void* handle, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
+							uint32_t flags, void* stream);
+
+/* sampleProbes (src/shaders/irradiance.glsl:145-237) at `count` points: device float3 arrays
+ * (packed xyz) position, normal, to_camera; writes device float3 rgb. */
+ddgi_status ddgi_sample(void* handle, const float* position, const float* normal, const float* to_camera,
+						uint64_t count, float* rgb_out, void* stream);
+
+/* ---- state access ---- */
+ddgi_status ddgi_get_atlas(void* handle, uint32_t kind, uint32_t layout, void* host_dst, size_t bytes);
+ddgi_status ddgi_set_atlas(void* handle, uint32_t kind, uint32_t layout, const void* host_src, size_t bytes);
+ddgi_status ddgi_get_states(void* handle, uint32_t* host_dst, size_t count);
+ddgi_status ddgi_set_states(void* handle, const uint32_t* host_src, size_t count);
+ddgi_status ddgi_get_timings(void* handle, ddgi_timings* out);
+ddgi_status ddgi_get_device_view(void* handle, ddgi_device_view* out);
+/* Save/restore atlases + states + schedule counters + hysteresis.  Call with dst NULL to query
+ * the size. */
+ddgi_status ddgi_save_state(void* handle, void* host_dst, size_t* bytes);
+ddgi_status ddgi_load_state(void* handle, const void* host_src, size_t bytes);
+/* Block the host until all work queued on the handle's last stream is done. */
+ddgi_status ddgi_synchronize(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDGI_H */

@@ -1,0 +1,928 @@
+// C-ABI of libddgi (include/ddgi.h): the IrradianceProbes surface of VulkanExp
+// (src/IrradianceProbes.hpp:16-129, src/IrradianceProbes.cpp:12-624) over HIP device memory.
+//
+// One handle owns: the probe grid parameters, two atlas pairs (exposed / next, swapped after each
+// update instead of the reference's work->exposed image copy), probe states, the ray buffers, the
+// per-frame blend weights and the scene (BVH + shading records).  Per-frame work is enqueued on
+// the caller's stream; nothing here synchronises unless the call returns host data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "bvh.hpp"
+#include "kernels.hpp"
+
+namespace ddgi {
+hipError_t launch_set_count(SelectState* st, uint32_t count, hipStream_t s);
+}
+
+using namespace ddgi;
+
+namespace {
+
+constexpr double kPiD = 3.1415926538;  // src/shaders/common.glsl:4
+constexpr uint32_t kStateMagic = 0x49474444u;  // "DDGI"
+constexpr uint32_t kStateVersion = 1u;
+
+thread_local std::string g_last_error;
+
+struct DeviceBuffer {
+	void* p = nullptr;
+	size_t bytes = 0;
+};
+
+struct Handle {
+	ddgi_grid_params params{};
+	ddgi_options opts{};
+	int device = 0;
+	uint32_t P = 0;
+	uint32_t max_rays = 0;
+	// atlases: [cur] is exposed, [cur ^ 1] receives the next update
+	float4* irr[2] = {nullptr, nullptr};
+	float2* dep[2] = {nullptr, nullptr};
+	int cur = 0;
+	uint32_t* states = nullptr;
+	uint32_t* indices = nullptr;
+	SelectState* sel = nullptr;
+	bool indices_are_iota = false;
+	float4* rays = nullptr;
+	float4* hits = nullptr;
+	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
+	uint16_t* perm = nullptr;   // kMaxRays
+	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512
+	float4* dirs = nullptr;     // world directions, kMaxRays
+	uint32_t fib_n = 0;
+	float* wt = nullptr;
+	float* inv_sum = nullptr;
+	void* w_hi = nullptr;
+	void* w_lo = nullptr;
+	uint32_t* visit = nullptr;  // traversal counters (optional statistics)
+	// scene
+	std::vector<DeviceBuffer> scene_bufs;
+	SceneDev sd{};
+	bool has_scene = false;
+	BvhHost bvh_meta;
+	size_t bvh_bytes = 0;
+	LightConst light{};
+	// timing
+	hipEvent_t ev[5] = {};
+	bool timing_pending = false;
+	ddgi_timings last{};
+	uint32_t last_count = 0xFFFFFFFFu;
+	uint32_t updated_probes = 0;  // hysteresis ramp bookkeeping (src/IrradianceProbes.cpp:463)
+	float uploaded_hysteresis = 0.0f;  // the value the kernels use (the reference's UBO copy)
+	hipStream_t last_stream = nullptr;
+	std::string err;
+};
+
+ddgi_status fail(Handle* h, ddgi_status s, const std::string& msg) {
+	if(h) h->err = msg;
+	g_last_error = msg;
+	return s;
+}
+
+#define HIP_TRY(h, expr)                                                                               \
+	do {                                                                                               \
+		hipError_t e_ = (expr);                                                                        \
+		if(e_ != hipSuccess) {                                                                         \
+			return fail((h), e_ == hipErrorOutOfMemory ? DDGI_ERROR_OUT_OF_MEMORY : DDGI_ERROR_HIP,    \
+						std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+		}                                                                                              \
+	} while(0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t bytes) {
+	void* q = nullptr;
+	hipError_t e = hipMalloc(&q, std::max<size_t>(bytes, 16));
+	*p = static_cast<T*>(q);
+	return e;
+}
+
+bool finite_all(const float* v, int n) {
+	for(int i = 0; i < n; ++i)
+		if(!std::isfinite(v[i])) return false;
+	return true;
+}
+
+ddgi_status validate_params(const ddgi_grid_params* p, uint32_t max_rays, std::string& why) {
+	for(int k = 0; k < 3; ++k) {
+		if(p->resolution[k] < 2) {
+			why = "resolution must be >= 2 on every axis (probeGridCellSize divides by resolution - 1)";
+			return DDGI_ERROR_INVALID_ARGUMENT;
+		}
+		if(!(p->extent_max[k] > p->extent_min[k])) {
+			why = "extent_max must exceed extent_min on every axis";
+			return DDGI_ERROR_INVALID_ARGUMENT;
+		}
+	}
+	if(!finite_all(p->extent_min, 3) || !finite_all(p->extent_max, 3) || !std::isfinite(p->hysteresis) ||
+	   !std::isfinite(p->depth_sharpness) || !std::isfinite(p->shadow_bias)) {
+		why = "non-finite grid parameter";
+		return DDGI_ERROR_INVALID_ARGUMENT;
+	}
+	if(p->rays_per_probe == 0 || p->rays_per_probe > max_rays) {
+		why = "rays_per_probe must be in [1, max_rays_per_probe]";
+		return DDGI_ERROR_INVALID_ARGUMENT;
+	}
+	if(p->color_res != 8 || p->depth_res != 16) {
+		why = "only colorRes 8 and depthRes 16 are supported (the reference bakes them into its shaders)";
+		return DDGI_ERROR_UNSUPPORTED;
+	}
+	const uint64_t P = uint64_t(p->resolution[0]) * p->resolution[1] * p->resolution[2];
+	if(P > (1u << 24)) {
+		why = "more than 2^24 probes";
+		return DDGI_ERROR_UNSUPPORTED;
+	}
+	return DDGI_OK;
+}
+
+GridConst make_grid_const(const Handle& h) {
+	GridConst g{};
+	const ddgi_grid_params& p = h.params;
+	for(int k = 0; k < 3; ++k) {
+		g.emin[k] = p.extent_min[k];
+		g.cell[k] = (p.extent_max[k] - p.extent_min[k]) / float(p.resolution[k] - 1);
+		g.res[k] = p.resolution[k];
+	}
+	g.cell_len = std::sqrt(g.cell[0] * g.cell[0] + g.cell[1] * g.cell[1] + g.cell[2] * g.cell[2]);
+	const float ex = p.extent_max[0] - p.extent_min[0], ey = p.extent_max[1] - p.extent_min[1],
+				ez = p.extent_max[2] - p.extent_min[2];
+	g.tmax_primary = std::sqrt(ex * ex + ey * ey + ez * ez);
+	g.hysteresis = h.uploaded_hysteresis;
+	g.sharpness = p.depth_sharpness;
+	g.shadow_bias = p.shadow_bias;
+	g.n = p.rays_per_probe;
+	g.probe_count = h.P;
+	return g;
+}
+
+// sphericalFibonacci (src/shaders/irradiance.glsl:52-64) evaluated in double, rounded to float.
+void fib_host(uint32_t i, uint32_t n, float out[3]) {
+	const double di = double(i), dn = double(n);
+	const double PHI = std::sqrt(5.0) * 0.5 + 0.5;
+	const double a = di * (PHI - 1.0);
+	const double phi = 2.0 * kPiD * (a - std::floor(a));
+	const double ct = 1.0 - (2.0 * di + 1.0) * (1.0 / dn);
+	double s2 = 1.0 - ct * ct;
+	s2 = s2 < 0.0 ? 0.0 : (s2 > 1.0 ? 1.0 : s2);
+	const double st = std::sqrt(s2);
+	out[0] = float(std::cos(phi) * st);
+	out[1] = float(std::sin(phi) * st);
+	out[2] = float(ct);
+}
+
+uint32_t morton2(uint32_t x, uint32_t y) {
+	uint32_t r = 0;
+	for(int b = 0; b < 8; ++b) r |= ((x >> b) & 1u) << (2 * b) | ((y >> b) & 1u) << (2 * b + 1);
+	return r;
+}
+
+// Upload fib(i, n) for the frame rays, the 512 init rays, and a thread->ray order that groups
+// directions by octahedral cell, so a wavefront traces a compact cone of directions.
+ddgi_status ensure_fib(Handle* h) {
+	const uint32_t n = h->params.rays_per_probe;
+	if(h->fib_n == n) return DDGI_OK;
+	std::vector<float> f(size_t(kMaxRays) * 4, 0.0f), f512(512 * 4, 0.0f);
+	for(uint32_t i = 0; i < n; ++i) fib_host(i, n, &f[i * 4]);
+	for(uint32_t i = 0; i < 512; ++i) fib_host(i, n, &f512[i * 4]);
+	std::vector<uint32_t> key(n);
+	for(uint32_t i = 0; i < n; ++i) {
+		const double x = f[i * 4], y = f[i * 4 + 1], z = f[i * 4 + 2];
+		const double l1 = std::fabs(x) + std::fabs(y) + std::fabs(z);
+		double u = x / l1, v = y / l1;
+		if(z < 0.0) {
+			const double nu = (1.0 - std::fabs(v)) * (u >= 0.0 ? 1.0 : -1.0);
+			const double nv = (1.0 - std::fabs(u)) * (v >= 0.0 ? 1.0 : -1.0);
+			u = nu;
+			v = nv;
+		}
+		const uint32_t qu = std::min(15u, uint32_t((u * 0.5 + 0.5) * 16.0));
+		const uint32_t qv = std::min(15u, uint32_t((v * 0.5 + 0.5) * 16.0));
+		key[i] = morton2(qu, qv);
+	}
+	std::vector<uint16_t> perm(kMaxRays);
+	std::iota(perm.begin(), perm.begin() + n, uint16_t(0));
+	std::stable_sort(perm.begin(), perm.begin() + n, [&](uint16_t a, uint16_t b) { return key[a] < key[b]; });
+	for(uint32_t i = n; i < kMaxRays; ++i) perm[i] = uint16_t(i);
+	HIP_TRY(h, hipMemcpy(h->fib, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(h->fib512, f512.data(), f512.size() * sizeof(float), hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(h->perm, perm.data(), perm.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+	h->fib_n = n;
+	return DDGI_OK;
+}
+
+ddgi_status set_device(Handle* h) {
+	int cur = -1;
+	HIP_TRY(h, hipGetDevice(&cur));
+	if(cur != h->device) HIP_TRY(h, hipSetDevice(h->device));
+	return DDGI_OK;
+}
+
+void free_scene(Handle* h) {
+	for(auto& b : h->scene_bufs) (void)hipFree(b.p);
+	h->scene_bufs.clear();
+	h->has_scene = false;
+	h->sd = SceneDev{};
+}
+
+template <class T>
+ddgi_status upload(Handle* h, const T* src, size_t count, const T** dst) {
+	DeviceBuffer b;
+	b.bytes = std::max<size_t>(count * sizeof(T), 16);
+	HIP_TRY(h, hipMalloc(&b.p, b.bytes));
+	h->scene_bufs.push_back(b);
+	if(count) HIP_TRY(h, hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+	*dst = static_cast<const T*>(b.p);
+	return DDGI_OK;
+}
+
+ddgi_status make_rotation(const float* o, Rot& R) {
+	if(!o || !finite_all(o, 9)) return DDGI_ERROR_INVALID_ARGUMENT;
+	std::memcpy(R.r, o, sizeof(R.r));
+	return DDGI_OK;
+}
+
+void tiled_to_image(const float* tiled, float* img, const Handle& h, uint32_t res, uint32_t ch) {
+	const uint32_t rx = h.params.resolution[0], ry = h.params.resolution[1], rz = h.params.resolution[2];
+	const uint32_t W = res * rx * ry;
+	for(uint32_t p = 0; p < h.P; ++p) {
+		const uint32_t gx = p % rx, gy = (p % (rx * ry)) / rx, gz = p / (rx * ry);
+		const uint32_t ox = res * (gy * rx + gx), oy = res * gz;
+		for(uint32_t y = 0; y < res; ++y)
+			std::memcpy(&img[(size_t(oy + y) * W + ox) * ch], &tiled[(size_t(p) * res * res + y * res) * ch],
+						sizeof(float) * res * ch);
+	}
+	(void)rz;
+}
+
+void image_to_tiled(const float* img, float* tiled, const Handle& h, uint32_t res, uint32_t ch) {
+	const uint32_t rx = h.params.resolution[0], ry = h.params.resolution[1];
+	const uint32_t W = res * rx * ry;
+	for(uint32_t p = 0; p < h.P; ++p) {
+		const uint32_t gx = p % rx, gy = (p % (rx * ry)) / rx, gz = p / (rx * ry);
+		const uint32_t ox = res * (gy * rx + gx), oy = res * gz;
+		for(uint32_t y = 0; y < res; ++y)
+			std::memcpy(&tiled[(size_t(p) * res * res + y * res) * ch], &img[(size_t(oy + y) * W + ox) * ch],
+						sizeof(float) * res * ch);
+	}
+}
+
+Handle* H(void* p) { return static_cast<Handle*>(p); }
+
+void destroy_handle(Handle* h) {
+	free_scene(h);
+	for(int i = 0; i < 2; ++i) {
+		(void)hipFree(h->irr[i]);
+		(void)hipFree(h->dep[i]);
+	}
+	void* bufs[] = {h->states, h->indices, h->sel, h->rays, h->hits, h->fib, h->perm, h->fib512, h->dirs,
+					h->wt, h->inv_sum, h->w_hi, h->w_lo, h->visit};
+	for(void* b : bufs) (void)hipFree(b);
+	for(auto& e : h->ev)
+		if(e) (void)hipEventDestroy(e);
+	delete h;
+}
+
+ddgi_status enqueue_blend_and_publish(Handle* h, const Rot& R, const float4* rays, uint32_t max_slots, bool all,
+									  uint32_t flags, hipStream_t s) {
+	const GridConst g = make_grid_const(*h);
+	const uint32_t mode = (flags & DDGI_UPDATE_BLEND_BF16) ? kBlendBF16 : ((flags & DDGI_UPDATE_BLEND_FP16) ? kBlendFP16 : kBlendFP32);
+	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s));
+	HIP_TRY(h, launch_blend_weights(g, h->dirs, h->wt, h->inv_sum, h->w_hi, h->w_lo, mode, s));
+	const int nxt = h->cur ^ 1;
+	if(!all) {
+		// probes that are not updated keep their tiles: start the next atlas as a copy
+		HIP_TRY(h, hipMemcpyAsync(h->irr[nxt], h->irr[h->cur], size_t(h->P) * 64 * sizeof(float4), hipMemcpyDeviceToDevice, s));
+		HIP_TRY(h, hipMemcpyAsync(h->dep[nxt], h->dep[h->cur], size_t(h->P) * 256 * sizeof(float2), hipMemcpyDeviceToDevice, s));
+	}
+	BlendArgs b{};
+	b.grid = g;
+	b.indices = h->indices;
+	b.count = &h->sel->count;
+	b.rays = rays;
+	b.wt = h->wt;
+	b.inv_sum = h->inv_sum;
+	b.w_hi16 = h->w_hi;
+	b.w_lo16 = h->w_lo;
+	b.states = h->states;
+	b.irr_prev = h->irr[h->cur];
+	b.dep_prev = h->dep[h->cur];
+	b.irr_out = h->irr[nxt];
+	b.dep_out = h->dep[nxt];
+	b.max_slots = max_slots;
+	HIP_TRY(h, launch_blend(b, mode, s));
+	h->cur = nxt;  // publish (reference: vkCmdCopyImage work -> exposed, src/IrradianceProbes.cpp:530-576)
+	return DDGI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t ddgi_abi_version(void) { return DDGI_ABI_VERSION; }
+
+const char* ddgi_status_string(ddgi_status s) {
+	switch(s) {
+		case DDGI_OK: return "ok";
+		case DDGI_ERROR_INVALID_ARGUMENT: return "invalid argument";
+		case DDGI_ERROR_OUT_OF_MEMORY: return "out of memory";
+		case DDGI_ERROR_HIP: return "HIP runtime error";
+		case DDGI_ERROR_UNSUPPORTED: return "unsupported";
+		case DDGI_ERROR_NO_SCENE: return "no scene";
+		case DDGI_ERROR_INVALID_STATE: return "invalid state";
+		case DDGI_ERROR_NO_DEVICE: return "no device";
+	}
+	return "unknown status";
+}
+
+const char* ddgi_last_error(void* handle) {
+	if(handle) return H(handle)->err.c_str();
+	return g_last_error.c_str();
+}
+
+ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* options, void** out_handle) {
+	if(!params || !out_handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	*out_handle = nullptr;
+	ddgi_options opts{};
+	if(options) opts = *options;
+	if(opts.max_rays_per_probe == 0) opts.max_rays_per_probe = kMaxRays;
+	if(opts.max_rays_per_probe > kMaxRays) return fail(nullptr, DDGI_ERROR_UNSUPPORTED, "max_rays_per_probe above 512");
+	if(opts.target_hysteresis == 0.0f) opts.target_hysteresis = 0.98f;  // src/IrradianceProbes.hpp:46
+	std::string why;
+	ddgi_status st = validate_params(params, opts.max_rays_per_probe, why);
+	if(st != DDGI_OK) return fail(nullptr, st, why);
+	int ndev = 0;
+	if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, DDGI_ERROR_NO_DEVICE, "no HIP device");
+	Handle* h = new(std::nothrow) Handle;
+	if(!h) return fail(nullptr, DDGI_ERROR_OUT_OF_MEMORY, "host allocation failed");
+	h->params = *params;
+	h->uploaded_hysteresis = params->hysteresis;
+	h->opts = opts;
+	if(opts.device >= 0) {
+		if(opts.device >= ndev) {
+			delete h;
+			return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "device ordinal out of range");
+		}
+		h->device = opts.device;
+	} else if(hipGetDevice(&h->device) != hipSuccess) {
+		delete h;
+		return fail(nullptr, DDGI_ERROR_NO_DEVICE, "hipGetDevice failed");
+	}
+	hipDeviceProp_t prop;
+	if(hipGetDeviceProperties(&prop, h->device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+		delete h;
+		return fail(nullptr, DDGI_ERROR_NO_DEVICE, "device is not gfx950");
+	}
+	auto cleanup_fail = [&](ddgi_status s) {
+		std::string m = h->err;
+		destroy_handle(h);
+		return fail(nullptr, s, m);
+	};
+	if((st = set_device(h)) != DDGI_OK) return cleanup_fail(st);
+	h->P = uint32_t(params->resolution[0]) * params->resolution[1] * params->resolution[2];
+	h->max_rays = opts.max_rays_per_probe;
+	const size_t P = h->P, R = h->max_rays;
+	hipError_t e = hipSuccess;
+	for(int i = 0; i < 2 && e == hipSuccess; ++i) {
+		e = dalloc(&h->irr[i], P * 64 * sizeof(float4));
+		if(e == hipSuccess) e = dalloc(&h->dep[i], P * 256 * sizeof(float2));
+	}
+	if(e == hipSuccess) e = dalloc(&h->states, P * 4);
+	if(e == hipSuccess) e = dalloc(&h->indices, P * 4);
+	if(e == hipSuccess) e = dalloc(&h->sel, sizeof(SelectState));
+	if(e == hipSuccess) e = dalloc(&h->rays, P * R * sizeof(float4));
+	if(e == hipSuccess) e = dalloc(&h->hits, P * R * sizeof(float4));
+	if(e == hipSuccess) e = dalloc(&h->fib, kMaxRays * sizeof(float4));
+	if(e == hipSuccess) e = dalloc(&h->perm, kMaxRays * sizeof(uint16_t));
+	if(e == hipSuccess) e = dalloc(&h->fib512, 512 * sizeof(float4));
+	if(e == hipSuccess) e = dalloc(&h->dirs, 512 * sizeof(float4));
+	if(e == hipSuccess) e = dalloc(&h->wt, size_t(kMaxRays) * kBlendTexels * sizeof(float));
+	if(e == hipSuccess) e = dalloc(&h->inv_sum, kBlendTexels * sizeof(float));
+	if(e == hipSuccess) e = dalloc(&h->w_hi, size_t(kMaxRays) * kBlendTexels * 2);
+	if(e == hipSuccess) e = dalloc(&h->w_lo, size_t(kMaxRays) * kBlendTexels * 2);
+	if(e == hipSuccess) e = dalloc(&h->visit, 16);
+	if(e != hipSuccess) {
+		h->err = std::string("allocation failed: ") + hipGetErrorString(e);
+		return cleanup_fail(e == hipErrorOutOfMemory ? DDGI_ERROR_OUT_OF_MEMORY : DDGI_ERROR_HIP);
+	}
+	// zeroed atlases (the reference leaves them uninitialised), all probes enabled (state 1)
+	e = hipMemset(h->irr[0], 0, P * 64 * sizeof(float4));
+	if(e == hipSuccess) e = hipMemset(h->irr[1], 0, P * 64 * sizeof(float4));
+	if(e == hipSuccess) e = hipMemset(h->dep[0], 0, P * 256 * sizeof(float2));
+	if(e == hipSuccess) e = hipMemset(h->dep[1], 0, P * 256 * sizeof(float2));
+	if(e == hipSuccess) e = hipMemset(h->sel, 0, sizeof(SelectState));
+	if(e == hipSuccess) e = hipMemset(h->rays, 0, P * R * sizeof(float4));
+	if(e == hipSuccess) e = hipMemset(h->visit, 0, 16);
+	if(e == hipSuccess) {
+		std::vector<uint32_t> ones(P, 1u);
+		e = hipMemcpy(h->states, ones.data(), P * 4, hipMemcpyHostToDevice);
+	}
+	for(int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
+	if(e != hipSuccess) {
+		h->err = std::string("initialisation failed: ") + hipGetErrorString(e);
+		return cleanup_fail(DDGI_ERROR_HIP);
+	}
+	h->light.dir[0] = 0.2f / std::sqrt(4.08f);  // LightBuffer default (src/Light.hpp:7-8)
+	h->light.dir[1] = 2.0f / std::sqrt(4.08f);
+	h->light.dir[2] = 0.2f / std::sqrt(4.08f);
+	for(int k = 0; k < 3; ++k) {
+		h->light.color[k] = 10.0f;
+		h->light.sky[k] = 0.5f;
+	}
+	if((st = ensure_fib(h)) != DDGI_OK) return cleanup_fail(st);
+	*out_handle = h;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_destroy(void* handle) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	set_device(h);
+	(void)hipDeviceSynchronize();
+	destroy_handle(h);
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_params(void* handle, const ddgi_grid_params* params) {
+	if(!handle || !params) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	for(int k = 0; k < 3; ++k)
+		if(params->resolution[k] != h->params.resolution[k])
+			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "resolution is fixed at creation (re-create the handle to resize)");
+	std::string why;
+	ddgi_status st = validate_params(params, h->max_rays, why);
+	if(st != DDGI_OK) return fail(h, st, why);
+	if((st = set_device(h)) != DDGI_OK) return st;
+	h->params = *params;
+	h->uploaded_hysteresis = params->hysteresis;  // updateUniforms (src/IrradianceProbes.cpp:106-108)
+	return ensure_fib(h);
+}
+
+ddgi_status ddgi_get_params(void* handle, ddgi_grid_params* out) {
+	if(!handle || !out) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	*out = H(handle)->params;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_light(void* handle, const ddgi_light* light) {
+	if(!handle || !light) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	if(!finite_all(light->direction, 3) || !finite_all(light->color, 3))
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "non-finite light");
+	for(int k = 0; k < 3; ++k) {
+		h->light.dir[k] = light->direction[k];
+		h->light.color[k] = light->color[k];
+	}
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_sky(void* handle, const float rgb[3]) {
+	if(!handle || !rgb) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	if(!finite_all(rgb, 3)) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "non-finite sky");
+	for(int k = 0; k < 3; ++k) H(handle)->light.sky[k] = rgb[k];
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices,
+						   size_t index_count, const ddgi_offset_entry* offsets, size_t offset_count,
+						   const ddgi_instance* instances, size_t instance_count, const ddgi_material* materials,
+						   size_t material_count) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if((vertex_count && !vertices) || (index_count && !indices) || (offset_count && !offsets) ||
+	   (instance_count && !instances) || (material_count && !materials))
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null array with non-zero count");
+	for(size_t k = 0; k < offset_count; ++k)
+		if(offsets[k].material_index >= material_count)
+			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "offset entry references a missing material");
+	for(size_t k = 0; k < material_count; ++k) {
+		const ddgi_material& m = materials[k];
+		if(!std::isfinite(m.metallic) || !std::isfinite(m.roughness) || !finite_all(m.base_color, 3) || !finite_all(m.emissive, 3))
+			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "non-finite material");
+	}
+	BvhHost bvh;
+	const char* why = nullptr;
+	if(!build_bvh(vertices, vertex_count, indices, index_count, offsets, offset_count, instances, instance_count, bvh, &why))
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, why ? why : "invalid scene");
+	if(bvh.tri_count == 0) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "scene has no triangles");
+	if(bvh.max_depth >= 64) return fail(h, DDGI_ERROR_UNSUPPORTED, "BVH deeper than the 64-entry traversal stack");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	(void)hipDeviceSynchronize();  // the previous scene may still be read by queued work
+	free_scene(h);
+	SceneDev sd{};
+	std::vector<InstanceDev> inst(instance_count);
+	for(size_t k = 0; k < instance_count; ++k) {
+		const float* M = instances[k].transform;
+		const double a[9] = {M[0], M[1], M[2], M[4], M[5], M[6], M[8], M[9], M[10]};
+		const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+		if(det == 0.0) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "singular instance transform");
+		const double cof[9] = {a[4] * a[8] - a[5] * a[7], -(a[3] * a[8] - a[5] * a[6]), a[3] * a[7] - a[4] * a[6],
+							   -(a[1] * a[8] - a[2] * a[7]), a[0] * a[8] - a[2] * a[6], -(a[0] * a[7] - a[1] * a[6]),
+							   a[1] * a[5] - a[2] * a[4], -(a[0] * a[5] - a[2] * a[3]), a[0] * a[4] - a[1] * a[3]};
+		for(int q = 0; q < 9; ++q) inst[k].nm[q] = float(cof[q] / det);
+		inst[k].custom_index = instances[k].custom_index;
+		inst[k].prim_base = bvh.inst_prim_base[k];
+		inst[k].mask = instances[k].mask;
+	}
+	std::vector<float> mats(material_count * 12);
+	for(size_t k = 0; k < material_count; ++k) std::memcpy(&mats[k * 12], &materials[k], 48);
+	const float4* nodes = nullptr;
+	const float4* tris = nullptr;
+	const InstanceDev* ip = nullptr;
+	const uint32_t* op = nullptr;
+	const uint32_t* ixp = nullptr;
+	const float4* vp = nullptr;
+	const float* mp = nullptr;
+	if((st = upload(h, reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &nodes)) != DDGI_OK ||
+	   (st = upload(h, reinterpret_cast<const float4*>(bvh.tris.data()), bvh.tris.size() / 4, &tris)) != DDGI_OK ||
+	   (st = upload(h, inst.data(), inst.size(), &ip)) != DDGI_OK ||
+	   (st = upload(h, reinterpret_cast<const uint32_t*>(offsets), offset_count * 3, &op)) != DDGI_OK ||
+	   (st = upload(h, indices, index_count, &ixp)) != DDGI_OK ||
+	   (st = upload(h, reinterpret_cast<const float4*>(vertices), vertex_count * 4, &vp)) != DDGI_OK ||
+	   (st = upload(h, mats.data(), mats.size(), &mp)) != DDGI_OK) {
+		free_scene(h);
+		return st;
+	}
+	sd.nodes = nodes;
+	sd.tris = tris;
+	sd.inst = ip;
+	sd.offsets = op;
+	sd.indices = ixp;
+	sd.vertices = vp;
+	sd.materials = mp;
+	sd.tri_count = bvh.tri_count;
+	sd.node_count = bvh.node_count;
+	sd.instance_count = uint32_t(instance_count);
+	sd.stack_depth = bvh.max_depth < 32 ? 32 : 64;
+	h->sd = sd;
+	h->bvh_bytes = bvh.nodes.size() * 4 + bvh.tris.size() * 4;
+	bvh.nodes.clear();
+	bvh.tris.clear();
+	h->bvh_meta = std::move(bvh);
+	h->has_scene = true;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "no scene");
+	if(node_count) *node_count = h->sd.node_count;
+	if(triangle_count) *triangle_count = h->sd.tri_count;
+	if(bytes) *bytes = h->bvh_bytes;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_init_probes(void* handle, const float orientation[9], void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_init_probes");
+	Rot R;
+	if(make_rotation(orientation, R) != DDGI_OK) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid orientation");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK || (st = ensure_fib(h)) != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	// initProbes resets the host-side hysteresis that the ramp moves, without uploading it
+	// (src/IrradianceProbes.cpp:358)
+	h->params.hysteresis = 0.0f;
+	const GridConst g = make_grid_const(*h);
+	HIP_TRY(h, launch_frame_dirs(R, h->fib512, h->dirs, 512, s));
+	HIP_TRY(h, launch_init_probes(h->sd, g, h->dirs, h->states, nullptr, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_update");
+	if((flags & DDGI_UPDATE_BLEND_BF16) && (flags & DDGI_UPDATE_BLEND_FP16))
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "BF16 and FP16 blends are exclusive");
+	Rot R;
+	if(make_rotation(orientation, R) != DDGI_OK) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid orientation");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK || (st = ensure_fib(h)) != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	const bool timing = !(flags & DDGI_UPDATE_NO_TIMING);
+	const bool all = (flags & DDGI_UPDATE_ALL_PROBES) != 0;
+	if(timing) HIP_TRY(h, hipEventRecord(h->ev[0], s));
+	uint32_t max_slots = h->P;
+	if(all) {
+		if(!h->indices_are_iota) HIP_TRY(h, launch_iota(h->indices, h->P, h->sel, s));
+		else HIP_TRY(h, launch_set_count(h->sel, h->P, s));
+		h->indices_are_iota = true;
+	} else {
+		HIP_TRY(h, launch_select(h->states, h->P, h->opts.probes_per_update, h->sel, h->indices, s));
+		h->indices_are_iota = false;
+		if(h->opts.probes_per_update) max_slots = std::min(max_slots, h->opts.probes_per_update);
+	}
+	if(timing) HIP_TRY(h, hipEventRecord(h->ev[1], s));
+	if(flags & DDGI_UPDATE_HYSTERESIS_RAMP) {
+		// src/IrradianceProbes.cpp:462-476 with its order: in the first branch the uniforms are
+		// uploaded BEFORE the value moves, so the change reaches the kernels one ramp step later
+		uint32_t cnt = h->P;
+		if(!all) {
+			SelectState ss;
+			HIP_TRY(h, hipMemcpyAsync(&ss, h->sel, sizeof(ss), hipMemcpyDeviceToHost, s));
+			HIP_TRY(h, hipStreamSynchronize(s));
+			cnt = ss.count;
+		}
+		if(h->updated_probes >= h->P) {
+			const float target = h->opts.target_hysteresis;
+			if(std::fabs(h->params.hysteresis - target) > 0.05f) {
+				h->uploaded_hysteresis = h->params.hysteresis;
+				h->params.hysteresis += 0.1f * (target - h->params.hysteresis);
+			} else if(h->params.hysteresis != target) {
+				h->params.hysteresis = target;
+				h->uploaded_hysteresis = target;
+			}
+			h->updated_probes -= h->P;
+		}
+		h->updated_probes += cnt;
+	}
+	const GridConst g = make_grid_const(*h);
+	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s));
+	TraceFrameArgs a{};
+	a.scene = h->sd;
+	a.grid = g;
+	a.light = h->light;
+	a.indices = h->indices;
+	a.count = &h->sel->count;
+	a.dirs = h->dirs;
+	a.perm = h->perm;
+	a.states = h->states;
+	a.irr = h->irr[h->cur];
+	a.dep = h->dep[h->cur];
+	a.hits = h->hits;
+	a.rays = h->rays;
+	a.max_slots = max_slots;
+	a.visit_counters = nullptr;
+	HIP_TRY(h, launch_trace_primary(a, s));
+	HIP_TRY(h, launch_shade(a, s));
+	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
+	if((st = enqueue_blend_and_publish(h, R, h->rays, max_slots, all, flags, s)) != DDGI_OK) return st;
+	if(timing) {
+		HIP_TRY(h, hipEventRecord(h->ev[3], s));
+		HIP_TRY(h, hipEventRecord(h->ev[4], s));
+		h->timing_pending = true;
+	}
+	h->last_count = all ? h->P : 0xFFFFFFFFu;
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_indices, uint32_t count,
+							const float orientation[9], uint32_t flags, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!rays || !probe_indices || count == 0 || count > h->P)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "rays/probe_indices null or count outside [1, P]");
+	if((flags & DDGI_UPDATE_BLEND_BF16) && (flags & DDGI_UPDATE_BLEND_FP16))
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "BF16 and FP16 blends are exclusive");
+	Rot R;
+	if(make_rotation(orientation, R) != DDGI_OK) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid orientation");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK || (st = ensure_fib(h)) != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	const bool timing = !(flags & DDGI_UPDATE_NO_TIMING);
+	if(timing) {
+		HIP_TRY(h, hipEventRecord(h->ev[0], s));
+		HIP_TRY(h, hipEventRecord(h->ev[1], s));
+		HIP_TRY(h, hipEventRecord(h->ev[2], s));
+	}
+	HIP_TRY(h, hipMemcpyAsync(h->indices, probe_indices, size_t(count) * 4, hipMemcpyDeviceToDevice, s));
+	HIP_TRY(h, launch_set_count(h->sel, count, s));
+	h->indices_are_iota = false;
+	// the listed probes may not cover the grid: the next atlas starts as a copy of the exposed one
+	if((st = enqueue_blend_and_publish(h, R, static_cast<const float4*>(rays), count, false, flags, s)) != DDGI_OK)
+		return st;
+	if(timing) {
+		HIP_TRY(h, hipEventRecord(h->ev[3], s));
+		HIP_TRY(h, hipEventRecord(h->ev[4], s));
+		h->timing_pending = true;
+	}
+	h->last_count = count;
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_trace_rays(void* handle, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
+							uint32_t flags, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "no scene");
+	if(count && (!rays || !hits)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null rays/hits");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, launch_trace_rays(h->sd, rays, hits, count, mask, (flags & DDGI_TRACE_ANY_HIT) != 0, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_sample(void* handle, const float* position, const float* normal, const float* to_camera, uint64_t count,
+						float* rgb_out, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(count && (!position || !normal || !to_camera || !rgb_out)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null array");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, launch_sample(make_grid_const(*h), h->states, h->irr[h->cur], h->dep[h->cur], position, normal, to_camera,
+							 count, rgb_out, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_atlas(void* handle, uint32_t kind, uint32_t layout, void* host_dst, size_t bytes) {
+	if(!handle || !host_dst) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	if(kind > 1 || layout > 1) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "unknown atlas kind/layout");
+	const uint32_t res = kind == 0 ? 8 : 16, ch = kind == 0 ? 4 : 2;
+	const size_t need = size_t(h->P) * res * res * ch * sizeof(float);
+	if(bytes != need) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "destination size mismatch");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	const void* src = kind == 0 ? static_cast<const void*>(h->irr[h->cur]) : static_cast<const void*>(h->dep[h->cur]);
+	if(layout == DDGI_LAYOUT_TILED) {
+		HIP_TRY(h, hipMemcpy(host_dst, src, need, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> tmp(need / sizeof(float));
+		HIP_TRY(h, hipMemcpy(tmp.data(), src, need, hipMemcpyDeviceToHost));
+		tiled_to_image(tmp.data(), static_cast<float*>(host_dst), *h, res, ch);
+	}
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_atlas(void* handle, uint32_t kind, uint32_t layout, const void* host_src, size_t bytes) {
+	if(!handle || !host_src) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	if(kind > 1 || layout > 1) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "unknown atlas kind/layout");
+	const uint32_t res = kind == 0 ? 8 : 16, ch = kind == 0 ? 4 : 2;
+	const size_t need = size_t(h->P) * res * res * ch * sizeof(float);
+	if(bytes != need) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "source size mismatch");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	void* dst = kind == 0 ? static_cast<void*>(h->irr[h->cur]) : static_cast<void*>(h->dep[h->cur]);
+	if(layout == DDGI_LAYOUT_TILED) {
+		HIP_TRY(h, hipMemcpy(dst, host_src, need, hipMemcpyHostToDevice));
+	} else {
+		std::vector<float> tmp(need / sizeof(float));
+		image_to_tiled(static_cast<const float*>(host_src), tmp.data(), *h, res, ch);
+		HIP_TRY(h, hipMemcpy(dst, tmp.data(), need, hipMemcpyHostToDevice));
+	}
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_states(void* handle, uint32_t* host_dst, size_t count) {
+	if(!handle || !host_dst) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	if(count != h->P) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "count must equal the probe count");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	HIP_TRY(h, hipMemcpy(host_dst, h->states, count * 4, hipMemcpyDeviceToHost));
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_states(void* handle, const uint32_t* host_src, size_t count) {
+	if(!handle || !host_src) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	if(count != h->P) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "count must equal the probe count");
+	for(size_t i = 0; i < count; ++i)
+		if(host_src[i] > 8) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "probe states are in [0, 8]");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	HIP_TRY(h, hipMemcpy(h->states, host_src, count * 4, hipMemcpyHostToDevice));
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_timings(void* handle, ddgi_timings* out) {
+	if(!handle || !out) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	if(h->timing_pending) {
+		HIP_TRY(h, hipEventSynchronize(h->ev[4]));
+		float ms[4] = {};
+		for(int i = 0; i < 4; ++i) HIP_TRY(h, hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+		h->last.select_ms = ms[0];
+		h->last.trace_ms = ms[1];
+		h->last.blend_ms = ms[2];
+		h->last.border_ms = 0.0f;
+		h->last.copy_ms = 0.0f;
+		HIP_TRY(h, hipEventElapsedTime(&h->last.total_ms, h->ev[0], h->ev[4]));
+		uint32_t cnt = h->last_count;
+		if(cnt == 0xFFFFFFFFu) {
+			SelectState ss;
+			HIP_TRY(h, hipMemcpy(&ss, h->sel, sizeof(ss), hipMemcpyDeviceToHost));
+			cnt = ss.count;
+		}
+		h->last.probes_updated = cnt;
+		h->last.rays_traced = cnt * h->params.rays_per_probe;
+		h->timing_pending = false;
+	}
+	*out = h->last;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_device_view(void* handle, ddgi_device_view* out) {
+	if(!handle || !out) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	out->irradiance = h->irr[h->cur];
+	out->depth = h->dep[h->cur];
+	out->states = h->states;
+	out->rays = h->rays;
+	out->hits = h->hits;
+	out->probe_indices = h->indices;
+	out->ray_directions = h->dirs;
+	out->probe_count = h->P;
+	out->max_rays_per_probe = h->max_rays;
+	out->last_update_count = h->last_count;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_synchronize(void* handle) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	return DDGI_OK;
+}
+
+// Saved state: header (magic, version, P, rays/probe, hysteresis bits, updated_probes, select
+// state) + irradiance tiles + depth tiles + states.
+ddgi_status ddgi_save_state(void* handle, void* host_dst, size_t* bytes) {
+	if(!handle || !bytes) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	const size_t irr_b = size_t(h->P) * 64 * 16, dep_b = size_t(h->P) * 256 * 8, st_b = size_t(h->P) * 4;
+	const size_t hdr = 16 * 4;
+	const size_t need = hdr + irr_b + dep_b + st_b;
+	if(!host_dst) {
+		*bytes = need;
+		return DDGI_OK;
+	}
+	if(*bytes < need) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "buffer too small");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	uint32_t head[16] = {};
+	SelectState ss;
+	HIP_TRY(h, hipMemcpy(&ss, h->sel, sizeof(ss), hipMemcpyDeviceToHost));
+	head[0] = kStateMagic;
+	head[1] = kStateVersion;
+	head[2] = h->P;
+	head[3] = h->params.rays_per_probe;
+	std::memcpy(&head[4], &h->params.hysteresis, 4);
+	head[5] = h->updated_probes;
+	head[6] = ss.last_offset;
+	head[7] = ss.loop_index;
+	std::memcpy(&head[8], &h->uploaded_hysteresis, 4);
+	char* d = static_cast<char*>(host_dst);
+	std::memcpy(d, head, hdr);
+	HIP_TRY(h, hipMemcpy(d + hdr, h->irr[h->cur], irr_b, hipMemcpyDeviceToHost));
+	HIP_TRY(h, hipMemcpy(d + hdr + irr_b, h->dep[h->cur], dep_b, hipMemcpyDeviceToHost));
+	HIP_TRY(h, hipMemcpy(d + hdr + irr_b + dep_b, h->states, st_b, hipMemcpyDeviceToHost));
+	*bytes = need;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_load_state(void* handle, const void* host_src, size_t bytes) {
+	if(!handle || !host_src) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	const size_t irr_b = size_t(h->P) * 64 * 16, dep_b = size_t(h->P) * 256 * 8, st_b = size_t(h->P) * 4;
+	const size_t hdr = 16 * 4;
+	if(bytes != hdr + irr_b + dep_b + st_b) return fail(h, DDGI_ERROR_INVALID_STATE, "state size does not match this grid");
+	uint32_t head[16];
+	std::memcpy(head, host_src, hdr);
+	if(head[0] != kStateMagic || head[1] != kStateVersion || head[2] != h->P)
+		return fail(h, DDGI_ERROR_INVALID_STATE, "state belongs to another grid or version");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	const char* s = static_cast<const char*>(host_src);
+	HIP_TRY(h, hipMemcpy(h->irr[h->cur], s + hdr, irr_b, hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(h->dep[h->cur], s + hdr + irr_b, dep_b, hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(h->states, s + hdr + irr_b + dep_b, st_b, hipMemcpyHostToDevice));
+	SelectState ss{head[6], head[7], 0, 0};
+	HIP_TRY(h, hipMemcpy(h->sel, &ss, sizeof(ss), hipMemcpyHostToDevice));
+	std::memcpy(&h->params.hysteresis, &head[4], 4);
+	std::memcpy(&h->uploaded_hysteresis, &head[8], 4);
+	h->updated_probes = head[5];
+	h->indices_are_iota = false;
+	return DDGI_OK;
+}
+
+}  // extern "C"

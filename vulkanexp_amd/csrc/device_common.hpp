@@ -1,0 +1,342 @@
+// Device-side building blocks shared by the probe kernels (gfx950).
+//
+// Compiled with -ffp-contract=off in the trace translation unit: the intersection arithmetic
+// must be exactly the FP32 operations listed here, in this order, with no fused multiply-add,
+// so that hit ids, t and barycentrics are reproducible bit for bit on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace ddgi {
+
+// ---------------------------------------------------------------------------------------------
+// Ray precomputation for the watertight test (Woop, Benthin, Wald, JCGT 2(1) 2013): kz is the
+// dominant axis of the direction (ties resolved x < y < z as written below), (kx, ky) the other
+// two in cyclic order, swapped when d[kz] < 0 to keep the winding; shear constants Sx, Sy, Sz.
+struct RayPrep {
+	float o[3];
+	int kx, ky, kz;
+	float sx, sy, sz;
+	float inv[3];  // 1/d for the slab tests
+	float okx, oky, okz;
+};
+
+__device__ __forceinline__ float sel3(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
+__device__ __forceinline__ void ray_prepare(float ox, float oy, float oz, float dx, float dy, float dz, RayPrep& r) {
+	const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+	const int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+	int kx = kz + 1;
+	if(kx == 3) kx = 0;
+	int ky = kx + 1;
+	if(ky == 3) ky = 0;
+	const float dkz = sel3(dx, dy, dz, kz);
+	if(dkz < 0.0f) {
+		const int t = kx;
+		kx = ky;
+		ky = t;
+	}
+	r.kx = kx;
+	r.ky = ky;
+	r.kz = kz;
+	r.sx = sel3(dx, dy, dz, kx) / dkz;
+	r.sy = sel3(dx, dy, dz, ky) / dkz;
+	r.sz = 1.0f / dkz;
+	r.o[0] = ox;
+	r.o[1] = oy;
+	r.o[2] = oz;
+	r.inv[0] = 1.0f / dx;
+	r.inv[1] = 1.0f / dy;
+	r.inv[2] = 1.0f / dz;
+	r.okx = sel3(ox, oy, oz, kx);
+	r.oky = sel3(ox, oy, oz, ky);
+	r.okz = sel3(ox, oy, oz, kz);
+}
+
+// Returns true when the triangle is hit with tmin <= t <= tmax; U,V,W,det are the edge functions.
+__device__ __forceinline__ bool tri_intersect(const RayPrep& r, float4 a, float4 b, float4 c, float tmin, float tmax,
+											  float& t, float& V, float& W, float& det) {
+	const float Akx = sel3(a.x, a.y, a.z, r.kx) - r.okx, Aky = sel3(a.x, a.y, a.z, r.ky) - r.oky,
+				Akz = sel3(a.x, a.y, a.z, r.kz) - r.okz;
+	const float Bkx = sel3(b.x, b.y, b.z, r.kx) - r.okx, Bky = sel3(b.x, b.y, b.z, r.ky) - r.oky,
+				Bkz = sel3(b.x, b.y, b.z, r.kz) - r.okz;
+	const float Ckx = sel3(c.x, c.y, c.z, r.kx) - r.okx, Cky = sel3(c.x, c.y, c.z, r.ky) - r.oky,
+				Ckz = sel3(c.x, c.y, c.z, r.kz) - r.okz;
+	const float Ax = Akx - r.sx * Akz, Ay = Aky - r.sy * Akz;
+	const float Bx = Bkx - r.sx * Bkz, By = Bky - r.sy * Bkz;
+	const float Cx = Ckx - r.sx * Ckz, Cy = Cky - r.sy * Ckz;
+	float U = Cx * By - Cy * Bx;
+	V = Ax * Cy - Ay * Cx;
+	W = Bx * Ay - By * Ax;
+	if(U == 0.0f || V == 0.0f || W == 0.0f) {
+		U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+		V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+		W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+	}
+	if((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+	det = U + V + W;
+	if(det == 0.0f) return false;
+	const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+	const float T = U * Az + V * Bz + W * Cz;
+	t = T / det;
+	return t >= tmin && t <= tmax;
+}
+
+// Conservative slab test of one child box: [tn, tf] clipped to [tmin, tmax], far end widened by
+// a few ulps (robust traversal, Ize 2013); boxes are also inflated at build time.
+__device__ __forceinline__ bool slab(const RayPrep& r, float lox, float hix, float loy, float hiy, float loz, float hiz,
+									 float tmin, float tmax, float& tn) {
+	const float x0 = (lox - r.o[0]) * r.inv[0], x1 = (hix - r.o[0]) * r.inv[0];
+	const float y0 = (loy - r.o[1]) * r.inv[1], y1 = (hiy - r.o[1]) * r.inv[1];
+	const float z0 = (loz - r.o[2]) * r.inv[2], z1 = (hiz - r.o[2]) * r.inv[2];
+	tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+	const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+	return tn <= tf * 1.000001f;
+}
+
+struct HitResult {
+	float t;
+	uint32_t slot;  // triangle position in the BVH triangle array, 0xFFFFFFFF = miss
+	uint32_t gid;   // global triangle id (ties go to the lower one)
+	uint32_t meta;  // mask | flip_facing << 8 of the hit triangle
+	float V, W, det;
+};
+
+// Closest-hit (ANY = false; ties -> lowest global id) or any-hit traversal of the BVH, with a
+// per-thread stack in LDS laid out [entry][thread] (conflict-free).  `stack` points at this
+// thread's column; `stride` is the block size.
+template <bool ANY, int STACK>
+__device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+											  const RayPrep& r, float tmin, float tmax, uint32_t mask,
+											  uint32_t* stack, int stride, uint32_t* visits) {
+	HitResult best;
+	best.t = tmax;
+	best.slot = 0xFFFFFFFFu;
+	best.gid = 0xFFFFFFFFu;
+	best.meta = 0u;
+	best.V = best.W = best.det = 0.0f;
+	int sp = 0;
+	uint32_t node = 0;  // root (always an inner node)
+	uint32_t n_nodes = 0, n_tris = 0;
+	for(;;) {
+		// descend through inner nodes
+		while(!(node & kLeafFlag)) {
+			const float4* nr = nodes + size_t(node) * 4;
+			const float4 a = nr[0], b = nr[1], c = nr[2], d = nr[3];
+			++n_nodes;
+			float t0, t1;
+			const bool h0 = slab(r, a.x, a.y, a.z, a.w, c.x, c.y, tmin, best.t, t0);
+			const bool h1 = slab(r, b.x, b.y, b.z, b.w, c.z, c.w, tmin, best.t, t1);
+			const uint32_t c0 = __float_as_uint(d.x), c1 = __float_as_uint(d.y);
+			if(h0 && h1) {
+				const bool swap = t1 < t0;
+				node = swap ? c1 : c0;
+				stack[sp * stride] = swap ? c0 : c1;
+				++sp;
+			} else if(h0) {
+				node = c0;
+			} else if(h1) {
+				node = c1;
+			} else {
+				if(sp == 0) goto done;
+				--sp;
+				node = stack[sp * stride];
+			}
+		}
+		// leaf: up to 8 triangles
+		{
+			const uint32_t first = (node & ~kLeafFlag) >> 3, cnt = (node & 7u) + 1u;
+			for(uint32_t k = 0; k < cnt; ++k) {
+				const float4* tr = tris + size_t(first + k) * 3;
+				const float4 ta = tr[0], tb = tr[1], tc = tr[2];
+				++n_tris;
+				if((__float_as_uint(tc.w) & mask) == 0u) continue;
+				float t, V, W, det;
+				if(!tri_intersect(r, ta, tb, tc, tmin, best.t, t, V, W, det)) continue;
+				const uint32_t gid = __float_as_uint(ta.w);
+				if(best.slot != 0xFFFFFFFFu && !(t < best.t || gid < best.gid)) continue;
+				best.t = t;
+				best.slot = first + k;
+				best.gid = gid;
+				best.meta = __float_as_uint(tc.w);
+				best.V = V;
+				best.W = W;
+				best.det = det;
+				if(ANY) goto done;
+			}
+		}
+		if(sp == 0) break;
+		--sp;
+		node = stack[sp * stride];
+	}
+done:
+	if(visits) {
+		visits[0] = n_nodes;
+		visits[1] = n_tris;
+	}
+	return best;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Octahedral maps
+__device__ __forceinline__ float sign_not_zero(float v) { return v >= 0.0f ? 1.0f : -1.0f; }
+__device__ __forceinline__ float gsign(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+
+// spherePointToOctohedralUV (src/shaders/irradiance.glsl:119-138), GLSL sign() semantics.
+__device__ __forceinline__ void oct_uv(float dx, float dy, float dz, float& u, float& v) {
+	const float sx = gsign(dx), sy = gsign(dy), sz = gsign(dz);
+	const float sum = dx * sx + dy * sy + dz * sz;
+	float ox = dx / sum, oy = dy / sum;
+	const float oz = dz / sum;
+	if(oz < 0.0f) {
+		const float ax = fabsf(ox), ay = fabsf(oy);
+		ox = sx * (1.0f - ay);
+		oy = sy * (1.0f - ax);
+	}
+	u = ox * 0.5f + 0.5f;
+	v = oy * 0.5f + 0.5f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bilinear fetch inside one probe tile (texel coords 1 + (res-2)*uv, centres at k + 0.5).
+template <int RES>
+__device__ __forceinline__ void tile_coords(float u, float v, int& x0, int& y0, int& x1, int& y1, float& wx, float& wy) {
+	const float sx = 1.0f + float(RES - 2) * u - 0.5f;
+	const float sy = 1.0f + float(RES - 2) * v - 0.5f;
+	const float fx = floorf(sx), fy = floorf(sy);
+	wx = sx - fx;
+	wy = sy - fy;
+	x0 = min(max(int(fx), 0), RES - 1);
+	y0 = min(max(int(fy), 0), RES - 1);
+	x1 = min(x0 + 1, RES - 1);
+	y1 = min(y0 + 1, RES - 1);
+}
+
+struct Vec3 {
+	float x, y, z;
+};
+__device__ __forceinline__ Vec3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ Vec3 operator+(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ Vec3 operator-(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ Vec3 operator*(Vec3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ Vec3 operator*(Vec3 a, Vec3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ Vec3 normalize3(Vec3 a) {
+	const float l = sqrtf(dot3(a, a));
+	return {a.x / l, a.y / l, a.z / l};
+}
+
+// sampleProbes (src/shaders/irradiance.glsl:145-237) on the tiled FP32 atlases.
+__device__ __forceinline__ Vec3 sample_probes(const GridConst& g, const uint32_t* __restrict__ states,
+											  const float4* __restrict__ irr, const float2* __restrict__ dep,
+											  Vec3 position, Vec3 normal, Vec3 to_camera) {
+	const Vec3 acell = v3(fabsf(g.cell[0]), fabsf(g.cell[1]), fabsf(g.cell[2]));
+	const Vec3 emin = v3(g.emin[0], g.emin[1], g.emin[2]);
+	const Vec3 gc = v3((position.x - emin.x) / acell.x, (position.y - emin.y) / acell.y, (position.z - emin.z) / acell.z);
+	if(gc.x < 0.0f || gc.y < 0.0f || gc.z < 0.0f) return v3(0.0f, 0.0f, 0.0f);
+	const Vec3 biased = position + (normal + to_camera) * g.shadow_bias;
+	const int f0 = int(gc.x), f1 = int(gc.y), f2 = int(gc.z);
+	const Vec3 p0 = v3(float(f0) * g.cell[0] + emin.x, float(f1) * g.cell[1] + emin.y, float(f2) * g.cell[2] + emin.z);
+	Vec3 alpha = v3((position.x - p0.x) / acell.x, (position.y - p0.y) / acell.y, (position.z - p0.z) / acell.z);
+	alpha = v3(fminf(fmaxf(alpha.x, 0.0f), 1.0f), fminf(fmaxf(alpha.y, 0.0f), 1.0f), fminf(fmaxf(alpha.z, 0.0f), 1.0f));
+	float cu, cv;
+	oct_uv(normal.x, normal.y, normal.z, cu, cv);
+	int cx0, cy0, cx1, cy1;
+	float cwx, cwy;
+	tile_coords<8>(cu, cv, cx0, cy0, cx1, cy1, cwx, cwy);
+	Vec3 fin = v3(0, 0, 0), fb = v3(0, 0, 0);
+	float tw = 0.0f, tfb = 0.0f;
+	for(int i = 0; i < 8; ++i) {
+		const int o0 = i & 1, o1 = (i >> 1) & 1, o2 = (i >> 2) & 1;
+		const int p0i = f0 + o0, p1i = f1 + o1, p2i = f2 + o2;
+		if(p0i > g.res[0] - 1 || p1i > g.res[1] - 1 || p2i > g.res[2] - 1) continue;
+		const uint32_t lin = uint32_t(p0i) + uint32_t(g.res[0]) * (uint32_t(p1i) + uint32_t(g.res[1]) * uint32_t(p2i));
+		if(states[lin] == 0u) continue;
+		const Vec3 pp = v3(float(p0i) * g.cell[0] + emin.x, float(p1i) * g.cell[1] + emin.y, float(p2i) * g.cell[2] + emin.z);
+		const Vec3 dtp = normalize3(pp - position);
+		const Vec3 bdir = pp - biased;
+		const Vec3 nb = normalize3(bdir);
+		float du, dv;
+		oct_uv(-nb.x, -nb.y, -nb.z, du, dv);
+		const float trx = o0 ? alpha.x : 1.0f - alpha.x;
+		const float try_ = o1 ? alpha.y : 1.0f - alpha.y;
+		const float trz = o2 ? alpha.z : 1.0f - alpha.z;
+		float weight = 1.0f;
+		const float bfw = fmaxf(0.0001f, (dot3(dtp, normal) + 1.0f) * 0.5f);
+		weight *= bfw * bfw + 0.2f;
+		float fbw = weight;
+		// Chebyshev visibility from the bilinear depth moments
+		int dx0, dy0, dx1, dy1;
+		float dwx, dwy;
+		tile_coords<16>(du, dv, dx0, dy0, dx1, dy1, dwx, dwy);
+		const float2* dt = dep + size_t(lin) * 256;
+		const float2 m00 = dt[dy0 * 16 + dx0], m10 = dt[dy0 * 16 + dx1], m01 = dt[dy1 * 16 + dx0], m11 = dt[dy1 * 16 + dx1];
+		const float mean = (m00.x * (1.0f - dwx) + m10.x * dwx) * (1.0f - dwy) + (m01.x * (1.0f - dwx) + m11.x * dwx) * dwy;
+		const float m2 = (m00.y * (1.0f - dwx) + m10.y * dwx) * (1.0f - dwy) + (m01.y * (1.0f - dwx) + m11.y * dwx) * dwy;
+		const float variance = fabsf(mean * mean - m2);
+		const float bdist = sqrtf(dot3(bdir, bdir));
+		const float dm = fmaxf(bdist - mean, 0.0001f);
+		float cheb = variance / (variance + dm * dm);
+		cheb = fmaxf(cheb * cheb * cheb, 0.0f);
+		weight *= (bdist <= mean) ? 1.0f : cheb;
+		weight = fmaxf(0.000001f, weight);
+		if(weight < 0.2f) weight *= weight * weight * (1.0f / (0.2f * 0.2f));
+		const float trw = trx * try_ * trz + 0.001f;
+		weight *= trw;
+		fbw *= trw;
+		const float4* ct = irr + size_t(lin) * 64;
+		const float4 c00 = ct[cy0 * 8 + cx0], c10 = ct[cy0 * 8 + cx1], c01 = ct[cy1 * 8 + cx0], c11 = ct[cy1 * 8 + cx1];
+		const float cr = (c00.x * (1.0f - cwx) + c10.x * cwx) * (1.0f - cwy) + (c01.x * (1.0f - cwx) + c11.x * cwx) * cwy;
+		const float cg = (c00.y * (1.0f - cwx) + c10.y * cwx) * (1.0f - cwy) + (c01.y * (1.0f - cwx) + c11.y * cwx) * cwy;
+		const float cb = (c00.z * (1.0f - cwx) + c10.z * cwx) * (1.0f - cwy) + (c01.z * (1.0f - cwx) + c11.z * cwx) * cwy;
+		const Vec3 col = v3(sqrtf(cr), sqrtf(cg), sqrtf(cb));
+		fin = fin + col * weight;
+		tw += weight;
+		fb = fb + col * fbw;
+		tfb += fbw;
+	}
+	if(tw > 1e-3f) fin = fin * (1.0f / tw);
+	if(tfb > 1e-3f) fb = fb * (1.0f / tfb);
+	fin = fin * fin;
+	fb = fb * fb;
+	const float k = 8.0f * fminf(fmaxf(tw, 0.0f), 1.0f / 8.0f);
+	return v3(fb.x * (1.0f - k) + fin.x * k, fb.y * (1.0f - k) + fin.y * k, fb.z * (1.0f - k) + fin.z * k);
+}
+
+// pbrMetallicRoughness (src/shaders/pbrMetallicRoughness.glsl:43-84)
+__device__ __forceinline__ Vec3 pbr_metallic_roughness(Vec3 n, Vec3 v, Vec3 light_color, Vec3 light_dir, Vec3 albedo,
+													   float metal, float rough) {
+	const float f0 = 0.04f;
+	const Vec3 diffuse = albedo * ((1.0f - f0) * (1.0f - metal));
+	const float alpha = rough * rough;
+	const Vec3 spec = v3(f0 * (1.0f - metal) + albedo.x * metal, f0 * (1.0f - metal) + albedo.y * metal,
+						 f0 * (1.0f - metal) + albedo.z * metal);
+	const float reflectance = fmaxf(fmaxf(spec.x, spec.y), spec.z);
+	const float r90 = fminf(fmaxf(reflectance * 25.0f, 0.0f), 1.0f);
+	const Vec3 l = normalize3(light_dir);
+	const Vec3 h = normalize3(l + v);
+	const float NdotL = fminf(fmaxf(dot3(n, l), 0.001f), 1.0f);
+	const float NdotV = fminf(fmaxf(fabsf(dot3(n, v)), 0.001f), 1.0f);
+	const float NdotH = fminf(fmaxf(dot3(n, h), 0.0f), 1.0f);
+	const float VdotH = fminf(fmaxf(dot3(v, h), 0.0f), 1.0f);
+	const float om = fminf(fmaxf(1.0f - VdotH, 0.0f), 1.0f);
+	const float fw = om * om * om * om * om;
+	const Vec3 F = v3(spec.x + (r90 - spec.x) * fw, spec.y + (r90 - spec.y) * fw, spec.z + (r90 - spec.z) * fw);
+	const float a2 = alpha * alpha;
+	const float attL = 2.0f * NdotL / (NdotL + sqrtf(a2 + (1.0f - a2) * (NdotL * NdotL)));
+	const float attV = 2.0f * NdotV / (NdotV + sqrtf(a2 + (1.0f - a2) * (NdotV * NdotV)));
+	const float G = attL * attV;
+	const float ah = NdotH * alpha;
+	const float k = alpha / ((1.0f - NdotH * NdotH) + ah * ah);
+	const float D = fminf(fmaxf(k * k * (1.0f / kPiF), 0.0f), 4.0f);
+	const float sp = G * D / (4.0f * NdotL * NdotV);
+	const float inv_pi = 1.0f / kPiF;
+	return v3(NdotL * light_color.x * ((1.0f - F.x) * diffuse.x * inv_pi + F.x * sp),
+			  NdotL * light_color.y * ((1.0f - F.y) * diffuse.y * inv_pi + F.y * sp),
+			  NdotL * light_color.z * ((1.0f - F.z) * diffuse.z * inv_pi + F.z * sp));
+}
+
+}  // namespace ddgi

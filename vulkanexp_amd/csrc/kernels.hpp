@@ -1,0 +1,133 @@
+// Kernel-argument structs and launcher declarations shared by the HIP translation units and the
+// C-ABI implementation.  Everything a kernel reads is passed by value or as raw device pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ddgi.h"
+#include "bvh.hpp"
+
+namespace ddgi {
+
+constexpr float kPiF = 3.1415926538f;  // src/shaders/common.glsl:4
+constexpr uint32_t kMaxRays = 512;
+constexpr uint32_t kIrrTexels = 36;    // 6x6 interior of the 8x8 tile
+constexpr uint32_t kDepTexels = 196;   // 14x14 interior of the 16x16 tile
+constexpr uint32_t kBlendTexels = kIrrTexels + kDepTexels;
+constexpr uint32_t kTraceBlock = 128;  // threads per traversal block
+
+// Grid constants derived on the host (FP32, same operations as the oracle's ray generation).
+struct GridConst {
+	float emin[3];
+	float cell[3];        // (max - min) / (res - 1)
+	float cell_len;       // length(cell)
+	float hysteresis;
+	float sharpness;
+	float shadow_bias;
+	float tmax_primary;   // length(max - min)
+	int32_t res[3];
+	uint32_t n;           // rays per probe
+	uint32_t probe_count;
+};
+
+struct Rot {
+	float r[9];  // row-major, rows X, Y, Z
+};
+
+struct LightConst {
+	float dir[3];    // toward the light
+	float color[3];  // radiance
+	float sky[3];    // constant miss radiance
+};
+
+// Per-instance shading record: normal matrix transpose(inverse(M)) row-major, custom index
+// (offset-table entry) and the global id of the instance's first triangle.
+struct InstanceDev {
+	float nm[9];
+	uint32_t custom_index;
+	uint32_t prim_base;
+	uint32_t mask;
+};
+
+struct SceneDev {
+	const float4* nodes;
+	const float4* tris;
+	const InstanceDev* inst;
+	const uint32_t* offsets;   // 3 per entry (material, vertex offset, index offset)
+	const uint32_t* indices;
+	const float4* vertices;    // 4 float4 per vertex (64-byte record)
+	const float* materials;    // 12 floats per material
+	uint32_t tri_count;
+	uint32_t node_count;
+	uint32_t instance_count;
+	uint32_t stack_depth;      // 32 or 64
+};
+
+// Internal hit record: (t, slot | back << 31, u, v); slot = triangle position in the BVH array,
+// 0xFFFFFFFF = miss (t = -1).
+struct TraceFrameArgs {
+	SceneDev scene;
+	GridConst grid;
+	LightConst light;
+	const uint32_t* indices;   // probe linear index per slot
+	const uint32_t* count;     // device count of slots
+	const float4* dirs;        // world ray directions of the frame
+	const uint16_t* perm;      // thread order -> ray index (directionally coherent)
+	const uint32_t* states;    // probe states (read by sampleProbes)
+	const float4* irr;         // exposed irradiance atlas (tiled)
+	const float2* dep;         // exposed depth atlas (tiled)
+	float4* hits;
+	float4* rays;              // (rgb, depth)
+	uint32_t max_slots;        // grid sized for this many slots
+	uint32_t* visit_counters;  // optional: [0] node fetches, [1] triangle fetches (atomics)
+};
+
+struct SelectState {
+	uint32_t last_offset;
+	uint32_t loop_index;
+	uint32_t count;
+	uint32_t pad;
+};
+
+struct BlendArgs {
+	GridConst grid;
+	const uint32_t* indices;
+	const uint32_t* count;
+	const float4* rays;         // slot-major (rgb, depth)
+	const float* wt;            // [n][232] weights, irradiance texels then depth texels
+	const float* inv_sum;       // [232]
+	const void* w_lo16;         // MFMA paths: weights split hi/lo in 16-bit types
+	const void* w_hi16;
+	uint32_t* states;
+	const float4* irr_prev;
+	const float2* dep_prev;
+	float4* irr_out;
+	float2* dep_out;
+	uint32_t max_slots;
+};
+
+// ---- launchers (return hipGetLastError of the launch) ----
+hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s);
+hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);
+hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s);
+hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
+							 bool any_hit, hipStream_t s);
+hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
+							  uint32_t* backface_counts, hipStream_t s);
+hipError_t launch_select(const uint32_t* states, uint32_t P, uint32_t cap, SelectState* st, uint32_t* indices,
+						 hipStream_t s);
+hipError_t launch_iota(uint32_t* indices, uint32_t P, SelectState* st, hipStream_t s);
+hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float4* irr, const float2* dep,
+						 const float* pos, const float* nrm, const float* tocam, uint64_t count, float* out,
+						 hipStream_t s);
+hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, float* wt, float* inv_sum, void* w_hi16,
+								void* w_lo16, uint32_t mode, hipStream_t s);
+hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s);
+
+// blend modes
+constexpr uint32_t kBlendFP32 = 0;
+constexpr uint32_t kBlendBF16 = 1;
+constexpr uint32_t kBlendFP16 = 2;
+
+}  // namespace ddgi

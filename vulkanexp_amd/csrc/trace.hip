@@ -1,0 +1,376 @@
+// Probe ray tracing on gfx950: ray generation (src/shaders/traceProbes.rgen:29-49), BVH traversal
+// replacing the RT cores, closest-hit shading (src/shaders/closesthit.glsl:133-289, NO_REFLECTION),
+// shadow rays (closesthit.glsl:252-281, shadow.rmiss), the constant-sky miss, probe-state init
+// (src/shaders/probesInit.rgen:31-64), device-side probe selection
+// (src/IrradianceProbes.cpp:396-424) and standalone sampleProbes queries.
+//
+// Built with -ffp-contract=off (see device_common.hpp).
+#include "device_common.hpp"
+
+namespace ddgi {
+
+namespace {
+
+__device__ __forceinline__ void probe_origin(const GridConst& g, uint32_t linear, float o[3]) {
+	const uint32_t rx = uint32_t(g.res[0]), ry = uint32_t(g.res[1]);
+	const uint32_t gx = linear % rx, gy = (linear % (rx * ry)) / rx, gz = linear / (rx * ry);
+	o[0] = float(gx) * g.cell[0] + g.emin[0];
+	o[1] = float(gy) * g.cell[1] + g.emin[1];
+	o[2] = float(gz) * g.cell[2] + g.emin[2];
+}
+
+__device__ __forceinline__ float4 pack_hit(const HitResult& h) {
+	if(h.slot == 0xFFFFFFFFu) return make_float4(-1.0f, __uint_as_float(0xFFFFFFFFu), 0.0f, 0.0f);
+	const bool back = (h.det < 0.0f) != ((h.meta & 0x100u) != 0u);
+	return make_float4(h.t, __uint_as_float(h.slot | (back ? 0x80000000u : 0u)), h.V / h.det, h.W / h.det);
+}
+
+// ---- per-frame world directions: dirs[i] = R * fib[i] (FP32, no FMA) ----
+__global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= count) return;
+	const float4 f = fib[i];
+	float d[3];
+	for(int k = 0; k < 3; ++k) d[k] = R.r[k * 3 + 0] * f.x + R.r[k * 3 + 1] * f.y + R.r[k * 3 + 2] * f.z;
+	dirs[i] = make_float4(d[0], d[1], d[2], 0.0f);
+}
+
+// ---- primary probe rays: closest hit, mask STATIC|DYNAMIC, tmin 0.01, tmax = |max - min| ----
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint32_t n = a.grid.n;
+	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+	const uint32_t slot = gtid / n, j = gtid - slot * n;
+	if(slot >= *a.count) return;
+	const uint32_t i = a.perm[j];
+	float o[3];
+	probe_origin(a.grid, a.indices[slot], o);
+	const float4 d = a.dirs[i];
+	RayPrep r;
+	ray_prepare(o[0], o[1], o[2], d.x, d.y, d.z, r);
+	uint32_t visits[2];
+	const HitResult h = traverse<false, STACK>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
+											   DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack + threadIdx.x, kTraceBlock,
+											   a.visit_counters ? visits : nullptr);
+	a.hits[size_t(slot) * n + i] = pack_hit(h);
+	if(a.visit_counters) {
+		atomicAdd(&a.visit_counters[0], visits[0]);
+		atomicAdd(&a.visit_counters[1], visits[1]);
+	}
+}
+
+// ---- shading of the primary hits + shadow ray (one thread per ray) ----
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint32_t n = a.grid.n;
+	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+	const uint32_t slot = gtid / n, j = gtid - slot * n;
+	if(slot >= *a.count) return;
+	const uint32_t i = a.perm[j];
+	const size_t ri = size_t(slot) * n + i;
+	const float4 hit = a.hits[ri];
+	const uint32_t code = __float_as_uint(hit.y);
+	if(code == 0xFFFFFFFFu) {  // miss.rmiss: sky radiance, depth -1
+		a.rays[ri] = make_float4(a.light.sky[0], a.light.sky[1], a.light.sky[2], -1.0f);
+		return;
+	}
+	const float t = hit.x;
+	if(code & 0x80000000u) {  // back face: closesthit.glsl:137-141
+		a.rays[ri] = make_float4(0.0f, 0.0f, 0.0f, t * 0.8f);
+		return;
+	}
+	const uint32_t tslot = code & 0x7FFFFFFFu;
+	float o[3];
+	probe_origin(a.grid, a.indices[slot], o);
+	const float4 d4 = a.dirs[i];
+	const Vec3 d = v3(d4.x, d4.y, d4.z);
+	const float b1 = hit.z, b2 = hit.w, b0 = 1.0f - b1 - b2;
+	const Vec3 position = v3(d.x * t + o[0], d.y * t + o[1], d.z * t + o[2]);
+	// fetch geometry and material records (closesthit.glsl:147-159)
+	const float4* tr = a.scene.tris + size_t(tslot) * 3;
+	const uint32_t gid = __float_as_uint(tr[0].w), inst = __float_as_uint(tr[1].w);
+	const InstanceDev I = a.scene.inst[inst];
+	const uint32_t prim = gid - I.prim_base;
+	const uint32_t* oe = a.scene.offsets + size_t(I.custom_index) * 3;
+	const uint32_t mat = oe[0], voff = oe[1], ib = oe[2] + 3u * prim;
+	Vec3 nrm[3];
+	for(int k = 0; k < 3; ++k) {
+		const float4* vr = a.scene.vertices + size_t(voff + a.scene.indices[ib + k]) * 4;
+		const float4 q1 = vr[1], q2 = vr[2];
+		nrm[k] = v3(q1.z, q1.w, q2.x);
+	}
+	const Vec3 tsn = normalize3(nrm[0] * b0 + nrm[1] * b1 + nrm[2] * b2);
+	const Vec3 normal = normalize3(v3(I.nm[0] * tsn.x + I.nm[1] * tsn.y + I.nm[2] * tsn.z,
+									  I.nm[3] * tsn.x + I.nm[4] * tsn.y + I.nm[5] * tsn.z,
+									  I.nm[6] * tsn.x + I.nm[7] * tsn.y + I.nm[8] * tsn.z));
+	const float* m = a.scene.materials + size_t(mat) * 12;
+	const float metal = m[0], rough = m[1];
+	const Vec3 albedo = v3(m[2], m[3], m[4]);
+	Vec3 color = v3(m[5], m[6], m[7]);
+	const float f0 = 0.04f;
+	const Vec3 diffuse = albedo * ((1.0f - f0) * (1.0f - metal));
+	const Vec3 spec = v3(f0 * (1.0f - metal) + albedo.x * metal, f0 * (1.0f - metal) + albedo.y * metal,
+						 f0 * (1.0f - metal) + albedo.z * metal);
+	const float ndd = dot3(normal, d);
+	const Vec3 refl = d - normal * (2.0f * ndd);
+	const Vec3 to_cam = v3(-d.x, -d.y, -d.z);
+	const Vec3 reflection = sample_probes(a.grid, a.states, a.irr, a.dep, position, refl, to_cam);
+	color = color + spec * reflection;
+	const Vec3 indirect = sample_probes(a.grid, a.states, a.irr, a.dep, position, normal, to_cam);
+	color = color + indirect * diffuse;
+	// shadow ray toward the light: any hit, mask 0xFF, tmin 0.1, tmax 10000
+	RayPrep sr;
+	ray_prepare(position.x, position.y, position.z, a.light.dir[0], a.light.dir[1], a.light.dir[2], sr);
+	const HitResult sh =
+		traverse<true, STACK>(a.scene.nodes, a.scene.tris, sr, 0.1f, 10000.0f, 0xFFu, stack + threadIdx.x, kTraceBlock, nullptr);
+	if(sh.slot == 0xFFFFFFFFu) {
+		const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
+		color = color + pbr_metallic_roughness(normal, normalize3(to_cam), v3(a.light.color[0], a.light.color[1], a.light.color[2]),
+											   L, albedo, metal, rough);
+		if(L.y < 0.0f) {
+			const float fade = 1.0f - fminf(fmaxf(-L.y, 0.0f), 0.1f) / 0.1f;
+			color = color * fade;
+		}
+	}
+	a.rays[ri] = make_float4(color.x, color.y, color.z, t);
+}
+
+// ---- arbitrary ray queries (public ddgi_hit records) ----
+template <int STACK, bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneDev sc, const ddgi_ray* __restrict__ rays,
+															ddgi_hit* __restrict__ hits, uint64_t count, uint32_t mask) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint64_t id = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+	if(id >= count) return;
+	const ddgi_ray q = rays[id];
+	RayPrep r;
+	ray_prepare(q.origin[0], q.origin[1], q.origin[2], q.direction[0], q.direction[1], q.direction[2], r);
+	const HitResult h = traverse<ANY, STACK>(sc.nodes, sc.tris, r, q.tmin, q.tmax, mask, stack + threadIdx.x, kTraceBlock, nullptr);
+	ddgi_hit out;
+	if(h.slot == 0xFFFFFFFFu) {
+		out.t = -1.0f;
+		out.u = out.v = 0.0f;
+		out.instance = out.primitive = 0xFFFFFFFFu;
+		out.flags = 0;
+	} else {
+		const float4 p = pack_hit(h);
+		const uint32_t inst = __float_as_uint(sc.tris[size_t(h.slot) * 3 + 1].w);
+		out.t = h.t;
+		out.u = p.z;
+		out.v = p.w;
+		out.instance = inst;
+		out.primitive = h.gid - sc.inst[inst].prim_base;
+		out.flags = (__float_as_uint(p.y) & 0x80000000u) ? 1u : 0u;
+	}
+	hits[id] = out;
+}
+
+// ---- probe-state init: one block per probe, 512 rays (probesInit.rgen:31-64) ----
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_init_probes(SceneDev sc, GridConst g, const float4* __restrict__ dirs512,
+															 uint32_t* __restrict__ states, uint32_t* __restrict__ backface_out) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	__shared__ uint32_t s_back, s_affect;
+	const uint32_t probe = blockIdx.x;
+	if(threadIdx.x == 0) {
+		s_back = 0;
+		s_affect = 0;
+	}
+	__syncthreads();
+	float o[3];
+	probe_origin(g, probe, o);
+	const float max_distance = g.cell_len;
+	const float tmax = 1.5f * max_distance;
+	uint32_t back = 0, affect = 0;
+	for(uint32_t i = threadIdx.x; i < 512u; i += kTraceBlock) {
+		const float4 d = dirs512[i];
+		RayPrep r;
+		ray_prepare(o[0], o[1], o[2], d.x, d.y, d.z, r);
+		const HitResult h = traverse<false, STACK>(sc.nodes, sc.tris, r, 0.01f, tmax, DDGI_MASK_STATIC, stack + threadIdx.x,
+												   kTraceBlock, nullptr);
+		const float depth = (h.slot == 0xFFFFFFFFu) ? 3.402823466e+38f : h.t;
+		if(depth < max_distance) {
+			const bool is_back = (h.det < 0.0f) != ((h.meta & 0x100u) != 0u);
+			if(is_back) ++back;
+			const float px = o[0] + depth * d.x, py = o[1] + depth * d.y, pz = o[2] + depth * d.z;
+			if(fabsf(px - o[0]) < g.cell[0] && fabsf(py - o[1]) < g.cell[1] && fabsf(pz - o[2]) < g.cell[2]) affect = 1;
+		}
+	}
+	atomicAdd(&s_back, back);
+	if(affect) atomicOr(&s_affect, 1u);
+	__syncthreads();
+	if(threadIdx.x == 0) {
+		uint32_t st;
+		if(float(s_back) > 0.5f * float(g.n)) st = 0;
+		else st = s_affect ? 1u : 8u;
+		states[probe] = st;
+		if(backface_out) backface_out[probe] = s_back;
+	}
+}
+
+// ---- device-side selectProbesToUpdate: one block of 1024 threads scans the round-robin order ----
+constexpr int kSelectBlock = 1024;
+
+__global__ __launch_bounds__(kSelectBlock) void k_select(const uint32_t* __restrict__ states, uint32_t P, uint32_t cap,
+														 SelectState* st, uint32_t* __restrict__ out) {
+	__shared__ uint32_t wave_sums[kSelectBlock / 64];
+	__shared__ uint32_t s_base, s_stop_q, s_done;
+	const uint32_t off = st->last_offset, loop = st->loop_index;
+	if(threadIdx.x == 0) {
+		s_base = 0;
+		s_stop_q = 0xFFFFFFFFu;
+		s_done = 0;
+	}
+	__syncthreads();
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	for(uint32_t chunk = 0; chunk < P; chunk += kSelectBlock) {
+		const uint32_t q = chunk + threadIdx.x;
+		bool pred = false;
+		uint32_t idx = 0;
+		if(q < P) {
+			idx = off + q;
+			uint32_t L = loop;
+			if(idx >= P) {
+				idx -= P;
+				L += 1u;
+			}
+			const uint32_t s = states[idx];
+			pred = (s != 0u) && ((idx + L) % s) == 0u;
+		}
+		// block-wide exclusive scan of pred
+		const uint64_t ballot = __ballot(pred);
+		const uint32_t in_wave = __popcll(ballot & ((uint64_t(1) << lane) - 1u));
+		if(lane == 0) wave_sums[wave] = __popcll(ballot);
+		__syncthreads();
+		uint32_t wave_off = 0;
+		for(uint32_t w = 0; w < wave; ++w) wave_off += wave_sums[w];
+		const uint32_t pos = s_base + wave_off + in_wave;
+		if(pred && (cap == 0u || pos < cap)) {
+			out[pos] = idx;
+			if(cap != 0u && pos == cap - 1u) s_stop_q = q;  // exactly one thread
+		}
+		__syncthreads();
+		if(threadIdx.x == 0) {
+			uint32_t tot = 0;
+			for(uint32_t w = 0; w < kSelectBlock / 64; ++w) tot += wave_sums[w];
+			s_base += tot;
+			if(cap != 0u && s_base >= cap) s_done = 1;
+		}
+		__syncthreads();
+		if(s_done) break;
+	}
+	if(threadIdx.x == 0) {
+		uint32_t cnt = s_base;
+		if(cap != 0u && cnt > cap) cnt = cap;
+		st->count = cnt;
+		if(s_stop_q != 0xFFFFFFFFu) {
+			const uint32_t next = off + s_stop_q + 1u;
+			st->last_offset = next >= P ? next - P : next;
+			st->loop_index = loop + (next >= P ? 1u : 0u);
+		} else {
+			st->last_offset = off;
+			st->loop_index = loop + 1u;
+		}
+	}
+}
+
+__global__ void k_set_count(SelectState* st, uint32_t c) {
+	if(blockIdx.x == 0 && threadIdx.x == 0) st->count = c;
+}
+
+__global__ void k_iota(uint32_t* __restrict__ out, uint32_t P, SelectState* st) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < P) out[i] = i;
+	if(i == 0) st->count = P;
+}
+
+// ---- sampleProbes at arbitrary points ----
+__global__ void k_sample(GridConst g, const uint32_t* __restrict__ states, const float4* __restrict__ irr,
+						 const float2* __restrict__ dep, const float* __restrict__ pos, const float* __restrict__ nrm,
+						 const float* __restrict__ tocam, uint64_t count, float* __restrict__ out) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(i >= count) return;
+	const Vec3 r = sample_probes(g, states, irr, dep, v3(pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]),
+								 v3(nrm[i * 3], nrm[i * 3 + 1], nrm[i * 3 + 2]), v3(tocam[i * 3], tocam[i * 3 + 1], tocam[i * 3 + 2]));
+	out[i * 3] = r.x;
+	out[i * 3 + 1] = r.y;
+	out[i * 3 + 2] = r.z;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s) {
+	hipLaunchKernelGGL(k_frame_dirs, dim3(blocks_for(count, 256)), dim3(256), 0, s, R, fib, dirs, count);
+	return hipGetLastError();
+}
+
+hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
+	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
+	if(threads == 0) return hipSuccess;
+	const dim3 grid(blocks_for(threads, kTraceBlock));
+	if(a.scene.stack_depth <= 32) hipLaunchKernelGGL(k_trace_primary<32>, grid, dim3(kTraceBlock), 0, s, a);
+	else hipLaunchKernelGGL(k_trace_primary<64>, grid, dim3(kTraceBlock), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
+	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
+	if(threads == 0) return hipSuccess;
+	const dim3 grid(blocks_for(threads, kTraceBlock));
+	if(a.scene.stack_depth <= 32) hipLaunchKernelGGL(k_shade<32>, grid, dim3(kTraceBlock), 0, s, a);
+	else hipLaunchKernelGGL(k_shade<64>, grid, dim3(kTraceBlock), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
+							 bool any_hit, hipStream_t s) {
+	if(count == 0) return hipSuccess;
+	const dim3 grid(blocks_for(count, kTraceBlock));
+	if(sc.stack_depth <= 32) {
+		if(any_hit) hipLaunchKernelGGL((k_trace_rays<32, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+		else hipLaunchKernelGGL((k_trace_rays<32, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+	} else {
+		if(any_hit) hipLaunchKernelGGL((k_trace_rays<64, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+		else hipLaunchKernelGGL((k_trace_rays<64, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
+							  uint32_t* backface_counts, hipStream_t s) {
+	if(g.probe_count == 0) return hipSuccess;
+	if(sc.stack_depth <= 32)
+		hipLaunchKernelGGL(k_init_probes<32>, dim3(g.probe_count), dim3(kTraceBlock), 0, s, sc, g, dirs512, states, backface_counts);
+	else
+		hipLaunchKernelGGL(k_init_probes<64>, dim3(g.probe_count), dim3(kTraceBlock), 0, s, sc, g, dirs512, states, backface_counts);
+	return hipGetLastError();
+}
+
+hipError_t launch_select(const uint32_t* states, uint32_t P, uint32_t cap, SelectState* st, uint32_t* indices,
+						 hipStream_t s) {
+	hipLaunchKernelGGL(k_select, dim3(1), dim3(kSelectBlock), 0, s, states, P, cap, st, indices);
+	return hipGetLastError();
+}
+
+hipError_t launch_set_count(SelectState* st, uint32_t count, hipStream_t s) {
+	hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, s, st, count);
+	return hipGetLastError();
+}
+
+hipError_t launch_iota(uint32_t* indices, uint32_t P, SelectState* st, hipStream_t s) {
+	hipLaunchKernelGGL(k_iota, dim3(blocks_for(P, 256)), dim3(256), 0, s, indices, P, st);
+	return hipGetLastError();
+}
+
+hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float4* irr, const float2* dep, const float* pos,
+						 const float* nrm, const float* tocam, uint64_t count, float* out, hipStream_t s) {
+	if(count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_sample, dim3(blocks_for(count, 256)), dim3(256), 0, s, g, states, irr, dep, pos, nrm, tocam, count, out);
+	return hipGetLastError();
+}
+
+}  // namespace ddgi
