@@ -121,6 +121,7 @@ typedef struct ddgi_options {
 #define DDGI_UPDATE_BLEND_FP16 0x4u       /* blend on FP16 MFMA, FP32 accumulation */
 #define DDGI_UPDATE_HYSTERESIS_RAMP 0x8u  /* reference hysteresis ramp (src/IrradianceProbes.cpp:462-476) */
 #define DDGI_UPDATE_NO_TIMING 0x10u       /* skip per-phase HIP events */
+#define DDGI_UPDATE_COUNT_VISITS 0x20u    /* count BVH node/triangle fetches (slower; statistics) */
 
 /* Atlas kinds / layouts */
 #define DDGI_ATLAS_IRRADIANCE 0u /* float4 texels (rgb, 1) */
@@ -162,7 +163,14 @@ typedef struct ddgi_timings {
 	float total_ms;
 	uint32_t probes_updated;
 	uint32_t rays_traced;
+	float primary_ms; /* closest-hit traversal of the probe rays (part of trace_ms) */
+	float shade_ms;   /* closest-hit shading + shadow rays (part of trace_ms) */
 } ddgi_timings;
+
+/* Traversal statistics of the last update run with DDGI_UPDATE_COUNT_VISITS:
+ * [0] primary node fetches, [1] primary triangle fetches, [2] shadow node fetches,
+ * [3] shadow triangle fetches, [4] shadow rays.  Node = 64 bytes, triangle = 48 bytes. */
+#define DDGI_TRACE_STATS 5
 
 /* Raw device view of a handle's buffers for zero-copy interop (torch). Valid until the next
  * ddgi_update / ddgi_set_scene / ddgi_destroy on the handle. */
@@ -236,6 +244,7 @@ ddgi_status ddgi_set_atlas(void* handle, uint32_t kind, uint32_t layout, const v
 ddgi_status ddgi_get_states(void* handle, uint32_t* host_dst, size_t count);
 ddgi_status ddgi_set_states(void* handle, const uint32_t* host_src, size_t count);
 ddgi_status ddgi_get_timings(void* handle, ddgi_timings* out);
+ddgi_status ddgi_get_trace_stats(void* handle, uint64_t out[DDGI_TRACE_STATS]);
 ddgi_status ddgi_get_device_view(void* handle, ddgi_device_view* out);
 /* Save/restore atlases + states + schedule counters + hysteresis.  Call with dst NULL to query
  * the size. */

@@ -22,6 +22,8 @@ UPDATE_BLEND_BF16 = 0x2
 UPDATE_BLEND_FP16 = 0x4
 UPDATE_HYSTERESIS_RAMP = 0x8
 UPDATE_NO_TIMING = 0x10
+UPDATE_COUNT_VISITS = 0x20
+NODE_BYTES, TRIANGLE_BYTES = 64, 48
 ATLAS_IRRADIANCE, ATLAS_DEPTH = 0, 1
 LAYOUT_TILED, LAYOUT_IMAGE = 0, 1
 TRACE_ANY_HIT = 0x1
@@ -36,7 +38,8 @@ class DdgiError(RuntimeError):
 class Timings(C.Structure):
     _fields_ = [("select_ms", C.c_float), ("trace_ms", C.c_float), ("blend_ms", C.c_float),
                 ("border_ms", C.c_float), ("copy_ms", C.c_float), ("total_ms", C.c_float),
-                ("probes_updated", C.c_uint32), ("rays_traced", C.c_uint32)]
+                ("probes_updated", C.c_uint32), ("rays_traced", C.c_uint32),
+                ("primary_ms", C.c_float), ("shade_ms", C.c_float)]
 
 
 class DeviceView(C.Structure):
@@ -74,6 +77,7 @@ _SIGNATURES = {
     "ddgi_get_states": (C.c_int, [_P, _P, _SZ]),
     "ddgi_set_states": (C.c_int, [_P, _P, _SZ]),
     "ddgi_get_timings": (C.c_int, [_P, _P]),
+    "ddgi_get_trace_stats": (C.c_int, [_P, _P]),
     "ddgi_get_device_view": (C.c_int, [_P, _P]),
     "ddgi_save_state": (C.c_int, [_P, _P, C.POINTER(C.c_size_t)]),
     "ddgi_load_state": (C.c_int, [_P, _P, _SZ]),

@@ -63,7 +63,7 @@ struct Handle {
 	float* inv_sum = nullptr;
 	void* w_hi = nullptr;
 	void* w_lo = nullptr;
-	uint32_t* visit = nullptr;  // traversal counters (optional statistics)
+	unsigned long long* visit = nullptr;  // traversal counters (statistics runs)
 	// scene
 	std::vector<DeviceBuffer> scene_bufs;
 	SceneDev sd{};
@@ -72,7 +72,7 @@ struct Handle {
 	size_t bvh_bytes = 0;
 	LightConst light{};
 	// timing
-	hipEvent_t ev[5] = {};
+	hipEvent_t ev[6] = {};
 	bool timing_pending = false;
 	ddgi_timings last{};
 	uint32_t last_count = 0xFFFFFFFFu;
@@ -407,7 +407,7 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->inv_sum, kBlendTexels * sizeof(float));
 	if(e == hipSuccess) e = dalloc(&h->w_hi, size_t(kMaxRays) * kBlendTexels * 2);
 	if(e == hipSuccess) e = dalloc(&h->w_lo, size_t(kMaxRays) * kBlendTexels * 2);
-	if(e == hipSuccess) e = dalloc(&h->visit, 16);
+	if(e == hipSuccess) e = dalloc(&h->visit, 8 * sizeof(unsigned long long));
 	if(e != hipSuccess) {
 		h->err = std::string("allocation failed: ") + hipGetErrorString(e);
 		return cleanup_fail(e == hipErrorOutOfMemory ? DDGI_ERROR_OUT_OF_MEMORY : DDGI_ERROR_HIP);
@@ -419,12 +419,12 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = hipMemset(h->dep[1], 0, P * 256 * sizeof(float2));
 	if(e == hipSuccess) e = hipMemset(h->sel, 0, sizeof(SelectState));
 	if(e == hipSuccess) e = hipMemset(h->rays, 0, P * R * sizeof(float4));
-	if(e == hipSuccess) e = hipMemset(h->visit, 0, 16);
+	if(e == hipSuccess) e = hipMemset(h->visit, 0, 8 * sizeof(unsigned long long));
 	if(e == hipSuccess) {
 		std::vector<uint32_t> ones(P, 1u);
 		e = hipMemcpy(h->states, ones.data(), P * 4, hipMemcpyHostToDevice);
 	}
-	for(int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
+	for(int i = 0; i < 6 && e == hipSuccess; ++i) e = hipEventCreate(&h->ev[i]);
 	if(e != hipSuccess) {
 		h->err = std::string("initialisation failed: ") + hipGetErrorString(e);
 		return cleanup_fail(DDGI_ERROR_HIP);
@@ -665,13 +665,18 @@ ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags
 	a.rays = h->rays;
 	a.max_slots = max_slots;
 	a.visit_counters = nullptr;
+	if(flags & DDGI_UPDATE_COUNT_VISITS) {
+		HIP_TRY(h, hipMemsetAsync(h->visit, 0, 8 * sizeof(unsigned long long), s));
+		a.visit_counters = h->visit;
+	}
 	HIP_TRY(h, launch_trace_primary(a, s));
-	HIP_TRY(h, launch_shade(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
+	HIP_TRY(h, launch_shade(a, s));
+	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
 	if((st = enqueue_blend_and_publish(h, R, h->rays, max_slots, all, flags, s)) != DDGI_OK) return st;
 	if(timing) {
-		HIP_TRY(h, hipEventRecord(h->ev[3], s));
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));
+		HIP_TRY(h, hipEventRecord(h->ev[5], s));
 		h->timing_pending = true;
 	}
 	h->last_count = all ? h->P : 0xFFFFFFFFu;
@@ -693,11 +698,8 @@ ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_in
 	if(st != DDGI_OK || (st = ensure_fib(h)) != DDGI_OK) return st;
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	const bool timing = !(flags & DDGI_UPDATE_NO_TIMING);
-	if(timing) {
-		HIP_TRY(h, hipEventRecord(h->ev[0], s));
-		HIP_TRY(h, hipEventRecord(h->ev[1], s));
-		HIP_TRY(h, hipEventRecord(h->ev[2], s));
-	}
+	if(timing)
+		for(int k = 0; k < 4; ++k) HIP_TRY(h, hipEventRecord(h->ev[k], s));
 	HIP_TRY(h, hipMemcpyAsync(h->indices, probe_indices, size_t(count) * 4, hipMemcpyDeviceToDevice, s));
 	HIP_TRY(h, launch_set_count(h->sel, count, s));
 	h->indices_are_iota = false;
@@ -705,8 +707,8 @@ ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_in
 	if((st = enqueue_blend_and_publish(h, R, static_cast<const float4*>(rays), count, false, flags, s)) != DDGI_OK)
 		return st;
 	if(timing) {
-		HIP_TRY(h, hipEventRecord(h->ev[3], s));
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));
+		HIP_TRY(h, hipEventRecord(h->ev[5], s));
 		h->timing_pending = true;
 	}
 	h->last_count = count;
@@ -814,15 +816,17 @@ ddgi_status ddgi_get_timings(void* handle, ddgi_timings* out) {
 	ddgi_status st = set_device(h);
 	if(st != DDGI_OK) return st;
 	if(h->timing_pending) {
-		HIP_TRY(h, hipEventSynchronize(h->ev[4]));
-		float ms[4] = {};
-		for(int i = 0; i < 4; ++i) HIP_TRY(h, hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+		HIP_TRY(h, hipEventSynchronize(h->ev[5]));
+		float ms[5] = {};
+		for(int i = 0; i < 5; ++i) HIP_TRY(h, hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
 		h->last.select_ms = ms[0];
-		h->last.trace_ms = ms[1];
-		h->last.blend_ms = ms[2];
+		h->last.primary_ms = ms[1];
+		h->last.shade_ms = ms[2];
+		h->last.trace_ms = ms[1] + ms[2];
+		h->last.blend_ms = ms[3];
 		h->last.border_ms = 0.0f;
 		h->last.copy_ms = 0.0f;
-		HIP_TRY(h, hipEventElapsedTime(&h->last.total_ms, h->ev[0], h->ev[4]));
+		HIP_TRY(h, hipEventElapsedTime(&h->last.total_ms, h->ev[0], h->ev[5]));
 		uint32_t cnt = h->last_count;
 		if(cnt == 0xFFFFFFFFu) {
 			SelectState ss;
@@ -834,6 +838,18 @@ ddgi_status ddgi_get_timings(void* handle, ddgi_timings* out) {
 		h->timing_pending = false;
 	}
 	*out = h->last;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_trace_stats(void* handle, uint64_t out[DDGI_TRACE_STATS]) {
+	if(!handle || !out) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
+	Handle* h = H(handle);
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	HIP_TRY(h, hipStreamSynchronize(h->last_stream));
+	unsigned long long v[8];
+	HIP_TRY(h, hipMemcpy(v, h->visit, sizeof(v), hipMemcpyDeviceToHost));
+	for(int i = 0; i < DDGI_TRACE_STATS; ++i) out[i] = v[i];
 	return DDGI_OK;
 }
 

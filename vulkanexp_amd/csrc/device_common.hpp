@@ -107,11 +107,12 @@ struct HitResult {
 
 // Closest-hit (ANY = false; ties -> lowest global id) or any-hit traversal of the BVH, with a
 // per-thread stack in LDS laid out [entry][thread] (conflict-free).  `stack` points at this
-// thread's column; `stride` is the block size.
-template <bool ANY, int STACK>
+// thread's column; `stride` is the block size.  With COUNT, n_nodes / n_tris receive the number
+// of 64-byte node and 48-byte triangle fetches (the algorithmic bytes of the traversal).
+template <bool ANY, int STACK, bool COUNT = false>
 __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
 											  const RayPrep& r, float tmin, float tmax, uint32_t mask,
-											  uint32_t* stack, int stride, uint32_t* visits) {
+											  uint32_t* stack, int stride, uint32_t& n_nodes_out, uint32_t& n_tris_out) {
 	HitResult best;
 	best.t = tmax;
 	best.slot = 0xFFFFFFFFu;
@@ -173,9 +174,9 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 		node = stack[sp * stride];
 	}
 done:
-	if(visits) {
-		visits[0] = n_nodes;
-		visits[1] = n_tris;
+	if(COUNT) {
+		n_nodes_out = n_nodes;
+		n_tris_out = n_tris;
 	}
 	return best;
 }

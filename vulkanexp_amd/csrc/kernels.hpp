@@ -80,7 +80,9 @@ struct TraceFrameArgs {
 	float4* hits;
 	float4* rays;              // (rgb, depth)
 	uint32_t max_slots;        // grid sized for this many slots
-	uint32_t* visit_counters;  // optional: [0] node fetches, [1] triangle fetches (atomics)
+	// optional (statistics runs only): [0]/[1] primary node / triangle fetches, [2]/[3] shadow
+	// node / triangle fetches, [4] shadow rays
+	unsigned long long* visit_counters;
 };
 
 struct SelectState {

@@ -36,7 +36,7 @@ __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __re
 }
 
 // ---- primary probe rays: closest hit, mask STATIC|DYNAMIC, tmin 0.01, tmax = |max - min| ----
-template <int STACK>
+template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
 	const uint32_t n = a.grid.n;
@@ -49,19 +49,19 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a)
 	const float4 d = a.dirs[i];
 	RayPrep r;
 	ray_prepare(o[0], o[1], o[2], d.x, d.y, d.z, r);
-	uint32_t visits[2];
-	const HitResult h = traverse<false, STACK>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
-											   DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack + threadIdx.x, kTraceBlock,
-											   a.visit_counters ? visits : nullptr);
+	uint32_t nn = 0, nt = 0;
+	const HitResult h = traverse<false, STACK, COUNT>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
+													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack + threadIdx.x,
+													  kTraceBlock, nn, nt);
 	a.hits[size_t(slot) * n + i] = pack_hit(h);
-	if(a.visit_counters) {
-		atomicAdd(&a.visit_counters[0], visits[0]);
-		atomicAdd(&a.visit_counters[1], visits[1]);
+	if(COUNT) {
+		atomicAdd(&a.visit_counters[0], (unsigned long long)nn);
+		atomicAdd(&a.visit_counters[1], (unsigned long long)nt);
 	}
 }
 
 // ---- shading of the primary hits + shadow ray (one thread per ray) ----
-template <int STACK>
+template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
 	const uint32_t n = a.grid.n;
@@ -123,8 +123,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 	// shadow ray toward the light: any hit, mask 0xFF, tmin 0.1, tmax 10000
 	RayPrep sr;
 	ray_prepare(position.x, position.y, position.z, a.light.dir[0], a.light.dir[1], a.light.dir[2], sr);
-	const HitResult sh =
-		traverse<true, STACK>(a.scene.nodes, a.scene.tris, sr, 0.1f, 10000.0f, 0xFFu, stack + threadIdx.x, kTraceBlock, nullptr);
+	uint32_t nn = 0, nt = 0;
+	const HitResult sh = traverse<true, STACK, COUNT>(a.scene.nodes, a.scene.tris, sr, 0.1f, 10000.0f, 0xFFu,
+													  stack + threadIdx.x, kTraceBlock, nn, nt);
+	if(COUNT) {
+		atomicAdd(&a.visit_counters[2], (unsigned long long)nn);
+		atomicAdd(&a.visit_counters[3], (unsigned long long)nt);
+		atomicAdd(&a.visit_counters[4], 1ull);
+	}
 	if(sh.slot == 0xFFFFFFFFu) {
 		const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
 		color = color + pbr_metallic_roughness(normal, normalize3(to_cam), v3(a.light.color[0], a.light.color[1], a.light.color[2]),
@@ -147,7 +153,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneDev sc, const d
 	const ddgi_ray q = rays[id];
 	RayPrep r;
 	ray_prepare(q.origin[0], q.origin[1], q.origin[2], q.direction[0], q.direction[1], q.direction[2], r);
-	const HitResult h = traverse<ANY, STACK>(sc.nodes, sc.tris, r, q.tmin, q.tmax, mask, stack + threadIdx.x, kTraceBlock, nullptr);
+	uint32_t nn, nt;
+	const HitResult h = traverse<ANY, STACK>(sc.nodes, sc.tris, r, q.tmin, q.tmax, mask, stack + threadIdx.x, kTraceBlock, nn, nt);
 	ddgi_hit out;
 	if(h.slot == 0xFFFFFFFFu) {
 		out.t = -1.0f;
@@ -188,8 +195,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_init_probes(SceneDev sc, GridCo
 		const float4 d = dirs512[i];
 		RayPrep r;
 		ray_prepare(o[0], o[1], o[2], d.x, d.y, d.z, r);
+		uint32_t nn, nt;
 		const HitResult h = traverse<false, STACK>(sc.nodes, sc.tris, r, 0.01f, tmax, DDGI_MASK_STATIC, stack + threadIdx.x,
-												   kTraceBlock, nullptr);
+												   kTraceBlock, nn, nt);
 		const float depth = (h.slot == 0xFFFFFFFFu) ? 3.402823466e+38f : h.t;
 		if(depth < max_distance) {
 			const bool is_back = (h.det < 0.0f) != ((h.meta & 0x100u) != 0u);
@@ -312,8 +320,14 @@ hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
 	if(threads == 0) return hipSuccess;
 	const dim3 grid(blocks_for(threads, kTraceBlock));
-	if(a.scene.stack_depth <= 32) hipLaunchKernelGGL(k_trace_primary<32>, grid, dim3(kTraceBlock), 0, s, a);
-	else hipLaunchKernelGGL(k_trace_primary<64>, grid, dim3(kTraceBlock), 0, s, a);
+	const bool cnt = a.visit_counters != nullptr;
+	if(a.scene.stack_depth <= 32) {
+		if(cnt) hipLaunchKernelGGL((k_trace_primary<32, true>), grid, dim3(kTraceBlock), 0, s, a);
+		else hipLaunchKernelGGL((k_trace_primary<32, false>), grid, dim3(kTraceBlock), 0, s, a);
+	} else {
+		if(cnt) hipLaunchKernelGGL((k_trace_primary<64, true>), grid, dim3(kTraceBlock), 0, s, a);
+		else hipLaunchKernelGGL((k_trace_primary<64, false>), grid, dim3(kTraceBlock), 0, s, a);
+	}
 	return hipGetLastError();
 }
 
@@ -321,8 +335,14 @@ hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
 	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
 	if(threads == 0) return hipSuccess;
 	const dim3 grid(blocks_for(threads, kTraceBlock));
-	if(a.scene.stack_depth <= 32) hipLaunchKernelGGL(k_shade<32>, grid, dim3(kTraceBlock), 0, s, a);
-	else hipLaunchKernelGGL(k_shade<64>, grid, dim3(kTraceBlock), 0, s, a);
+	const bool cnt = a.visit_counters != nullptr;
+	if(a.scene.stack_depth <= 32) {
+		if(cnt) hipLaunchKernelGGL((k_shade<32, true>), grid, dim3(kTraceBlock), 0, s, a);
+		else hipLaunchKernelGGL((k_shade<32, false>), grid, dim3(kTraceBlock), 0, s, a);
+	} else {
+		if(cnt) hipLaunchKernelGGL((k_shade<64, true>), grid, dim3(kTraceBlock), 0, s, a);
+		else hipLaunchKernelGGL((k_shade<64, false>), grid, dim3(kTraceBlock), 0, s, a);
+	}
 	return hipGetLastError();
 }
 

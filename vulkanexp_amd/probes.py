@@ -168,6 +168,13 @@ class IrradianceProbes:
         check(self._lib.ddgi_get_timings(self.handle, C.byref(t)), self.handle)
         return {k: getattr(t, k) for k, _ in _lib.Timings._fields_}
 
+    def trace_stats(self) -> dict:
+        """BVH fetch counts of the last update run with UPDATE_COUNT_VISITS."""
+        out = np.zeros(5, np.uint64)
+        check(self._lib.ddgi_get_trace_stats(self.handle, _p(out)), self.handle)
+        return {"primary_nodes": int(out[0]), "primary_tris": int(out[1]), "shadow_nodes": int(out[2]),
+                "shadow_tris": int(out[3]), "shadow_rays": int(out[4])}
+
     def device_view(self) -> _lib.DeviceView:
         v = _lib.DeviceView()
         check(self._lib.ddgi_get_device_view(self.handle, C.byref(v)), self.handle)

@@ -193,11 +193,12 @@ def _transform(scale, angle, translate):
     return m.astype(np.float32).reshape(12)
 
 
-def clustered_scene(seed: int = 2, n_instances: int = 256, extent=(60.0, 20.0, 40.0), n_materials=256) -> Scene:
+def clustered_scene(seed: int = 2, n_instances: int = 256, extent=(60.0, 20.0, 40.0), n_materials=256,
+                    base_scale: float = 1.5) -> Scene:
     """Instance 0: a 60x40 m floor grid of 1024 triangles; instances 1..: 1024-triangle meshes, half
     radially displaced lat-long spheres, half 8x8x12-segment boxes, each with its own mesh and
-    material (albedo U[0.05, 0.95]^3), uniform scale lognormal(sigma 0.5) x 0.8 m, a random yaw,
-    positions clustered around 8 centres.  256 instances give 262,144 triangles."""
+    material (albedo U[0.05, 0.95]^3), uniform scale base_scale x lognormal(sigma 0.5) (capped at
+    5 m), a random yaw, positions clustered around 8 centres.  256 instances give 262,144 triangles."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sb = SceneBuilder()
     mats = [sb.add_material(rng.uniform(0.05, 0.95, 3)) for _ in range(n_materials)]
@@ -214,9 +215,9 @@ def clustered_scene(seed: int = 2, n_instances: int = 256, extent=(60.0, 20.0, 4
     sb.add_instance(sb.add_mesh(fpos, fn, ftris, mats[0]))
     centres = np.stack([rng.uniform(6, ex[0] - 6, 8), rng.uniform(4, ex[2] - 4, 8)], -1)
     for k in range(1, n_instances):
-        scale = 0.8 * math.exp(rng.normal(0.0, 0.5))
-        scale = min(scale, 4.0)
-        cx, cz = centres[rng.integers(0, 8)] + rng.normal(0.0, 5.0, 2)
+        scale = base_scale * math.exp(rng.normal(0.0, 0.5))
+        scale = min(scale, 5.0)
+        cx, cz = centres[rng.integers(0, 8)] + rng.normal(0.0, 4.0, 2)
         cx, cz = float(np.clip(cx, 2, ex[0] - 2)), float(np.clip(cz, 2, ex[2] - 2))
         cy = scale + rng.uniform(0.0, 0.6 * ex[1] - 2 * scale) if 0.6 * ex[1] > 2 * scale else scale
         if k % 2 == 1:
