@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blend", choices=["fp32", "bf16", "fp16"], default="fp32")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 3 elementwise copies of 256 MiB (torch.add(x, 0)) so that profiler byte "
+                         "counters can be scaled against a kernel of known traffic")
     return ap.parse_args()
 
 
@@ -140,6 +143,13 @@ def main():
     kernels["blend"]["TFLOP/s"] = round(blend_flops / (avg["blend_ms"] * 1e-3) / 1e12, 2)
     trace_stats = {"nodes_per_primary_ray": st["primary_nodes"] / rays, "tris_per_primary_ray": st["primary_tris"] / rays,
                    "shadow_rays": st["shadow_rays"], "nodes_per_shadow_ray": st["shadow_nodes"] / max(1, st["shadow_rays"])}
+
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
 
     cpu = None
     if not args.no_cpu:

@@ -1026,6 +1026,20 @@ void orc_blend(const ddgi_grid_params* p, const float* dirs, const double* rays,
 	});
 }
 
+// ---- border copy only (probesCopyBorders.comp) of the listed probes, in place ----
+void orc_border_copy(const ddgi_grid_params* p, const uint32_t* indices, uint32_t count, double* irr, double* dep) {
+	Grid g = make_grid(p);
+	for(uint32_t sl = 0; sl < count; ++sl) {
+		int gi[3];
+		g.grid_index(indices[sl], gi);
+		uint32_t ox, oy;
+		g.tile_origin(gi, p->color_res, ox, oy);
+		border_copy(irr, g.image_width(p->color_res), 3, ox, oy, p->color_res);
+		g.tile_origin(gi, p->depth_res, ox, oy);
+		border_copy(dep, g.image_width(p->depth_res), 2, ox, oy, p->depth_res);
+	}
+}
+
 // ---- sampleProbes at n points (double xyz arrays) ----
 void orc_sample(const ddgi_grid_params* p, const uint32_t* states, const double* irr, const double* dep,
 				const double* pos, const double* nrm, const double* tocam, uint64_t n, double* out, int fp32) {

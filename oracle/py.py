@@ -54,6 +54,7 @@ def lib():
             "orc_shade_aux_size": (u64, []),
             "orc_blend": (None, [P, P, P, P, u32, P, P, P, P, P, C.c_int]),
             "orc_sample": (None, [P, P, P, P, P, P, P, u64, P, C.c_int]),
+            "orc_border_copy": (None, [P, P, u32, P, P]),
             "orc_pbr": (None, [P, P, P, P, P, C.c_double, C.c_double, P]),
             "orc_init_probes": (None, [P, P, P, P, P]),
             "orc_select": (u32, [P, u32, u32, P, P, P]),
@@ -232,6 +233,12 @@ def blend(grid: Grid, dirs: np.ndarray, rays: np.ndarray, indices, states, irr, 
     lib().orc_blend(_p(grid.params), _p(dirs), _p(rays), _p(indices), len(indices), _p(states), _p(irr), _p(dep),
                     _p(mc), _p(oor), int(fp32))
     return mc, oor
+
+
+def border_copy(grid: Grid, indices, irr, dep) -> None:
+    indices = np.ascontiguousarray(indices, np.uint32)
+    assert irr.flags.c_contiguous and dep.flags.c_contiguous
+    lib().orc_border_copy(_p(grid.params), _p(indices), len(indices), _p(irr), _p(dep))
 
 
 def sample(grid: Grid, states, irr, dep, pos, nrm, tocam, fp32=False) -> np.ndarray:

@@ -1,0 +1,113 @@
+"""The C-ABI library (libddgi.so) on the host: it loads, exports every function include/ddgi.h
+declares, its record layouts match the header, and argument errors come back as status codes
+(no GPU work is launched here)."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+from vulkanexp_amd import _lib, scenes
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load()
+    names = _lib.header_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib._SIGNATURES) == set(names)
+    assert lib.ddgi_abi_version() == 1
+
+
+def test_record_layouts_match_header():
+    text = open(_lib.HEADER_PATH).read()
+
+    def fields(struct):
+        body = re.search(r"typedef struct " + struct + r" \{(.*?)\} " + struct + ";", text, re.S).group(1)
+        return re.findall(r"^\s*(?:const\s+)?\w+\*?\s+(\w+)(?:\[\w+\])?;", body, re.M)
+    assert fields("ddgi_grid_params") == list(scenes.GRID_DTYPE.names)
+    assert fields("ddgi_vertex") == list(scenes.VERTEX_DTYPE.names)
+    assert fields("ddgi_material") == list(scenes.MATERIAL_DTYPE.names)
+    assert fields("ddgi_offset_entry") == list(scenes.OFFSET_DTYPE.names)
+    assert fields("ddgi_instance") == list(scenes.INSTANCE_DTYPE.names)
+    assert fields("ddgi_light") == list(scenes.LIGHT_DTYPE.names)
+    assert fields("ddgi_timings") == [f for f, _ in _lib.Timings._fields_]
+    assert fields("ddgi_device_view") == [f for f, _ in _lib.DeviceView._fields_]
+    assert fields("ddgi_options") == [f for f, _ in _lib.Options._fields_]
+    assert C.sizeof(_lib.Timings) == 40 and C.sizeof(_lib.Options) == 32
+
+
+def test_status_strings():
+    lib = _lib.load()
+    for s in range(8):
+        assert lib.ddgi_status_string(s)
+    assert lib.ddgi_status_string(99) == b"unknown status"
+
+
+def _grid(**kw):
+    return scenes.grid_params((0, 0, 0), (10, 10, 10), (4, 4, 4), 128, **kw)
+
+
+def _create(params, opts=None):
+    lib = _lib.load()
+    h = C.c_void_p()
+    st = lib.ddgi_create(params.ctypes.data_as(C.c_void_p), C.byref(opts) if opts else None, C.byref(h))
+    return st, h
+
+
+@pytest.mark.parametrize("mutate,expect", [
+    (lambda g: g.__setitem__("resolution", (1, 4, 4)), 1),        # probeGridCellSize would divide by 0
+    (lambda g: g.__setitem__("extent_max", (0, 10, 10)), 1),
+    (lambda g: g.__setitem__("rays_per_probe", 0), 1),
+    (lambda g: g.__setitem__("rays_per_probe", 513), 1),
+    (lambda g: g.__setitem__("hysteresis", np.nan), 1),
+    (lambda g: g.__setitem__("color_res", 6), 4),                  # only the reference's 8/16 tiles
+])
+def test_create_rejects_invalid_grids(mutate, expect):
+    g = _grid()
+    mutate(g)
+    st, h = _create(g)
+    assert st == expect and not h.value
+    assert _lib.load().ddgi_last_error(None)
+
+
+def test_create_without_gpu_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    st, h = _create(_grid())
+    assert st == 7 and not h.value
+
+
+def test_null_handles_are_rejected():
+    lib = _lib.load()
+    R = np.eye(3, dtype=np.float32).reshape(9)
+    assert lib.ddgi_update(None, R.ctypes.data_as(C.c_void_p), 0, None) == 1
+    assert lib.ddgi_destroy(None) == 1
+    assert lib.ddgi_create(None, None, None) == 1
+    assert lib.ddgi_init_probes(None, R.ctypes.data_as(C.c_void_p), None) == 1
+    assert lib.ddgi_synchronize(None) == 1
+
+
+def test_scene_generators():
+    small = scenes.room_boxes_scene(1)
+    assert small.triangle_count == 780 and len(small.materials) == 65
+    assert scenes.room_boxes_scene(1, open_top=True).triangle_count == 778
+    big = scenes.clustered_scene(2)
+    assert big.triangle_count == 262144 and len(big.instances) == 256 and len(big.materials) == 256
+    # seeded: identical arrays for the same seed, different for another
+    again = scenes.room_boxes_scene(1)
+    assert np.array_equal(small.vertices, again.vertices) and np.array_equal(small.materials, again.materials)
+    assert not np.array_equal(small.vertices, scenes.room_boxes_scene(2).vertices)
+    # materials follow the configs: non-metallic, rough, no textures, albedo ranges
+    m = small.materials
+    assert np.all(m["metallic"] == 0) and np.all(m["roughness"] == 1)
+    assert np.all(m["albedo_texture"] == scenes.NO_TEXTURE)
+    assert np.all((m["base_color"][1:] >= 0.1) & (m["base_color"][1:] <= 0.9))
+    # orientations are rotations with rows X, Y, Z of genBasis
+    for R in scenes.orientations(3, 5):
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-6) and np.linalg.det(R) == pytest.approx(1.0, abs=1e-6)
+    light = scenes.sun_light()
+    d = light["direction"][:3]
+    assert np.allclose(d, -np.array([0.3, -1, 0.2]) / np.linalg.norm([0.3, -1, 0.2]), atol=1e-7)

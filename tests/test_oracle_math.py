@@ -1,0 +1,173 @@
+"""Pins the oracle's math (src/shaders/irradiance.glsl, probesUpdate.glsl, pbrMetallicRoughness.glsl)
+against closed forms and independent numpy computations."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import py as orc
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+PHI_M1 = (math.sqrt(5.0) - 1.0) / 2.0   # golden ratio - 1
+REF_PI = 3.1415926538                    # src/shaders/common.glsl:4
+
+
+@pytest.mark.parametrize("n", [64, 128, 192, 256, 512])
+def test_fibonacci_closed_form(n):
+    for i in range(n):
+        v = orc.fib64(i, n)
+        assert abs(v[2] - (1.0 - (2 * i + 1) / n)) < 1e-15          # z spacing exactly (2i+1)/n
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-14                  # on the unit sphere
+        phi = math.atan2(v[1], v[0]) % (2 * math.pi)
+        want = (2 * REF_PI * ((i * PHI_M1) % 1.0)) % (2 * math.pi)
+        if math.hypot(v[0], v[1]) > 1e-6:
+            assert abs((phi - want + math.pi) % (2 * math.pi) - math.pi) < 1e-9   # golden-angle spiral
+    # mean of the set is ~0 (near-uniform coverage)
+    pts = np.array([orc.fib64(i, n) for i in range(n)])
+    assert np.linalg.norm(pts.mean(0)) < 2.0 / n
+
+
+def test_fibonacci_beyond_n_is_non_unit_axis():
+    # probesInit.rgen traces 512 rays with n = raysPerProbe: for i >= n sin(theta) clamps to 0 and
+    # the direction is (0, 0, cos(theta)) with |cos(theta)| > 1 (a non-unit vector)
+    n = 128
+    for i in range(n, 512):
+        v = orc.fib64(i, n)
+        assert v[0] == 0.0 and v[1] == 0.0
+        assert v[2] == pytest.approx(1.0 - (2 * i + 1) / n, abs=1e-15)
+        assert v[2] < -1.0
+
+
+def test_fib_table32_is_fp64_rounded():
+    for n in (128, 256):
+        t = orc.fib_table32(n, n)
+        ref = np.array([orc.fib64(i, n) for i in range(n)]).astype(np.float32)
+        assert np.array_equal(t.view(np.uint32), ref.view(np.uint32))
+
+
+def _rand_dirs(k, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v = rng.normal(size=(k, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_oct_encode_decode_roundtrip():
+    for v in _rand_dirs(2000, 1):
+        o = orc.oct_encode(v)
+        assert abs(o[0]) <= 1 + 1e-12 and abs(o[1]) <= 1 + 1e-12
+        if v[2] >= 0:
+            assert abs(o[0]) + abs(o[1]) <= 1 + 1e-12       # upper hemisphere inside the diamond
+        assert np.allclose(orc.oct_decode(*o), v, atol=1e-12)
+
+
+def test_sphere_point_to_oct_uv_matches_encode_off_axes():
+    # "Should be close to 0.5 * octEncode(direction) + 0.5" (irradiance.glsl:120-121): identical
+    # whenever no component is exactly 0
+    for v in _rand_dirs(2000, 2):
+        uv = orc.oct_uv(v)
+        assert np.allclose(uv, 0.5 * orc.oct_encode(v) + 0.5, atol=1e-13)
+
+
+def test_sphere_point_to_oct_uv_sign_zero_quirk():
+    # sign(0) = 0 (H3): straight down maps to the tile centre, not to a corner as octEncode does
+    assert np.allclose(orc.oct_uv([0.0, 0.0, -1.0]), [0.5, 0.5])
+    assert np.allclose(0.5 * orc.oct_encode([0.0, 0.0, -1.0]) + 0.5, [1.0, 1.0])
+    # a -z direction with x == 0 keeps u at the centre line
+    uv = orc.oct_uv([0.0, 0.6, -0.8])
+    assert uv[0] == 0.5 and uv[1] == pytest.approx(0.5 + 0.5 * (1 - 0.0), abs=1e-12)
+
+
+def test_texel_directions_reference_constants():
+    # probesUpdate.glsl:33-42 uses 0.33333 and 0.142857 instead of 1/3 and 1/7
+    for kind, res, k in ((0, 6, 0.33333), (1, 14, 0.142857)):
+        d = orc.texel_dirs(kind)
+        for y in range(res):
+            for x in range(res):
+                exact = orc.oct_decode((x + 0.5) * 2.0 / res - 1.0, (y + 0.5) * 2.0 / res - 1.0)
+                ref = orc.oct_decode(k * (x - (res - 1) / 2), k * (y - (res - 1) / 2))
+                assert np.allclose(d[y, x], ref, atol=1e-14)
+                assert np.linalg.norm(d[y, x] - exact) < 5e-5      # the constants are 1e-5-close
+        # octahedral symmetry: mirrored texel coordinates give mirrored directions
+        assert np.allclose(d[:, ::-1, 0], -d[:, :, 0], atol=1e-12)
+        assert np.allclose(d[::-1, :, 1], -d[:, :, 1], atol=1e-12)
+
+
+def test_border_mirror_matches_reference_tables():
+    tables = json.load(open(os.path.join(GOLDEN, "border_tables.json")))
+    grid = orc.Grid(_grid((3, 2, 2)))
+    irr, dep = grid.zero_atlases()
+    rng = np.random.Generator(np.random.PCG64(3))
+    irr[:] = rng.uniform(size=irr.shape)
+    dep[:] = rng.uniform(size=dep.shape)
+    irr0, dep0 = irr.copy(), dep.copy()
+    probe = 1 + 3 * 1 + 6 * 1  # grid (1, 1, 1) of a 3x2x2 grid
+    orc.border_copy(grid, [probe], irr, dep)
+    gx, gy, gz = 1, 1, 1
+    for kind, img, img0, res in (("irradiance", irr, irr0, 8), ("depth", dep, dep0, 16)):
+        ox, oy = res * (gy * 3 + gx), res * gz
+        t = tables[kind]
+        touched = np.zeros(img.shape[:2], bool)
+        for (dx, dy), (sx, sy) in zip(t["dst"], t["src"]):
+            assert np.array_equal(img[oy + dy, ox + dx], img0[oy + sy, ox + sx]), (kind, dx, dy)
+            touched[oy + dy, ox + dx] = True
+        # nothing else changed (interiors and other probes)
+        assert np.array_equal(img[~touched], img0[~touched])
+        assert touched.sum() == 4 * res - 4
+
+
+def _grid(res, n=128, lo=(0.0, 0.0, 0.0), hi=(10.0, 10.0, 10.0), h=0.0, sharp=12.0):
+    from vulkanexp_amd.scenes import grid_params
+    return grid_params(lo, hi, res, n, hysteresis=h, depth_sharpness=sharp)
+
+
+def test_pbr_closed_forms():
+    n = np.array([0.0, 1.0, 0.0])
+    albedo = np.array([0.2, 0.5, 0.8])
+    lc = np.array([3.0, 3.0, 3.0])
+    # light, view and normal aligned: NdotL = NdotV = NdotH = VdotH = 1, G = 1, F = f0
+    for rough in (1.0, 0.5):
+        a = rough * rough
+        D = min((1.0 / a) ** 2 / REF_PI, 4.0)
+        spec = 0.04 * D * 1.0 / 4.0
+        diffuse = albedo * 0.96 * (1 - 0.04) / REF_PI
+        want = lc * (diffuse + spec)
+        got = orc.pbr(n, n, lc, n, albedo, 0.0, rough)
+        assert np.allclose(got, want, rtol=1e-12)
+    # fully metallic, aligned: no diffuse, F = albedo
+    a = 1.0
+    got = orc.pbr(n, n, lc, n, albedo, 1.0, 1.0)
+    assert np.allclose(got, lc * albedo * (1.0 / REF_PI) / 4.0, rtol=1e-12)
+    # grazing light below the horizon still contributes through NdotL clamped to 0.001
+    got = orc.pbr(n, n, lc, [1.0, -0.5, 0.0], albedo, 0.0, 1.0)
+    assert np.all(got > 0) and np.all(got < 0.01)
+
+
+def test_selection_reference_semantics():
+    # src/IrradianceProbes.cpp:396-424 by hand: pick idx if state != 0 and (idx + loop) % state == 0
+    states = np.array([1, 2, 0, 3, 8, 1], np.uint32)
+    got, off, loop = orc.select(states, 0, 0, 0)
+    assert list(got) == [0, 3, 5] and off == 0 and loop == 1        # (idx + 0) % s == 0
+    got, off, loop = orc.select(states, 0, off, loop)
+    assert list(got) == [0, 1, 5] and loop == 2                      # (idx + 1) % s == 0
+    # recompute independently
+    def ref(states, cap, off, loop):
+        out, idx, chk = [], off, 0
+        while chk < len(states) and (cap == 0 or len(out) < cap):
+            if states[idx] != 0 and (idx + loop) % states[idx] == 0:
+                out.append(idx)
+            idx += 1
+            if idx >= len(states):
+                idx, loop = 0, loop + 1
+            chk += 1
+        return out, idx, loop
+    rng = np.random.Generator(np.random.PCG64(4))
+    st = rng.choice(np.array([0, 1, 2, 3, 4, 5, 6, 7, 8], np.uint32), 97)
+    for cap in (0, 5, 17, 200):
+        off, loop = 0, 0
+        roff, rloop = 0, 0
+        for _ in range(9):
+            got, off, loop = orc.select(st, cap, off, loop)
+            want, roff, rloop = ref(st, cap, roff, rloop)
+            assert list(got) == want and off == roff and loop == rloop
