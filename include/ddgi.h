@@ -178,8 +178,10 @@ typedef struct ddgi_device_view {
 	void* irradiance;        /* exposed irradiance atlas, tiled float4 */
 	void* depth;             /* exposed depth atlas, tiled float2 */
 	void* states;            /* uint32[P] probe states */
-	void* rays;              /* float4[P*max_rays] (rgb, depth) of the last update, slot-major */
-	void* hits;              /* internal hit records of the last update */
+	void* rays;              /* float4[P*max_rays] (rgb, depth) of the last update, slot-major, entry
+	                            k of a slot holds ray ray_order[k] */
+	void* hits;              /* internal hit records of the last update, same order as rays */
+	void* ray_order;         /* uint16[max_rays]: ray index stored at each position of a slot */
 	void* probe_indices;     /* uint32[P] probes traced by the last update */
 	void* ray_directions;    /* float4[max_rays] world directions of the last update */
 	uint64_t probe_count;

@@ -1,0 +1,86 @@
+"""GPU blend modes against the FP64 oracle blend (src/shaders/probesUpdate.glsl) on the same ray
+buffer: the room + boxes scene and 8x4x8 grid with 512 rays per probe.
+
+Per-texel relative error |gpu - fp64| / max(|fp64|, 1e-3 * max|fp64|), stated per output:
+  FP32 (VALU)            irradiance and depth <= 2e-5
+  BF16 MFMA, hi/lo split irradiance and depth <= 1e-4
+  FP16 MFMA              irradiance <= 1e-3, depth <= 1e-3
+measured on the new-frame estimate (hysteresis 0) and after hysteresis 0.97."""
+import numpy as np
+import pytest
+
+from oracle import py as orc
+from vulkanexp_amd import IrradianceProbes, scenes
+from vulkanexp_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+BARS = {"fp32": (2e-5, 2e-5), "bf16": (1e-4, 1e-4), "fp16": (1e-3, 1e-3)}
+FLAGS = {"fp32": 0, "bf16": _lib.UPDATE_BLEND_BF16, "fp16": _lib.UPDATE_BLEND_FP16}
+
+
+def _rel(g, o):
+    floor = 1e-3 * max(np.abs(o).max(), 1e-12)
+    return np.abs(g - o) / np.maximum(np.abs(o), floor)
+
+
+@pytest.fixture(scope="module")
+def ray_buffer():
+    import torch
+    scene, grid = scenes.config_small(seed=1, rays=512, open_top=True)
+    p = IrradianceProbes(grid, max_rays_per_probe=512)
+    p.set_scene(scene)
+    p.set_light(scenes.sun_light())
+    p.set_sky(scenes.SKY_RADIANCE)
+    for R in scenes.orientations(31, 3):
+        p.update(R, _lib.UPDATE_ALL_PROBES)
+    R = scenes.orientations(32, 1)[0]
+    state = p.save_state()
+    p.update(R, _lib.UPDATE_ALL_PROBES)
+    p.synchronize()
+    rays, idx = p.last_update_rays()
+    p.load_state(state)
+    yield p, grid, rays, idx, R, state
+    p.destroy()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("hysteresis", [0.0, 0.97])
+def test_blend_modes_against_fp64(ray_buffer, mode, hysteresis):
+    import torch
+    p, grid, rays, idx, R, state = ray_buffer
+    g = np.array(grid)
+    g["hysteresis"] = hysteresis
+    p.load_state(state)        # restores atlases, states and the saved hysteresis ...
+    p.GridParameters = g
+    p.update_uniforms()        # ... which this then overrides
+    og = orc.Grid(g)
+    irr = p.irradiance()[..., :3].astype(np.float64)
+    dep = p.depth().astype(np.float64)
+    states = p.states()
+    t_rays = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    t_idx = torch.from_numpy(idx.view(np.int32).copy()).cuda()
+    p.blend_rays(t_rays, t_idx, R, FLAGS[mode])
+    p.synchronize()
+    gi, gd = p.irradiance()[..., :3], p.depth()
+    dirs = orc.frame_dirs(R, 512)
+    orc.blend(og, dirs, rays.astype(np.float64), idx, states, irr, dep)
+    ei, ed = _rel(gi, irr), _rel(gd, dep)
+    print(f"{mode} h={hysteresis}: irradiance max {ei.max():.2e} mean {ei.mean():.2e}; depth max {ed.max():.2e} mean {ed.mean():.2e}")
+    bi, bd = BARS[mode]
+    assert ei.max() <= bi and ed.max() <= bd
+    assert np.array_equal(p.states(), states)
+
+
+def test_bf16_and_fp32_select_same_states(ray_buffer):
+    import torch
+    p, grid, rays, idx, R, state = ray_buffer
+    t_rays = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    t_idx = torch.from_numpy(idx.view(np.int32).copy()).cuda()
+    out = []
+    for mode in ("fp32", "bf16"):
+        p.load_state(state)
+        p.blend_rays(t_rays, t_idx, R, FLAGS[mode])
+        p.synchronize()
+        out.append(p.states())
+    assert np.mean(out[0] == out[1]) >= 0.99

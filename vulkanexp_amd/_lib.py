@@ -44,7 +44,8 @@ class Timings(C.Structure):
 
 class DeviceView(C.Structure):
     _fields_ = [("irradiance", C.c_void_p), ("depth", C.c_void_p), ("states", C.c_void_p), ("rays", C.c_void_p),
-                ("hits", C.c_void_p), ("probe_indices", C.c_void_p), ("ray_directions", C.c_void_p),
+                ("hits", C.c_void_p), ("ray_order", C.c_void_p), ("probe_indices", C.c_void_p),
+                ("ray_directions", C.c_void_p),
                 ("probe_count", C.c_uint64), ("max_rays_per_probe", C.c_uint32), ("last_update_count", C.c_uint32)]
 
 

@@ -290,12 +290,14 @@ void destroy_handle(Handle* h) {
 	delete h;
 }
 
-ddgi_status enqueue_blend_and_publish(Handle* h, const Rot& R, const float4* rays, uint32_t max_slots, bool all,
-									  uint32_t flags, hipStream_t s) {
+// `order`: the ray order of the `rays` records (the traversal's thread order for ddgi_update,
+// nullptr = natural order for caller-provided buffers).
+ddgi_status enqueue_blend_and_publish(Handle* h, const Rot& R, const float4* rays, const uint16_t* order,
+									  uint32_t max_slots, bool all, uint32_t flags, hipStream_t s) {
 	const GridConst g = make_grid_const(*h);
 	const uint32_t mode = (flags & DDGI_UPDATE_BLEND_BF16) ? kBlendBF16 : ((flags & DDGI_UPDATE_BLEND_FP16) ? kBlendFP16 : kBlendFP32);
 	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s));
-	HIP_TRY(h, launch_blend_weights(g, h->dirs, h->wt, h->inv_sum, h->w_hi, h->w_lo, mode, s));
+	HIP_TRY(h, launch_blend_weights(g, h->dirs, order, h->wt, h->inv_sum, h->w_hi, h->w_lo, mode, s));
 	const int nxt = h->cur ^ 1;
 	if(!all) {
 		// probes that are not updated keep their tiles: start the next atlas as a copy
@@ -405,8 +407,9 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->dirs, 512 * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->wt, size_t(kMaxRays) * kBlendTexels * sizeof(float));
 	if(e == hipSuccess) e = dalloc(&h->inv_sum, kBlendTexels * sizeof(float));
-	if(e == hipSuccess) e = dalloc(&h->w_hi, size_t(kMaxRays) * kBlendTexels * 2);
-	if(e == hipSuccess) e = dalloc(&h->w_lo, size_t(kMaxRays) * kBlendTexels * 2);
+	// 16-bit MFMA weights: 256 padded rows x 512 rays
+	if(e == hipSuccess) e = dalloc(&h->w_hi, size_t(256) * kMaxRays * 2);
+	if(e == hipSuccess) e = dalloc(&h->w_lo, size_t(256) * kMaxRays * 2);
 	if(e == hipSuccess) e = dalloc(&h->visit, 8 * sizeof(unsigned long long));
 	if(e != hipSuccess) {
 		h->err = std::string("allocation failed: ") + hipGetErrorString(e);
@@ -673,7 +676,7 @@ ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
 	HIP_TRY(h, launch_shade(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
-	if((st = enqueue_blend_and_publish(h, R, h->rays, max_slots, all, flags, s)) != DDGI_OK) return st;
+	if((st = enqueue_blend_and_publish(h, R, h->rays, h->perm, max_slots, all, flags, s)) != DDGI_OK) return st;
 	if(timing) {
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));
 		HIP_TRY(h, hipEventRecord(h->ev[5], s));
@@ -704,7 +707,7 @@ ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_in
 	HIP_TRY(h, launch_set_count(h->sel, count, s));
 	h->indices_are_iota = false;
 	// the listed probes may not cover the grid: the next atlas starts as a copy of the exposed one
-	if((st = enqueue_blend_and_publish(h, R, static_cast<const float4*>(rays), count, false, flags, s)) != DDGI_OK)
+	if((st = enqueue_blend_and_publish(h, R, static_cast<const float4*>(rays), nullptr, count, false, flags, s)) != DDGI_OK)
 		return st;
 	if(timing) {
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));
@@ -861,6 +864,7 @@ ddgi_status ddgi_get_device_view(void* handle, ddgi_device_view* out) {
 	out->states = h->states;
 	out->rays = h->rays;
 	out->hits = h->hits;
+	out->ray_order = h->perm;
 	out->probe_indices = h->indices;
 	out->ray_directions = h->dirs;
 	out->probe_count = h->P;

@@ -50,49 +50,55 @@ __device__ __forceinline__ void texel_direction(uint32_t t, float& x, float& y, 
 	z /= l;
 }
 
-// One block per texel: weights for every ray, their sum, and the 16-bit hi/lo split.
-__global__ __launch_bounds__(256) void k_blend_weights(GridConst g, const float4* __restrict__ dirs, float* __restrict__ wt,
-													   float* __restrict__ inv_sum, __hip_bfloat16* __restrict__ w_hi,
-													   __hip_bfloat16* __restrict__ w_lo, __half* __restrict__ w_h,
-													   uint32_t mode, uint32_t n_pad) {
+// One block per padded weight row (256 rows: irradiance texels at rows 0..35, depth texels at rows
+// 48..243, the rest zero): the weight of every ray, their sum, and for the MFMA blends the 16-bit
+// copies in [row][n_pad] layout (BF16: hi/lo split; FP16: one copy).
+__global__ __launch_bounds__(256) void k_blend_weights(GridConst g, const float4* __restrict__ dirs,
+													   const uint16_t* __restrict__ order, float* __restrict__ wt,
+													   float* __restrict__ inv_sum, void* __restrict__ w_hi_v,
+													   void* __restrict__ w_lo_v, uint32_t mode, uint32_t n_pad) {
 	__shared__ float red[256];
-	const uint32_t t = blockIdx.x;
-	float tx, ty, tz;
-	texel_direction(t, tx, ty, tz);
-	const bool depth = t >= kIrrTexels;
-	float sum = 0.0f, sum16 = 0.0f;
+	const uint32_t row = blockIdx.x;
+	const bool irr_row = row < kIrrTexels;
+	const bool dep_row = row >= 48u && row < 48u + kDepTexels;
+	const bool valid = irr_row || dep_row;
+	const uint32_t t = irr_row ? row : row - 48u + kIrrTexels;  // texel index 0..231 when valid
+	float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+	if(valid) texel_direction(t, tx, ty, tz);
+	__bf16* w_hi = static_cast<__bf16*>(w_hi_v);
+	__bf16* w_lo = static_cast<__bf16*>(w_lo_v);
+	_Float16* w_h = static_cast<_Float16*>(w_hi_v);
+	float sum = 0.0f;
 	for(uint32_t i = threadIdx.x; i < n_pad; i += 256) {
 		float w = 0.0f;
-		if(i < g.n) {
-			const float4 d = dirs[i];
+		if(valid && i < g.n) {
+			const float4 d = dirs[order ? order[i] : i];
 			w = fmaxf(0.0f, tx * d.x + ty * d.y + tz * d.z);
-			if(depth) w = powf(w, g.sharpness);
-			sum += w;
-			wt[size_t(i) * kBlendTexels + t] = w;
+			if(dep_row) w = powf(w, g.sharpness);
+			if(mode == kBlendFP32) wt[size_t(i) * kBlendTexels + t] = w;
 		}
 		if(mode == kBlendBF16) {
-			// compensated split: w = hi + lo with both parts in bf16; normalise by the sum of what
-			// the MFMA actually multiplies
-			const __hip_bfloat16 hi = __float2bfloat16(w);
-			const float r = w - __bfloat162float(hi);
-			const __hip_bfloat16 lo = __float2bfloat16(r);
-			w_hi[size_t(t) * n_pad + i] = hi;
-			w_lo[size_t(t) * n_pad + i] = lo;
-			sum16 += __bfloat162float(hi) + __bfloat162float(lo);
+			// compensated split: w = hi + lo, both bf16; normalise by the sum of what the MFMA
+			// multiplies, so the rounding of the weights cancels in the normalisation
+			const __bf16 hi = __bf16(w);
+			const __bf16 lo = __bf16(w - float(hi));
+			w_hi[size_t(row) * n_pad + i] = hi;
+			w_lo[size_t(row) * n_pad + i] = lo;
+			w = float(hi) + float(lo);
 		} else if(mode == kBlendFP16) {
-			const __half h = __float2half(w);
-			w_h[size_t(t) * n_pad + i] = h;
-			sum16 += __half2float(h);
+			const _Float16 h = _Float16(w);
+			w_h[size_t(row) * n_pad + i] = h;
+			w = float(h);
 		}
+		sum += w;
 	}
-	const float s = (mode == kBlendFP32) ? sum : sum16;
-	red[threadIdx.x] = s;
+	red[threadIdx.x] = sum;
 	__syncthreads();
 	for(int o = 128; o > 0; o >>= 1) {
 		if(int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
 		__syncthreads();
 	}
-	if(threadIdx.x == 0) inv_sum[t] = red[0] > 1e-3f ? 1.0f / red[0] : 1.0f;
+	if(threadIdx.x == 0 && valid) inv_sum[t] = red[0] > 1e-3f ? 1.0f / red[0] : 1.0f;
 }
 
 // Border destinations mirroring interior texel (x, y) of an R x R tile (probesCopyBorders.comp
@@ -261,22 +267,191 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 	if(t < np) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
+// ---------------------------------------------------------------------------------------------
+// MFMA blend.  Per block: 16 probe slots.  GEMM shapes per block, as 16x16 output tiles of
+// v_mfma_f32_16x16x32_{bf16,f16}:
+//   irradiance: M = 48 (36 texels + 12 zero rows) x N = 48 (16 probes x rgb)   -> 3 x 3 tiles
+//   depth:      M = 208 (196 + 12)                x N = 32 (16 probes x d, d^2) -> 13 x 2 tiles
+// K = rays, in steps of 32.  The 16 M-tiles (3 irradiance, 13 depth) are dealt to the 4 waves
+// (wave w: tiles w, w+4, w+8, w+12), so each wave loads a weight fragment once per K step and
+// reuses it for all N-tiles of its row.  Weights (A) come straight from L2 in the padded
+// [256][n_pad] 16-bit layout; ray data (B) is staged in LDS per 64-ray chunk, column-major so a
+// lane's 8 consecutive K values are one 16-byte read.
+//   BF16: w = w_hi + w_lo, x = x_hi + x_lo, acc += w_hi x_hi + w_hi x_lo + w_lo x_hi (3 MFMAs;
+//         relative error ~2^-16), normalised by the sum of the split weights.
+//   FP16: one product of fp16 weights and data (relative error ~2^-11 per term).
+// Accumulator layout (verified for gfx950): lane l, register r holds D[4*(l/16)+r][l%16].
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaChunk = 64;              // rays staged per LDS chunk
+constexpr int kBStride = kMfmaChunk + 8;    // 16-bit elements per LDS column (padded)
+constexpr int kIrrCols = kProbesPerBlock * 3, kDepCols = kProbesPerBlock * 2;
+constexpr int kWRows = 256;                 // padded weight rows: irr 0..47, depth 48..255
+constexpr int kDepRow0 = 48;
+
+template <uint32_t MODE>
+struct MfmaTypes;
+template <>
+struct MfmaTypes<kBlendBF16> {
+	typedef __bf16 T;
+	typedef bf16x8_t V;
+	static constexpr int kParts = 2;
+	__device__ static floatx4_t mma(V a, V b, floatx4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct MfmaTypes<kBlendFP16> {
+	typedef _Float16 T;
+	typedef f16x8_t V;
+	static constexpr int kParts = 1;
+	__device__ static floatx4_t mma(V a, V b, floatx4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+
+template <uint32_t MODE>
+__global__ __launch_bounds__(256) void k_blend_mfma(BlendArgs a, uint32_t n_pad) {
+	typedef MfmaTypes<MODE> M;
+	typedef typename M::T T;
+	typedef typename M::V V;
+	constexpr int P = M::kParts;
+	// LDS: staging (B operand) and, after the K loop, the accumulator tiles (aliased)
+	constexpr int kStageBytes = P * (kIrrCols + kDepCols) * kBStride * 2;
+	constexpr int kOutBytes = (48 * kIrrCols + 208 * kDepCols) * 4;
+	__shared__ __attribute__((aligned(16))) unsigned char smem[kStageBytes > kOutBytes ? kStageBytes : kOutBytes];
+	__shared__ uint32_t s_oor[kProbesPerBlock];
+	__shared__ uint32_t s_maxc[kProbesPerBlock];
+	T* sb = reinterpret_cast<T*>(smem);  // [part][col][kBStride], irr cols first then depth cols
+	float* s_out_irr = reinterpret_cast<float*>(smem);          // [48][48]
+	float* s_out_dep = s_out_irr + 48 * kIrrCols;               // [208][32]
+
+	const uint32_t count = *a.count;
+	const uint32_t slot0 = blockIdx.x * kProbesPerBlock;
+	if(slot0 >= count) return;
+	const uint32_t np = min(uint32_t(kProbesPerBlock), count - slot0);
+	const uint32_t n = a.grid.n;
+	const float cell_len = a.grid.cell_len;
+	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+	const int lrow = lane & 15, lgrp = lane >> 4;
+	const T* __restrict__ wparts[2] = {static_cast<const T*>(a.w_hi16), static_cast<const T*>(a.w_lo16)};
+
+	floatx4_t acc[4][3];
+#pragma unroll
+	for(int q = 0; q < 4; ++q)
+#pragma unroll
+		for(int c = 0; c < 3; ++c) acc[q][c] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+
+	// staging assignment: thread t loads rays 4*(t%16) .. +3 of probe t/16 of the chunk
+	const int sp = t >> 4, sk = (t & 15) * 4;
+	uint32_t my_oor = 0;
+	for(uint32_t k0 = 0; k0 < n_pad; k0 += kMfmaChunk) {
+		__syncthreads();
+		for(int q = 0; q < 4; ++q) {
+			const uint32_t k = k0 + sk + q;
+			float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			float dc = 0.0f;
+			if(uint32_t(sp) < np && k < n) {
+				r = a.rays[size_t(slot0 + sp) * n + k];
+				if(r.w < 0.0f || r.w > cell_len) ++my_oor;
+				dc = fminf(cell_len, r.w);
+				if(dc < 0.0f) dc = cell_len;
+			}
+			const float v[5] = {r.x, r.y, r.z, dc, dc * dc};
+			const int col[5] = {sp * 3, sp * 3 + 1, sp * 3 + 2, kIrrCols + sp * 2, kIrrCols + sp * 2 + 1};
+#pragma unroll
+			for(int c = 0; c < 5; ++c) {
+				const T hi = T(v[c]);
+				sb[(0 * (kIrrCols + kDepCols) + col[c]) * kBStride + sk + q] = hi;
+				if(P == 2) sb[(1 * (kIrrCols + kDepCols) + col[c]) * kBStride + sk + q] = T(v[c] - float(hi));
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for(int ks = 0; ks < kMfmaChunk; ks += 32) {
+			const uint32_t kg = k0 + ks + lgrp * 8;  // this lane's 8 K values (same mapping for A and B)
+#pragma unroll
+			for(int q = 0; q < 4; ++q) {
+				const int mt = wave + 4 * q;            // 0..2 irradiance, 3..15 depth
+				const bool irr = mt < 3;
+				const int wrow = irr ? mt * 16 + lrow : kDepRow0 + (mt - 3) * 16 + lrow;
+				V A[2];
+#pragma unroll
+				for(int p = 0; p < P; ++p) A[p] = *reinterpret_cast<const V*>(wparts[p] + size_t(wrow) * n_pad + kg);
+				const int nt_count = irr ? 3 : 2;
+#pragma unroll
+				for(int nt = 0; nt < 3; ++nt) {
+					if(nt >= nt_count) break;
+					const int col = irr ? nt * 16 + lrow : kIrrCols + nt * 16 + lrow;
+					V B[2];
+#pragma unroll
+					for(int p = 0; p < P; ++p)
+						B[p] = *reinterpret_cast<const V*>(sb + (p * (kIrrCols + kDepCols) + col) * kBStride + ks + lgrp * 8);
+					acc[q][nt] = M::mma(A[0], B[0], acc[q][nt]);
+					if(P == 2) {
+						acc[q][nt] = M::mma(A[0], B[1], acc[q][nt]);
+						acc[q][nt] = M::mma(A[1], B[0], acc[q][nt]);
+					}
+				}
+			}
+		}
+	}
+	__syncthreads();  // staging done: reuse LDS for the accumulators
+#pragma unroll
+	for(int q = 0; q < 4; ++q) {
+		const int mt = wave + 4 * q;
+		const bool irr = mt < 3;
+#pragma unroll
+		for(int nt = 0; nt < 3; ++nt) {
+			if(!irr && nt == 2) break;
+#pragma unroll
+			for(int r = 0; r < 4; ++r) {
+				const int row = 4 * lgrp + r;
+				if(irr) s_out_irr[(mt * 16 + row) * kIrrCols + nt * 16 + lrow] = acc[q][nt][r];
+				else s_out_dep[((mt - 3) * 16 + row) * kDepCols + nt * 16 + lrow] = acc[q][nt][r];
+			}
+		}
+	}
+	// out-of-range counts: the 16 stagers of a probe are 16 consecutive lanes of one wave
+	for(int off = 8; off > 0; off >>= 1) my_oor += __shfl_xor(my_oor, off, 16);
+	if((t & 15) == 0) s_oor[sp] = my_oor;
+	if(t < kProbesPerBlock) s_maxc[t] = 0;
+	__syncthreads();
+	// epilogue: (probe, texel) pairs
+	for(uint32_t e = t; e < np * kBlendTexels; e += 256) {
+		const uint32_t p = e / kBlendTexels, tex = e % kBlendTexels;
+		float v[3];
+		if(tex < kIrrTexels) {
+			for(int c = 0; c < 3; ++c) v[c] = s_out_irr[tex * kIrrCols + p * 3 + c];
+		} else {
+			const uint32_t d = tex - kIrrTexels;
+			v[0] = s_out_dep[d * kDepCols + p * 2];
+			v[1] = s_out_dep[d * kDepCols + p * 2 + 1];
+			v[2] = 0.0f;
+		}
+		float mc = 0.0f;
+		store_texel(a, tex, a.indices[slot0 + p], v, a.inv_sum[tex], &mc);
+		if(tex < kIrrTexels) atomicMax(&s_maxc[p], __float_as_uint(mc));
+	}
+	__syncthreads();
+	if(uint32_t(t) < np) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
+}
+
 }  // namespace
 
-hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, float* wt, float* inv_sum, void* w_hi16,
-								void* w_lo16, uint32_t mode, hipStream_t s) {
-	const uint32_t n_pad = (g.n + 31u) & ~31u;
-	hipLaunchKernelGGL(k_blend_weights, dim3(kBlendTexels), dim3(256), 0, s, g, dirs, wt, inv_sum,
-					   static_cast<__hip_bfloat16*>(w_hi16), static_cast<__hip_bfloat16*>(w_lo16),
-					   static_cast<__half*>(w_hi16), mode, n_pad);
+hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, const uint16_t* order, float* wt, float* inv_sum,
+								void* w_hi16, void* w_lo16, uint32_t mode, hipStream_t s) {
+	const uint32_t n_pad = (g.n + kMfmaChunk - 1) & ~uint32_t(kMfmaChunk - 1);
+	hipLaunchKernelGGL(k_blend_weights, dim3(kWRows), dim3(256), 0, s, g, dirs, order, wt, inv_sum, w_hi16, w_lo16, mode,
+					   n_pad);
 	return hipGetLastError();
 }
 
 hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s) {
 	if(a.max_slots == 0) return hipSuccess;
 	const uint32_t blocks = (a.max_slots + kProbesPerBlock - 1) / kProbesPerBlock;
-	(void)mode;
-	hipLaunchKernelGGL(k_blend_fp32, dim3(blocks), dim3(kBlendThreads), 0, s, a);
+	const uint32_t n_pad = (a.grid.n + kMfmaChunk - 1) & ~uint32_t(kMfmaChunk - 1);
+	if(mode == kBlendBF16) hipLaunchKernelGGL(k_blend_mfma<kBlendBF16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
+	else if(mode == kBlendFP16) hipLaunchKernelGGL(k_blend_mfma<kBlendFP16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
+	else hipLaunchKernelGGL(k_blend_fp32, dim3(blocks), dim3(kBlendThreads), 0, s, a);
 	return hipGetLastError();
 }
 

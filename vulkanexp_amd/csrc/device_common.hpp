@@ -26,6 +26,17 @@ struct RayPrep {
 
 __device__ __forceinline__ float sel3(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
 
+// Workgroups are dealt round-robin to the 8 XCDs of an MI355X (workgroup i -> XCD i % 8), and each
+// XCD has its own L2.  Remap so that XCD x processes one contiguous range of logical blocks (a
+// slab of neighbouring probes whose rays touch overlapping BVH nodes and atlas tiles).  A
+// bijection on [0, nblocks) for any nblocks.
+constexpr uint32_t kNumXcd = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t block, uint32_t nblocks) {
+	const uint32_t q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+	const uint32_t x = block % kNumXcd, k = block / kNumXcd;
+	return x * q + min(x, r) + k;
+}
+
 __device__ __forceinline__ void ray_prepare(float ox, float oy, float oz, float dx, float dy, float dz, RayPrep& r) {
 	const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
 	const int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);

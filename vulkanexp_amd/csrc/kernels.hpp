@@ -123,8 +123,10 @@ hipError_t launch_iota(uint32_t* indices, uint32_t P, SelectState* st, hipStream
 hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float4* irr, const float2* dep,
 						 const float* pos, const float* nrm, const float* tocam, uint64_t count, float* out,
 						 hipStream_t s);
-hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, float* wt, float* inv_sum, void* w_hi16,
-								void* w_lo16, uint32_t mode, hipStream_t s);
+// Weight row k uses direction dirs[order[k]] (order = nullptr: identity), so that the blend can
+// consume ray records stored in the traversal's thread order.
+hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, const uint16_t* order, float* wt, float* inv_sum,
+								void* w_hi16, void* w_lo16, uint32_t mode, hipStream_t s);
 hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s);
 
 // blend modes

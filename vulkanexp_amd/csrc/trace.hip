@@ -40,7 +40,7 @@ template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
 	const uint32_t n = a.grid.n;
-	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+	const uint32_t gtid = xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x;
 	const uint32_t slot = gtid / n, j = gtid - slot * n;
 	if(slot >= *a.count) return;
 	const uint32_t i = a.perm[j];
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a)
 	const HitResult h = traverse<false, STACK, COUNT>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
 													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack + threadIdx.x,
 													  kTraceBlock, nn, nt);
-	a.hits[size_t(slot) * n + i] = pack_hit(h);
+	a.hits[size_t(slot) * n + j] = pack_hit(h);  // thread order: coalesced stores
 	if(COUNT) {
 		atomicAdd(&a.visit_counters[0], (unsigned long long)nn);
 		atomicAdd(&a.visit_counters[1], (unsigned long long)nt);
@@ -65,11 +65,11 @@ template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
 	const uint32_t n = a.grid.n;
-	const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+	const uint32_t gtid = xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x;
 	const uint32_t slot = gtid / n, j = gtid - slot * n;
 	if(slot >= *a.count) return;
 	const uint32_t i = a.perm[j];
-	const size_t ri = size_t(slot) * n + i;
+	const size_t ri = size_t(slot) * n + j;  // hits and rays are stored in thread (ray_order) order
 	const float4 hit = a.hits[ri];
 	const uint32_t code = __float_as_uint(hit.y);
 	if(code == 0xFFFFFFFFu) {  // miss.rmiss: sky radiance, depth -1

@@ -191,25 +191,33 @@ class IrradianceProbes:
         buf = np.frombuffer(blob, np.uint8).copy()
         check(self._lib.ddgi_load_state(self.handle, _p(buf), buf.size), self.handle)
 
+    def _ray_order(self, v, n):
+        import torch
+        return _device_copy(v.ray_order, n, torch.int16).numpy().view(np.uint16).astype(np.int64)
+
     def last_update_rays(self):
-        """(rays (count, n, 4) float32, probe indices (count,)) of the last update, copied to host."""
+        """(rays (count, n, 4) float32 in ray-index order, probe indices (count,)) of the last update."""
         import torch
         v = self.device_view()
         t = self.timings()
         count = t["probes_updated"]
         n = int(self.GridParameters["rays_per_probe"])
-        rays = _device_copy(v.rays, count * self.max_rays_per_probe * 4, torch.float32)[: count * n * 4]
+        stored = _device_copy(v.rays, count * n * 4, torch.float32).numpy().reshape(count, n, 4)
+        rays = np.empty_like(stored)
+        rays[:, self._ray_order(v, n)] = stored
         idx = _device_copy(v.probe_indices, count, torch.int32)
-        return rays.reshape(count, n, 4).numpy(), idx.numpy().view(np.uint32)
+        return rays, idx.numpy().view(np.uint32)
 
     def last_update_hits(self):
-        """Internal hit records (count, n, 4) as uint32 of the last update."""
+        """Internal hit records (count, n, 4) as uint32 of the last update, in ray-index order."""
         import torch
         v = self.device_view()
         count = self.timings()["probes_updated"]
         n = int(self.GridParameters["rays_per_probe"])
-        h = _device_copy(v.hits, count * n * 4, torch.int32)
-        return h.numpy().view(np.uint32).reshape(count, n, 4)
+        stored = _device_copy(v.hits, count * n * 4, torch.int32).numpy().view(np.uint32).reshape(count, n, 4)
+        hits = np.empty_like(stored)
+        hits[:, self._ray_order(v, n)] = stored
+        return hits
 
 
 def _device_copy(ptr: int, numel: int, dtype):
