@@ -240,6 +240,31 @@ ddgi_status ddgi_trace_rays(void* handle, const ddgi_ray* rays, ddgi_hit* hits, 
 ddgi_status ddgi_sample(void* handle, const float* position, const float* normal, const float* to_camera,
 						uint64_t count, float* rgb_out, void* stream);
 
+/* ---- differentiable probes (no counterpart in the reference, which is not differentiable) ----
+ * Gradients of a shaded-G-buffer loss with respect to per-material albedo and the sun radiance,
+ * through one probe update (ray radiance -> irradiance blend + border copy) and probe sampling
+ * (sampleProbes) at G-buffer texels, shaded as irradiance * albedo * (1 - 0.004) like the
+ * reference's gather (src/shaders/FinalGather.frag:62-72).  loss = mean over texels and channels
+ * of (shade - target)^2.
+ * ddgi_diff_prepare: runs one update of every probe with `orientation` (like ddgi_update with
+ *   DDGI_UPDATE_ALL_PROBES) and keeps what the parameters do not change: ray hits and shading
+ *   factors, the irradiance atlas before the update, the depth atlas and the probe states after
+ *   it, and the G-buffer (device arrays: position/normal/to_camera float[3*count], material
+ *   uint32[count], copied).  Requires non-metallic materials (radiance is then affine in albedo
+ *   and sun); material_count (<= 1024) albedo slots, every material id < material_count.
+ * ddgi_diff_forward: device albedo float[3*material_count], sun float[3], target float[3*count]
+ *   -> shade_out float[3*count], loss_out float[1].
+ * ddgi_diff_backward: differentiates the most recent ddgi_diff_forward of the handle (same albedo,
+ *   sun, target); grad_loss float[1] is d(total)/d(loss); writes grad_albedo float[3*material_count]
+ *   and grad_sun float[3].  Deterministic (fixed-point reductions). */
+ddgi_status ddgi_diff_prepare(void* handle, const float orientation[9], const float* gbuf_position,
+							  const float* gbuf_normal, const float* gbuf_to_camera, const uint32_t* gbuf_material,
+							  uint64_t texel_count, uint32_t material_count, void* stream);
+ddgi_status ddgi_diff_forward(void* handle, const float* albedo, const float* sun, const float* target, float* shade_out,
+							  float* loss_out, void* stream);
+ddgi_status ddgi_diff_backward(void* handle, const float* albedo, const float* sun, const float* target,
+							   const float* grad_loss, float* grad_albedo, float* grad_sun, void* stream);
+
 /* ---- state access ---- */
 ddgi_status ddgi_get_atlas(void* handle, uint32_t kind, uint32_t layout, void* host_dst, size_t bytes);
 ddgi_status ddgi_set_atlas(void* handle, uint32_t kind, uint32_t layout, const void* host_src, size_t bytes);

@@ -325,24 +325,38 @@ static void bilinear_tile(const Atlas& a, uint32_t ox, uint32_t oy, uint32_t res
 	}
 }
 
+// The irradiance-independent part of one sampleProbes call: for each of the 8 cage probes the
+// visibility/trilinear weight, the fallback weight and the bilinear footprint in the irradiance
+// tile.  (Everything the adjoint needs to recompute; nothing here depends on the irradiance.)
 template <class T>
-V3<T> sample_probes(const Grid& g, const uint32_t* states, const Atlas& irr, const Atlas& dep, V3<T> position,
-					V3<T> normal, V3<T> to_camera) {
+struct SampleCtx {
+	bool outside = true;
+	int used = 0;
+	uint32_t ox[8], oy[8];
+	int x0[8], y0[8], x1[8], y1[8];
+	T wx[8], wy[8], w[8], fw[8];
+	T total_w = 0, total_fb = 0;
+};
+
+template <class T>
+void sample_collect(const Grid& g, const uint32_t* states, const Atlas& dep, V3<T> position, V3<T> normal,
+					V3<T> to_camera, SampleCtx<T>& ctx) {
+	ctx = SampleCtx<T>();
 	const V3<T> cell = g.cell<T>();
 	const V3<T> emin(T(g.p.extent_min[0]), T(g.p.extent_min[1]), T(g.p.extent_min[2]));
 	const V3<T> acell = vabs(cell);
 	const V3<T> gc = (position - emin) / acell;
-	if(gc.x < T(0) || gc.y < T(0) || gc.z < T(0)) return {};
+	if(gc.x < T(0) || gc.y < T(0) || gc.z < T(0)) return;
+	ctx.outside = false;
 	const V3<T> bias = (normal + to_camera) * T(g.p.shadow_bias);
 	const V3<T> biased = position + bias;
 	int first[3] = {int(gc.x), int(gc.y), int(gc.z)};
 	const V3<T> p0 = g.position<T>(first);
 	V3<T> alpha = (position - p0) / acell;
 	alpha = {clampT(alpha.x, T(0), T(1)), clampT(alpha.y, T(0), T(1)), clampT(alpha.z, T(0), T(1))};
-
-	V3<T> final_color, fallback_color;
-	T total_w = 0, total_fb = 0;
 	const uint32_t cres = g.p.color_res, dres = g.p.depth_res;
+	T cu, cv;
+	sphere_to_oct_uv(normal, cu, cv);
 	for(int i = 0; i < 8; ++i) {
 		int off[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
 		int pc[3] = {first[0] + off[0], first[1] + off[1], first[2] + off[2]};
@@ -351,8 +365,7 @@ V3<T> sample_probes(const Grid& g, const uint32_t* states, const Atlas& irr, con
 		const V3<T> pp = g.position<T>(pc);
 		const V3<T> dir_to_probe = normalize(pp - position);
 		const V3<T> bdir = pp - biased;
-		T cu, cv, du, dv;
-		sphere_to_oct_uv(normal, cu, cv);
+		T du, dv;
 		sphere_to_oct_uv(-normalize(bdir), du, dv);
 		const V3<T> tri(off[0] ? alpha.x : T(1) - alpha.x, off[1] ? alpha.y : T(1) - alpha.y,
 						off[2] ? alpha.z : T(1) - alpha.z);
@@ -377,20 +390,92 @@ V3<T> sample_probes(const Grid& g, const uint32_t* states, const Atlas& irr, con
 		const T tw = tri.x * tri.y * tri.z + T(0.001);
 		weight *= tw;
 		fb_weight *= tw;
-		g.tile_origin(pc, cres, ox, oy);
-		T col[3];
-		bilinear_tile<T, 3>(irr, ox, oy, cres, cu, cv, col);
-		V3<T> color(std::sqrt(col[0]), std::sqrt(col[1]), std::sqrt(col[2]));
-		final_color = final_color + color * weight;
-		total_w += weight;
-		fallback_color = fallback_color + color * fb_weight;
-		total_fb += fb_weight;
+		const int k = ctx.used++;
+		g.tile_origin(pc, cres, ctx.ox[k], ctx.oy[k]);
+		// bilinear footprint of the colour tap (same arithmetic as bilinear_tile)
+		T sx = T(1) + T(cres - 2) * cu - T(0.5), sy = T(1) + T(cres - 2) * cv - T(0.5);
+		T fx = std::floor(sx), fy = std::floor(sy);
+		ctx.wx[k] = sx - fx;
+		ctx.wy[k] = sy - fy;
+		ctx.x0[k] = std::min(std::max(int(fx), 0), int(cres) - 1);
+		ctx.y0[k] = std::min(std::max(int(fy), 0), int(cres) - 1);
+		ctx.x1[k] = std::min(ctx.x0[k] + 1, int(cres) - 1);
+		ctx.y1[k] = std::min(ctx.y0[k] + 1, int(cres) - 1);
+		ctx.w[k] = weight;
+		ctx.fw[k] = fb_weight;
+		ctx.total_w += weight;
+		ctx.total_fb += fb_weight;
 	}
-	if(total_w > T(1e-3)) final_color = final_color * (T(1) / total_w);
-	if(total_fb > T(1e-3)) fallback_color = fallback_color * (T(1) / total_fb);
+}
+
+template <class T>
+static T tap_color(const SampleCtx<T>& ctx, int k, const Atlas& irr, int c) {
+	const T wx = ctx.wx[k], wy = ctx.wy[k];
+	const uint32_t ox = ctx.ox[k], oy = ctx.oy[k];
+	T t00 = T(irr.at(ox + ctx.x0[k], oy + ctx.y0[k], c)), t10 = T(irr.at(ox + ctx.x1[k], oy + ctx.y0[k], c));
+	T t01 = T(irr.at(ox + ctx.x0[k], oy + ctx.y1[k], c)), t11 = T(irr.at(ox + ctx.x1[k], oy + ctx.y1[k], c));
+	return (t00 * (T(1) - wx) + t10 * wx) * (T(1) - wy) + (t01 * (T(1) - wx) + t11 * wx) * wy;
+}
+
+template <class T>
+V3<T> sample_eval(const SampleCtx<T>& ctx, const Atlas& irr) {
+	if(ctx.outside) return {};
+	V3<T> final_color, fallback_color;
+	for(int k = 0; k < ctx.used; ++k) {
+		V3<T> color(std::sqrt(tap_color(ctx, k, irr, 0)), std::sqrt(tap_color(ctx, k, irr, 1)),
+					std::sqrt(tap_color(ctx, k, irr, 2)));
+		final_color = final_color + color * ctx.w[k];
+		fallback_color = fallback_color + color * ctx.fw[k];
+	}
+	if(ctx.total_w > T(1e-3)) final_color = final_color * (T(1) / ctx.total_w);
+	if(ctx.total_fb > T(1e-3)) fallback_color = fallback_color * (T(1) / ctx.total_fb);
 	final_color = final_color * final_color;
 	fallback_color = fallback_color * fallback_color;
-	return mix3(fallback_color, final_color, T(8) * clampT(total_w, T(0), T(1) / T(8)));
+	return mix3(fallback_color, final_color, T(8) * clampT(ctx.total_w, T(0), T(1) / T(8)));
+}
+
+// Adjoint of sample_eval with respect to the irradiance atlas: accumulates d out . g_out into
+// g_irr (image layout, 3 channels).  sqrt'(x) = 1 / (2 sqrt(x)) is evaluated at max(x, 1e-12).
+template <class T>
+void sample_adjoint(const SampleCtx<T>& ctx, const Atlas& irr, V3<T> g_out, double* g_irr) {
+	if(ctx.outside || ctx.used == 0) return;
+	const T k8 = T(8) * clampT(ctx.total_w, T(0), T(1) / T(8));
+	const T inv_w = ctx.total_w > T(1e-3) ? T(1) / ctx.total_w : T(1);
+	const T inv_fb = ctx.total_fb > T(1e-3) ? T(1) / ctx.total_fb : T(1);
+	for(int c = 0; c < 3; ++c) {
+		T s[8];
+		T fin = 0, fb = 0;
+		for(int k = 0; k < ctx.used; ++k) {
+			s[k] = std::sqrt(tap_color(ctx, k, irr, c));
+			fin += s[k] * ctx.w[k];
+			fb += s[k] * ctx.fw[k];
+		}
+		fin *= inv_w;  // normalised sqrt-space sums
+		fb *= inv_fb;
+		// out = (1 - k8) fb^2 + k8 fin^2
+		for(int k = 0; k < ctx.used; ++k) {
+			const T dout_ds = (T(1) - k8) * T(2) * fb * ctx.fw[k] * inv_fb + k8 * T(2) * fin * ctx.w[k] * inv_w;
+			const T col = s[k] * s[k];
+			const T dout_dcol = dout_ds * T(0.5) / std::sqrt(std::max(col, T(1e-12)));
+			const T gc = dout_dcol * g_out[c];
+			const T wx = ctx.wx[k], wy = ctx.wy[k];
+			auto add = [&](int x, int y, T wgt) {
+				g_irr[(uint64_t(ctx.oy[k] + y) * irr.width + ctx.ox[k] + x) * 3 + c] += double(gc * wgt);
+			};
+			add(ctx.x0[k], ctx.y0[k], (T(1) - wx) * (T(1) - wy));
+			add(ctx.x1[k], ctx.y0[k], wx * (T(1) - wy));
+			add(ctx.x0[k], ctx.y1[k], (T(1) - wx) * wy);
+			add(ctx.x1[k], ctx.y1[k], wx * wy);
+		}
+	}
+}
+
+template <class T>
+V3<T> sample_probes(const Grid& g, const uint32_t* states, const Atlas& irr, const Atlas& dep, V3<T> position,
+					V3<T> normal, V3<T> to_camera) {
+	SampleCtx<T> ctx;
+	sample_collect(g, states, dep, position, normal, to_camera, ctx);
+	return sample_eval(ctx, irr);
 }
 
 // =============================================================================================
@@ -428,13 +513,19 @@ V3<T> pbr_metallic_roughness(V3<T> n, V3<T> v, V3<T> light_color, V3<T> light_di
 // =============================================================================================
 // Closest-hit shading of one probe ray (src/shaders/closesthit.glsl:133-289, NO_REFLECTION) plus
 // the miss shader with a constant sky.  Writes (rgb, depth).
-struct ShadeAux {  // per-ray partials used by the adjoint tests (albedo / sun linearity)
-	double indirect[3];   // sampleProbes(position, normal) — multiplies albedo*(1-f0)*(1-metal)
-	double direct_d[3];   // unshadowed direct term's albedo coefficient (diffuse part / albedo)
-	double direct_s[3];   // unshadowed direct term's albedo-independent part (specular)
-	double fade;          // sun-below-horizon factor applied when lit
-	uint32_t lit;         // shadow ray missed
-	uint32_t material;    // material index of the hit (0xFFFFFFFF: miss/back face)
+// Per-ray radiance factors: for a non-metallic hit material the closest-hit radiance is affine in
+// the material albedo a and the light radiance L (the sun colour):
+//   rad = c0 + A (.) a + (B (.) a + C) (.) L
+// with the shadow visibility and the sun-below-horizon fade folded in.  Misses: c0 = sky,
+// back faces: 0.  (Metallic materials make F depend on a; the factors are only affine for
+// metallic == 0.)
+struct ShadeAux {
+	double c0[3];        // emissive + specular * reflection probes
+	double A[3];         // (1 - f0)(1 - metal) * sampleProbes(position, normal)
+	double B[3];         // lit: albedo-linear part of pbrMetallicRoughness per unit light radiance
+	double C[3];         // lit: albedo-independent part per unit light radiance
+	uint32_t lit;        // shadow ray missed
+	uint32_t material;   // material index of the hit (0xFFFFFFFF: miss / back face)
 };
 
 template <class T>
@@ -443,6 +534,11 @@ static void shade_ray(const Scene& s, const Grid& g, const Light& L, const V3<do
 	if(aux) {
 		std::memset(aux, 0, sizeof(*aux));
 		aux->material = 0xFFFFFFFFu;
+		if(!h.hit()) {
+			aux->c0[0] = sky.x;
+			aux->c0[1] = sky.y;
+			aux->c0[2] = sky.z;
+		}
 	}
 	if(!h.hit()) {  // miss.rmiss: depth -1, sky radiance
 		out[0] = T(sky.x);
@@ -531,17 +627,24 @@ static void shade_ray(const Scene& s, const Grid& g, const Light& L, const V3<do
 	if(aux) {
 		aux->material = oe.material_index;
 		aux->lit = shadowed ? 0u : 1u;
-		aux->fade = double(fade);
-		for(int c = 0; c < 3; ++c) aux->indirect[c] = double(indirect[c]);
+		const T f = shadowed ? T(1) : fade;
+		const V3<T> em(T(m.emissive[0]), T(m.emissive[1]), T(m.emissive[2]));
+		const V3<T> c0 = (em + spec * reflection) * f;
+		const V3<T> A = indirect * ((T(1) - T(0.04)) * (T(1) - metal) * f);
+		V3<T> B, Cc;
 		if(!shadowed) {
-			// split the direct term into its albedo-linear and albedo-independent parts
 			const V3<T> ldir(T(L.dir.x), T(L.dir.y), T(L.dir.z));
-			const V3<T> with = pbr_metallic_roughness<T>(normal, normalize(-d), V3<T>(L.color), ldir, albedo, metal, rough);
-			const V3<T> zero = pbr_metallic_roughness<T>(normal, normalize(-d), V3<T>(L.color), ldir, V3<T>(), metal, rough);
-			for(int c = 0; c < 3; ++c) {
-				aux->direct_s[c] = double(zero[c]);
-				aux->direct_d[c] = albedo[c] != T(0) ? double((with[c] - zero[c]) / albedo[c]) : 0.0;
-			}
+			const V3<T> one(1, 1, 1);
+			const V3<T> with = pbr_metallic_roughness<T>(normal, normalize(-d), one, ldir, one, metal, rough);
+			const V3<T> zero = pbr_metallic_roughness<T>(normal, normalize(-d), one, ldir, V3<T>(), metal, rough);
+			B = (with - zero) * f;
+			Cc = zero * f;
+		}
+		for(int c = 0; c < 3; ++c) {
+			aux->c0[c] = double(c0[c]);
+			aux->A[c] = double(A[c]);
+			aux->B[c] = double(B[c]);
+			aux->C[c] = double(Cc[c]);
 		}
 	}
 	out[0] = color.x;
@@ -1144,6 +1247,160 @@ uint32_t orc_select(const uint32_t* states, uint32_t P, uint32_t cap, uint32_t* 
 	}
 	*last_offset = idx;
 	return cnt;
+}
+
+// ---- differentiable probes: one update step as a function of (albedo[M][3], sun[3]) ----
+// rad_i = c0 + A (.) a[m_i] + (B (.) a[m_i] + C) (.) sun                  (ShadeAux factors)
+// irradiance texel = (1 - h) * sum_i w_ti rad_i / sum_i w_ti + h * prev   (probesUpdate.glsl)
+//                    then the border copy
+// out_x = sampleProbes(pos_x, n_x, v_x) on that atlas (depth atlas and states fixed)
+// shade_x = out_x (.) a[g_x] * (1 - 0.004)                               (FinalGather.frag:62-72)
+// loss = sum (shade - target)^2 / (3 N)
+// Returns the loss; with g_albedo / g_sun non-null also the exact FP64 gradient (hand-written
+// adjoint of every step).  irr_out (optional) receives the differentiated irradiance atlas.
+double orc_diff(const ddgi_grid_params* p, const uint32_t* states, const double* irr_prev, const double* dep,
+				const float* dirs, const uint32_t* indices, uint32_t count, const void* aux_v, const double* albedo,
+				uint32_t M, const double* sun, const double* pos, const double* nrm, const double* tocam,
+				const uint32_t* gmat, const double* target, uint64_t N, double* irr_out, double* shade_out,
+				double* g_albedo, double* g_sun) {
+	Grid g = make_grid(p);
+	const ShadeAux* aux = static_cast<const ShadeAux*>(aux_v);
+	const uint32_t n = p->rays_per_probe;
+	const double h = double(p->hysteresis);
+	const uint32_t W8 = g.image_width(8), H8 = g.image_height(8);
+	const size_t irr_size = size_t(W8) * H8 * 3;
+	// radiance
+	std::vector<double> rad(size_t(count) * n * 3);
+	for(size_t r = 0; r < size_t(count) * n; ++r) {
+		const ShadeAux& x = aux[r];
+		for(int c = 0; c < 3; ++c) {
+			double v = x.c0[c];
+			if(x.material != 0xFFFFFFFFu) {
+				const double a = albedo[size_t(x.material) * 3 + c];
+				v += x.A[c] * a + (x.B[c] * a + x.C[c]) * sun[c];
+			}
+			rad[r * 3 + c] = v;
+		}
+	}
+	// texel directions, weights and normalisation (per frame)
+	std::vector<double> wts(size_t(36) * n), inv(36);
+	for(int t = 0; t < 36; ++t) {
+		const V3<double> td = oct_decode<double>(texel_coord_irradiance<double>(t % 6), texel_coord_irradiance<double>(t / 6));
+		double s = 0;
+		for(uint32_t i = 0; i < n; ++i) {
+			const V3<double> rd(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]);
+			wts[size_t(t) * n + i] = std::max(0.0, dot(td, rd));
+			s += wts[size_t(t) * n + i];
+		}
+		inv[t] = s > 1e-3 ? 1.0 / s : 1.0;
+	}
+	std::vector<double> irr(irr_prev, irr_prev + irr_size);
+	for(uint32_t sl = 0; sl < count; ++sl) {
+		int gi[3];
+		g.grid_index(indices[sl], gi);
+		uint32_t ox, oy;
+		g.tile_origin(gi, 8, ox, oy);
+		for(int t = 0; t < 36; ++t) {
+			double S[3] = {0, 0, 0};
+			for(uint32_t i = 0; i < n; ++i)
+				for(int c = 0; c < 3; ++c) S[c] += wts[size_t(t) * n + i] * rad[(size_t(sl) * n + i) * 3 + c];
+			double* px = &irr[(size_t(oy + 1 + t / 6) * W8 + ox + 1 + t % 6) * 3];
+			for(int c = 0; c < 3; ++c) px[c] = (1.0 - h) * (S[c] * inv[t]) + h * px[c];
+		}
+		border_copy(irr.data(), W8, 3, ox, oy, 8);
+	}
+	if(irr_out) std::memcpy(irr_out, irr.data(), irr_size * sizeof(double));
+	Atlas A_irr{W8, H8, 3, irr.data()};
+	Atlas A_dep{g.image_width(16), g.image_height(16), 2, dep};
+	// G-buffer: forward (and per-chunk adjoint partials, reduced in chunk order)
+	const uint64_t chunk = 4096, nchunks = (N + chunk - 1) / chunk;
+	std::vector<double> loss_part(nchunks, 0.0);
+	const bool grad = g_albedo != nullptr;
+	std::vector<std::vector<double>> girr_part(grad ? nchunks : 0), galb_part(grad ? nchunks : 0);
+	ThreadPool::parallel_for(N, chunk, [&](uint64_t b, uint64_t e) {
+		const uint64_t ci = b / chunk;
+		double lsum = 0;
+		std::vector<double>* gi = nullptr;
+		std::vector<double>* ga = nullptr;
+		if(grad) {
+			girr_part[ci].assign(irr_size, 0.0);
+			galb_part[ci].assign(size_t(M) * 3, 0.0);
+			gi = &girr_part[ci];
+			ga = &galb_part[ci];
+		}
+		for(uint64_t x = b; x < e; ++x) {
+			SampleCtx<double> ctx;
+			sample_collect<double>(g, states, A_dep, V3<double>(pos[x * 3], pos[x * 3 + 1], pos[x * 3 + 2]),
+								   V3<double>(nrm[x * 3], nrm[x * 3 + 1], nrm[x * 3 + 2]),
+								   V3<double>(tocam[x * 3], tocam[x * 3 + 1], tocam[x * 3 + 2]), ctx);
+			const V3<double> out = sample_eval(ctx, A_irr);
+			const uint32_t m = gmat[x];
+			V3<double> gout;
+			for(int c = 0; c < 3; ++c) {
+				const double a = albedo[size_t(m) * 3 + c];
+				const double sh = out[c] * a * (1.0 - 0.004);
+				if(shade_out) shade_out[x * 3 + c] = sh;
+				const double diff = sh - target[x * 3 + c];
+				lsum += diff * diff;
+				if(grad) {
+					const double gs = 2.0 * diff / double(3 * N);
+					(*ga)[size_t(m) * 3 + c] += gs * out[c] * (1.0 - 0.004);
+					gout[c] = gs * a * (1.0 - 0.004);
+				}
+			}
+			if(grad) sample_adjoint(ctx, A_irr, gout, gi->data());
+		}
+		loss_part[ci] = lsum;
+	});
+	double loss = 0;
+	for(double v : loss_part) loss += v;
+	loss /= double(3 * N);
+	if(!grad) return loss;
+	std::vector<double> girr(irr_size, 0.0);
+	std::fill(g_albedo, g_albedo + size_t(M) * 3, 0.0);
+	for(uint64_t ci = 0; ci < nchunks; ++ci) {
+		for(size_t k = 0; k < irr_size; ++k) girr[k] += girr_part[ci][k];
+		for(size_t k = 0; k < size_t(M) * 3; ++k) g_albedo[k] += galb_part[ci][k];
+	}
+	for(int c = 0; c < 3; ++c) g_sun[c] = 0.0;
+	// adjoint of the border copy, hysteresis, normalisation and ray sum, then of the radiance
+	for(uint32_t sl = 0; sl < count; ++sl) {
+		int gi[3];
+		g.grid_index(indices[sl], gi);
+		uint32_t ox, oy;
+		g.tile_origin(gi, 8, ox, oy);
+		auto G = [&](uint32_t x, uint32_t y) { return &girr[(size_t(oy + y) * W8 + ox + x) * 3]; };
+		auto fold = [&](uint32_t dx, uint32_t dy, uint32_t sx, uint32_t sy) {
+			for(int c = 0; c < 3; ++c) G(sx, sy)[c] += G(dx, dy)[c];
+		};
+		const uint32_t R = 8;
+		fold(0, 0, R - 2, R - 2);
+		fold(0, R - 1, R - 2, 1);
+		fold(R - 1, 0, 1, R - 2);
+		fold(R - 1, R - 1, 1, 1);
+		for(uint32_t j = 1; j + 1 < R; ++j) {
+			fold(0, j, 1, R - 1 - j);
+			fold(R - 1, j, R - 2, R - 1 - j);
+			fold(j, 0, R - 1 - j, 1);
+			fold(j, R - 1, R - 1 - j, R - 2);
+		}
+		for(uint32_t i = 0; i < n; ++i) {
+			double grad_rad[3] = {0, 0, 0};
+			for(int t = 0; t < 36; ++t) {
+				const double* gt = G(1 + t % 6, 1 + t / 6);
+				const double k = (1.0 - h) * inv[t] * wts[size_t(t) * n + i];
+				for(int c = 0; c < 3; ++c) grad_rad[c] += k * gt[c];
+			}
+			const ShadeAux& x = aux[size_t(sl) * n + i];
+			if(x.material == 0xFFFFFFFFu) continue;
+			for(int c = 0; c < 3; ++c) {
+				const double a = albedo[size_t(x.material) * 3 + c];
+				g_albedo[size_t(x.material) * 3 + c] += grad_rad[c] * (x.A[c] + x.B[c] * sun[c]);
+				g_sun[c] += grad_rad[c] * (x.B[c] * a + x.C[c]);
+			}
+		}
+	}
+	return loss;
 }
 
 }  // extern "C"

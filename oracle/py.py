@@ -55,6 +55,7 @@ def lib():
             "orc_blend": (None, [P, P, P, P, u32, P, P, P, P, P, C.c_int]),
             "orc_sample": (None, [P, P, P, P, P, P, P, u64, P, C.c_int]),
             "orc_border_copy": (None, [P, P, u32, P, P]),
+            "orc_diff": (C.c_double, [P, P, P, P, P, P, u32, P, P, u32, P, P, P, P, P, P, u64, P, P, P, P]),
             "orc_pbr": (None, [P, P, P, P, P, C.c_double, C.c_double, P]),
             "orc_init_probes": (None, [P, P, P, P, P]),
             "orc_select": (u32, [P, u32, u32, P, P, P]),
@@ -220,8 +221,41 @@ class OracleScene:
         return states, back
 
 
-AUX_DTYPE = np.dtype([("indirect", "<f8", 3), ("direct_d", "<f8", 3), ("direct_s", "<f8", 3), ("fade", "<f8"),
+AUX_DTYPE = np.dtype([("c0", "<f8", 3), ("A", "<f8", 3), ("B", "<f8", 3), ("C", "<f8", 3),
                       ("lit", "<u4"), ("material", "<u4")])
+
+
+class DiffProblem:
+    """Inputs of the differentiable update (see orc_diff in ddgi_oracle.cpp): the per-ray factors
+    of one update, the atlases/states it leaves fixed, and a G-buffer with targets."""
+
+    def __init__(self, grid: Grid, states, irr_prev, dep, dirs, indices, aux, pos, nrm, tocam, gmat, n_materials):
+        self.grid = grid
+        self.states = np.ascontiguousarray(states, np.uint32)
+        self.irr_prev = np.ascontiguousarray(irr_prev, np.float64)
+        self.dep = np.ascontiguousarray(dep, np.float64)
+        self.dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, np.uint32)
+        self.aux = np.ascontiguousarray(aux)
+        assert self.aux.dtype == AUX_DTYPE
+        self.pos, self.nrm, self.tocam = (np.ascontiguousarray(a, np.float64).reshape(-1, 3) for a in (pos, nrm, tocam))
+        self.gmat = np.ascontiguousarray(gmat, np.uint32)
+        self.M = int(n_materials)
+        self.N = len(self.gmat)
+
+    def run(self, albedo, sun, target, grad=False):
+        albedo = np.ascontiguousarray(albedo, np.float64).reshape(self.M, 3)
+        sun = np.ascontiguousarray(sun, np.float64).reshape(3)
+        target = np.ascontiguousarray(target, np.float64).reshape(self.N, 3)
+        shade = np.zeros((self.N, 3))
+        irr = np.zeros_like(self.irr_prev)
+        ga = np.zeros((self.M, 3)) if grad else None
+        gs = np.zeros(3) if grad else None
+        loss = lib().orc_diff(_p(self.grid.params), _p(self.states), _p(self.irr_prev), _p(self.dep), _p(self.dirs),
+                              _p(self.indices), len(self.indices), _p(self.aux), _p(albedo), self.M, _p(sun),
+                              _p(self.pos), _p(self.nrm), _p(self.tocam), _p(self.gmat), _p(target), self.N,
+                              _p(irr), _p(shade), _p(ga), _p(gs))
+        return {"loss": loss, "shade": shade, "irradiance": irr, "g_albedo": ga, "g_sun": gs}
 
 
 def blend(grid: Grid, dirs: np.ndarray, rays: np.ndarray, indices, states, irr, dep, fp32=False):

@@ -83,6 +83,9 @@ _SIGNATURES = {
     "ddgi_save_state": (C.c_int, [_P, _P, C.POINTER(C.c_size_t)]),
     "ddgi_load_state": (C.c_int, [_P, _P, _SZ]),
     "ddgi_synchronize": (C.c_int, [_P]),
+    "ddgi_diff_prepare": (C.c_int, [_P, _P, _P, _P, _P, _P, _U64, _U32, _P]),
+    "ddgi_diff_forward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "ddgi_diff_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

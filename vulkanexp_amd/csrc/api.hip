@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "bvh.hpp"
+#include "diff.hpp"
 #include "kernels.hpp"
 
 namespace ddgi {
@@ -80,6 +81,23 @@ struct Handle {
 	float uploaded_hysteresis = 0.0f;  // the value the kernels use (the reference's UBO copy)
 	hipStream_t last_stream = nullptr;
 	std::string err;
+	bool has_metallic = false;  // any scene material with metallic != 0
+	uint32_t scene_materials = 0;
+	// differentiable update (ddgi_diff_*): buffers owned while prepared
+	struct Diff {
+		bool ready = false;
+		std::vector<DeviceBuffer> bufs;
+		DiffArgs d{};
+		float4* prev = nullptr;     // irradiance before the prepared update
+		float4* irr = nullptr;      // differentiated irradiance (forward output)
+		float4* rad = nullptr;
+		float* wt = nullptr;        // blend weights of the prepared frame (ray_order order)
+		float* inv_sum = nullptr;
+		float4* dirs = nullptr;
+		uint32_t* count = nullptr;  // = P (all probes)
+		float2* dep_scratch = nullptr;
+		uint32_t loss_parts = 0;
+	} diff;
 };
 
 ddgi_status fail(Handle* h, ddgi_status s, const std::string& msg) {
@@ -276,7 +294,13 @@ void image_to_tiled(const float* img, float* tiled, const Handle& h, uint32_t re
 
 Handle* H(void* p) { return static_cast<Handle*>(p); }
 
+void free_diff(Handle* h) {
+	for(auto& b : h->diff.bufs) (void)hipFree(b.p);
+	h->diff = Handle::Diff{};
+}
+
 void destroy_handle(Handle* h) {
+	free_diff(h);
 	free_scene(h);
 	for(int i = 0; i < 2; ++i) {
 		(void)hipFree(h->irr[i]);
@@ -505,10 +529,12 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	for(size_t k = 0; k < offset_count; ++k)
 		if(offsets[k].material_index >= material_count)
 			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "offset entry references a missing material");
+	bool metallic = false;
 	for(size_t k = 0; k < material_count; ++k) {
 		const ddgi_material& m = materials[k];
 		if(!std::isfinite(m.metallic) || !std::isfinite(m.roughness) || !finite_all(m.base_color, 3) || !finite_all(m.emissive, 3))
 			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "non-finite material");
+		metallic = metallic || m.metallic != 0.0f;
 	}
 	BvhHost bvh;
 	const char* why = nullptr;
@@ -571,6 +597,9 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	bvh.tris.clear();
 	h->bvh_meta = std::move(bvh);
 	h->has_scene = true;
+	h->has_metallic = metallic;
+	h->scene_materials = uint32_t(material_count);
+	free_diff(h);  // prepared factors refer to the previous scene
 	return DDGI_OK;
 }
 
@@ -603,6 +632,8 @@ ddgi_status ddgi_init_probes(void* handle, const float orientation[9], void* str
 	return DDGI_OK;
 }
 
+static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream_t s, float4* factors);
+
 ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags, void* stream) {
 	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
 	Handle* h = H(handle);
@@ -613,7 +644,11 @@ ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags
 	if(make_rotation(orientation, R) != DDGI_OK) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid orientation");
 	ddgi_status st = set_device(h);
 	if(st != DDGI_OK || (st = ensure_fib(h)) != DDGI_OK) return st;
-	hipStream_t s = static_cast<hipStream_t>(stream);
+	return run_update(h, R, flags, static_cast<hipStream_t>(stream), nullptr);
+}
+
+static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream_t s, float4* factors) {
+	ddgi_status st = DDGI_OK;
 	const bool timing = !(flags & DDGI_UPDATE_NO_TIMING);
 	const bool all = (flags & DDGI_UPDATE_ALL_PROBES) != 0;
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[0], s));
@@ -668,6 +703,7 @@ ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags
 	a.rays = h->rays;
 	a.max_slots = max_slots;
 	a.visit_counters = nullptr;
+	a.factors = factors;
 	if(flags & DDGI_UPDATE_COUNT_VISITS) {
 		HIP_TRY(h, hipMemsetAsync(h->visit, 0, 8 * sizeof(unsigned long long), s));
 		a.visit_counters = h->visit;
@@ -715,6 +751,213 @@ ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_in
 		h->timing_pending = true;
 	}
 	h->last_count = count;
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+extern "C++" {
+template <class T>
+static hipError_t diff_alloc(Handle* h, T** p, size_t bytes) {
+	DeviceBuffer b;
+	b.bytes = std::max<size_t>(bytes, 16);
+	hipError_t e = hipMalloc(&b.p, b.bytes);
+	if(e == hipSuccess) {
+		h->diff.bufs.push_back(b);
+		*p = static_cast<T*>(b.p);
+	}
+	return e;
+}
+}
+
+ddgi_status ddgi_diff_prepare(void* handle, const float orientation[9], const float* gbuf_position,
+							  const float* gbuf_normal, const float* gbuf_to_camera, const uint32_t* gbuf_material,
+							  uint64_t texel_count, uint32_t material_count, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_diff_prepare");
+	if(h->has_metallic)
+		return fail(h, DDGI_ERROR_UNSUPPORTED, "differentiation needs non-metallic materials (radiance affine in albedo)");
+	if(material_count == 0 || material_count > 1024 || material_count < h->scene_materials)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "material_count must cover the scene materials and be <= 1024");
+	if(texel_count == 0 || texel_count > (uint64_t(1) << 31) || !gbuf_position || !gbuf_normal || !gbuf_to_camera ||
+	   !gbuf_material)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "G-buffer arrays null or texel_count outside [1, 2^31]");
+	Rot R;
+	if(make_rotation(orientation, R) != DDGI_OK) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid orientation");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK || (st = ensure_fib(h)) != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	{
+		std::vector<uint32_t> mats(texel_count);
+		HIP_TRY(h, hipStreamSynchronize(s));
+		HIP_TRY(h, hipMemcpy(mats.data(), gbuf_material, texel_count * 4, hipMemcpyDeviceToHost));
+		for(uint32_t m : mats)
+			if(m >= material_count) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "G-buffer material id >= material_count");
+	}
+	free_diff(h);
+	Handle::Diff& D = h->diff;
+	const uint64_t P = h->P, n = h->params.rays_per_probe, rays = P * n, N = texel_count, M = material_count;
+	const uint32_t max_items = uint32_t(P + (N + kDiffChunk - 1) / kDiffChunk);
+	DiffArgs d{};
+	float4* factors = nullptr;
+	float *pos, *nrm, *toc, *out, *loss_part, *g_T;
+	uint32_t *gmat, *states, *indices, *cell_id, *cell_count, *cell_start, *cursor, *cell_item_start, *item_cell,
+		*item_begin, *item_count, *n_items, *sorted;
+	float2* dep;
+	long long *part_tile, *part_mat_pix, *part_mat_ray;
+	hipError_t e = diff_alloc(h, &factors, rays * 4 * sizeof(float4));
+	if(e == hipSuccess) e = diff_alloc(h, &D.rad, rays * sizeof(float4));
+	if(e == hipSuccess) e = diff_alloc(h, &D.prev, P * 64 * sizeof(float4));
+	if(e == hipSuccess) e = diff_alloc(h, &D.irr, P * 64 * sizeof(float4));
+	if(e == hipSuccess) e = diff_alloc(h, &dep, P * 256 * sizeof(float2));
+	if(e == hipSuccess) e = diff_alloc(h, &states, P * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &indices, P * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &D.wt, size_t(kMaxRays) * kBlendTexels * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &D.inv_sum, kBlendTexels * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &D.dirs, kMaxRays * sizeof(float4));
+	if(e == hipSuccess) e = diff_alloc(h, &D.count, 4);
+	if(e == hipSuccess) e = diff_alloc(h, &pos, N * 12);
+	if(e == hipSuccess) e = diff_alloc(h, &nrm, N * 12);
+	if(e == hipSuccess) e = diff_alloc(h, &toc, N * 12);
+	if(e == hipSuccess) e = diff_alloc(h, &gmat, N * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &out, N * 12);
+	D.loss_parts = uint32_t((N + 255) / 256);
+	if(e == hipSuccess) e = diff_alloc(h, &loss_part, size_t(D.loss_parts) * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &cell_id, N * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &cell_count, P * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &cell_start, (P + 1) * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &cursor, P * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &cell_item_start, (P + 1) * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &item_cell, size_t(max_items) * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &item_begin, size_t(max_items) * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &item_count, size_t(max_items) * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &n_items, 4);
+	if(e == hipSuccess) e = diff_alloc(h, &sorted, N * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &part_tile, size_t(max_items) * 8 * 192 * 8);
+	if(e == hipSuccess) e = diff_alloc(h, &part_mat_pix, size_t(max_items) * M * 3 * 8);
+	if(e == hipSuccess) e = diff_alloc(h, &g_T, P * 108 * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &part_mat_ray, P * (M * 3 + 3) * 8);
+	if(e != hipSuccess) {
+		free_diff(h);
+		return fail(h, e == hipErrorOutOfMemory ? DDGI_ERROR_OUT_OF_MEMORY : DDGI_ERROR_HIP,
+					std::string("diff allocation: ") + hipGetErrorString(e));
+	}
+	HIP_TRY(h, hipMemcpyAsync(pos, gbuf_position, N * 12, hipMemcpyDeviceToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(nrm, gbuf_normal, N * 12, hipMemcpyDeviceToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(toc, gbuf_to_camera, N * 12, hipMemcpyDeviceToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(gmat, gbuf_material, N * 4, hipMemcpyDeviceToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(D.prev, h->irr[h->cur], P * 64 * sizeof(float4), hipMemcpyDeviceToDevice, s));
+	// the prepared update: all probes, FP32 blend, shading factors recorded
+	if((st = run_update(h, R, DDGI_UPDATE_ALL_PROBES | DDGI_UPDATE_NO_TIMING, s, factors)) != DDGI_OK) {
+		free_diff(h);
+		return st;
+	}
+	HIP_TRY(h, hipMemcpyAsync(dep, h->dep[h->cur], P * 256 * sizeof(float2), hipMemcpyDeviceToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(states, h->states, P * 4, hipMemcpyDeviceToDevice, s));
+	const GridConst g = make_grid_const(*h);
+	HIP_TRY(h, launch_frame_dirs(R, h->fib, D.dirs, g.n, s));
+	HIP_TRY(h, launch_blend_weights(g, D.dirs, h->perm, D.wt, D.inv_sum, h->w_hi, h->w_lo, kBlendFP32, s));
+	{
+		std::vector<uint32_t> iota(P);
+		std::iota(iota.begin(), iota.end(), 0u);
+		const uint32_t cnt = uint32_t(P);
+		HIP_TRY(h, hipStreamSynchronize(s));
+		HIP_TRY(h, hipMemcpy(indices, iota.data(), P * 4, hipMemcpyHostToDevice));
+		HIP_TRY(h, hipMemcpy(D.count, &cnt, 4, hipMemcpyHostToDevice));
+	}
+	d.grid = g;
+	d.factors = factors;
+	d.perm = h->perm;
+	d.dirs = D.dirs;
+	d.inv_sum = D.inv_sum;
+	d.states = states;
+	d.dep = dep;
+	d.indices = indices;
+	d.slots = uint32_t(P);
+	d.rays = rays;
+	d.pos = pos;
+	d.nrm = nrm;
+	d.tocam = toc;
+	d.gmat = gmat;
+	d.N = N;
+	d.M = uint32_t(M);
+	d.sorted = sorted;
+	d.cell_item_start = cell_item_start;
+	d.item_begin = item_begin;
+	d.item_count = item_count;
+	d.n_items = n_items;
+	d.max_items = max_items;
+	d.rad = D.rad;
+	d.irr = D.irr;
+	d.out = out;
+	d.loss_part = loss_part;
+	d.part_tile = part_tile;
+	d.part_mat_pix = part_mat_pix;
+	d.g_T = g_T;
+	d.part_mat_ray = part_mat_ray;
+	HIP_TRY(h, diff_bin(d, cell_id, cell_count, cell_start, cursor, cell_item_start, item_cell, item_begin, item_count,
+						n_items, sorted, s));
+	D.d = d;
+	D.ready = true;
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_diff_forward(void* handle, const float* albedo, const float* sun, const float* target, float* shade_out,
+							  float* loss_out, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->diff.ready) return fail(h, DDGI_ERROR_INVALID_STATE, "ddgi_diff_prepare must precede ddgi_diff_forward");
+	if(!albedo || !sun || !target || !shade_out || !loss_out) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null array");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	Handle::Diff& D = h->diff;
+	DiffArgs d = D.d;
+	d.albedo = albedo;
+	d.sun = sun;
+	d.target = target;
+	d.shade = shade_out;
+	d.loss = loss_out;
+	HIP_TRY(h, diff_radiance(d, s));
+	BlendArgs b{};
+	b.grid = d.grid;
+	b.indices = d.indices;
+	b.count = D.count;
+	b.rays = D.rad;
+	b.wt = D.wt;
+	b.inv_sum = D.inv_sum;
+	b.states = nullptr;
+	b.irr_prev = D.prev;
+	b.dep_prev = d.dep;
+	b.irr_out = D.irr;
+	b.dep_out = nullptr;
+	b.max_slots = d.slots;
+	b.flags = kBlendIrradianceOnly;
+	HIP_TRY(h, launch_blend(b, kBlendFP32, s));
+	HIP_TRY(h, diff_gather_forward(d, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_diff_backward(void* handle, const float* albedo, const float* sun, const float* target,
+							   const float* grad_loss, float* grad_albedo, float* grad_sun, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->diff.ready) return fail(h, DDGI_ERROR_INVALID_STATE, "ddgi_diff_prepare must precede ddgi_diff_backward");
+	if(!albedo || !sun || !target || !grad_loss || !grad_albedo || !grad_sun)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null array");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	DiffArgs d = h->diff.d;
+	d.albedo = albedo;
+	d.sun = sun;
+	d.target = target;
+	d.grad_loss = grad_loss;
+	d.g_albedo = grad_albedo;
+	d.g_sun = grad_sun;
+	HIP_TRY(h, diff_backward(d, s));
 	h->last_stream = s;
 	return DDGI_OK;
 }

@@ -12,7 +12,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
-#include "kernels.hpp"
+#include "device_common.hpp"
 
 namespace ddgi {
 
@@ -21,34 +21,6 @@ namespace {
 constexpr int kBlendThreads = 256;  // one thread per texel (232 used)
 constexpr int kProbesPerBlock = 16;
 constexpr int kRayChunk = 32;
-
-// octDecode(specializedNormalizeLocalTexelCoord(c)) in FP32 (irradiance.glsl:107-112, probesUpdate.glsl:33-42)
-__device__ __forceinline__ void texel_direction(uint32_t t, float& x, float& y, float& z) {
-	float u, v;
-	if(t < kIrrTexels) {
-		const int cx = int(t % 6u), cy = int(t / 6u);
-		u = 0.33333f * (float(cx) - 2.5f);
-		v = 0.33333f * (float(cy) - 2.5f);
-	} else {
-		const uint32_t d = t - kIrrTexels;
-		const int cx = int(d % 14u), cy = int(d / 14u);
-		u = 0.142857f * (float(cx) - 6.5f);
-		v = 0.142857f * (float(cy) - 6.5f);
-	}
-	x = u;
-	y = v;
-	z = 1.0f - fabsf(u) - fabsf(v);
-	if(z < 0.0f) {
-		const float nx = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
-		const float ny = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
-		x = nx;
-		y = ny;
-	}
-	const float l = sqrtf(x * x + y * y + z * z);
-	x /= l;
-	y /= l;
-	z /= l;
-}
 
 // One block per padded weight row (256 rows: irradiance texels at rows 0..35, depth texels at rows
 // 48..243, the rest zero): the weight of every ray, their sum, and for the MFMA blends the 16-bit
@@ -252,7 +224,8 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 			}
 		}
 	}
-	if(active) {
+	const bool irr_only = (a.flags & kBlendIrradianceOnly) != 0u;
+	if(active && !(is_depth && irr_only)) {
 		const float inv = a.inv_sum[t];
 #pragma unroll
 		for(int p = 0; p < kProbesPerBlock; ++p) {
@@ -264,7 +237,7 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 		}
 	}
 	__syncthreads();
-	if(t < np) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
+	if(t < np && !irr_only) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -212,6 +212,36 @@ __device__ __forceinline__ void oct_uv(float dx, float dy, float dz, float& u, f
 	v = oy * 0.5f + 0.5f;
 }
 
+// Direction of blend texel t: octDecode(specializedNormalizeLocalTexelCoord(c)) in FP32
+// (irradiance.glsl:107-112, probesUpdate.glsl:33-42, with the reference's constants 0.33333 and
+// 0.142857); t < 36: irradiance texel (x, y) = (t % 6, t / 6), else depth texel t - 36 of 14x14.
+__device__ __forceinline__ void texel_direction(uint32_t t, float& x, float& y, float& z) {
+	float u, v;
+	if(t < kIrrTexels) {
+		const int cx = int(t % 6u), cy = int(t / 6u);
+		u = 0.33333f * (float(cx) - 2.5f);
+		v = 0.33333f * (float(cy) - 2.5f);
+	} else {
+		const uint32_t d = t - kIrrTexels;
+		const int cx = int(d % 14u), cy = int(d / 14u);
+		u = 0.142857f * (float(cx) - 6.5f);
+		v = 0.142857f * (float(cy) - 6.5f);
+	}
+	x = u;
+	y = v;
+	z = 1.0f - fabsf(u) - fabsf(v);
+	if(z < 0.0f) {
+		const float nx = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+		const float ny = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+		x = nx;
+		y = ny;
+	}
+	const float l = sqrtf(x * x + y * y + z * z);
+	x /= l;
+	y /= l;
+	z /= l;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Bilinear fetch inside one probe tile (texel coords 1 + (res-2)*uv, centres at k + 0.5).
 template <int RES>
@@ -241,6 +271,52 @@ __device__ __forceinline__ Vec3 normalize3(Vec3 a) {
 	return {a.x / l, a.y / l, a.z / l};
 }
 
+// Weight (visibility x backface x trilinear) and fallback weight of cage probe i of a
+// sampleProbes call (irradiance.glsl:164-206); false when the probe is off-grid or disabled.
+__device__ __forceinline__ bool cage_probe(const GridConst& g, const uint32_t* __restrict__ states,
+										   const float2* __restrict__ dep, Vec3 position, Vec3 normal, Vec3 biased,
+										   Vec3 emin, Vec3 alpha, int f0, int f1, int f2, int i, uint32_t& lin,
+										   float& weight, float& fbw) {
+	const int o0 = i & 1, o1 = (i >> 1) & 1, o2 = (i >> 2) & 1;
+	const int p0i = f0 + o0, p1i = f1 + o1, p2i = f2 + o2;
+	if(p0i > g.res[0] - 1 || p1i > g.res[1] - 1 || p2i > g.res[2] - 1) return false;
+	lin = uint32_t(p0i) + uint32_t(g.res[0]) * (uint32_t(p1i) + uint32_t(g.res[1]) * uint32_t(p2i));
+	if(states[lin] == 0u) return false;
+	const Vec3 pp = v3(float(p0i) * g.cell[0] + emin.x, float(p1i) * g.cell[1] + emin.y, float(p2i) * g.cell[2] + emin.z);
+	const Vec3 dtp = normalize3(pp - position);
+	const Vec3 bdir = pp - biased;
+	const Vec3 nb = normalize3(bdir);
+	float du, dv;
+	oct_uv(-nb.x, -nb.y, -nb.z, du, dv);
+	const float trx = o0 ? alpha.x : 1.0f - alpha.x;
+	const float try_ = o1 ? alpha.y : 1.0f - alpha.y;
+	const float trz = o2 ? alpha.z : 1.0f - alpha.z;
+	weight = 1.0f;
+	const float bfw = fmaxf(0.0001f, (dot3(dtp, normal) + 1.0f) * 0.5f);
+	weight *= bfw * bfw + 0.2f;
+	fbw = weight;
+	// Chebyshev visibility from the bilinear depth moments
+	int dx0, dy0, dx1, dy1;
+	float dwx, dwy;
+	tile_coords<16>(du, dv, dx0, dy0, dx1, dy1, dwx, dwy);
+	const float2* dt = dep + size_t(lin) * 256;
+	const float2 m00 = dt[dy0 * 16 + dx0], m10 = dt[dy0 * 16 + dx1], m01 = dt[dy1 * 16 + dx0], m11 = dt[dy1 * 16 + dx1];
+	const float mean = (m00.x * (1.0f - dwx) + m10.x * dwx) * (1.0f - dwy) + (m01.x * (1.0f - dwx) + m11.x * dwx) * dwy;
+	const float m2 = (m00.y * (1.0f - dwx) + m10.y * dwx) * (1.0f - dwy) + (m01.y * (1.0f - dwx) + m11.y * dwx) * dwy;
+	const float variance = fabsf(mean * mean - m2);
+	const float bdist = sqrtf(dot3(bdir, bdir));
+	const float dm = fmaxf(bdist - mean, 0.0001f);
+	float cheb = variance / (variance + dm * dm);
+	cheb = fmaxf(cheb * cheb * cheb, 0.0f);
+	weight *= (bdist <= mean) ? 1.0f : cheb;
+	weight = fmaxf(0.000001f, weight);
+	if(weight < 0.2f) weight *= weight * weight * (1.0f / (0.2f * 0.2f));
+	const float trw = trx * try_ * trz + 0.001f;
+	weight *= trw;
+	fbw *= trw;
+	return true;
+}
+
 // sampleProbes (src/shaders/irradiance.glsl:145-237) on the tiled FP32 atlases.
 __device__ __forceinline__ Vec3 sample_probes(const GridConst& g, const uint32_t* __restrict__ states,
 											  const float4* __restrict__ irr, const float2* __restrict__ dep,
@@ -262,43 +338,9 @@ __device__ __forceinline__ Vec3 sample_probes(const GridConst& g, const uint32_t
 	Vec3 fin = v3(0, 0, 0), fb = v3(0, 0, 0);
 	float tw = 0.0f, tfb = 0.0f;
 	for(int i = 0; i < 8; ++i) {
-		const int o0 = i & 1, o1 = (i >> 1) & 1, o2 = (i >> 2) & 1;
-		const int p0i = f0 + o0, p1i = f1 + o1, p2i = f2 + o2;
-		if(p0i > g.res[0] - 1 || p1i > g.res[1] - 1 || p2i > g.res[2] - 1) continue;
-		const uint32_t lin = uint32_t(p0i) + uint32_t(g.res[0]) * (uint32_t(p1i) + uint32_t(g.res[1]) * uint32_t(p2i));
-		if(states[lin] == 0u) continue;
-		const Vec3 pp = v3(float(p0i) * g.cell[0] + emin.x, float(p1i) * g.cell[1] + emin.y, float(p2i) * g.cell[2] + emin.z);
-		const Vec3 dtp = normalize3(pp - position);
-		const Vec3 bdir = pp - biased;
-		const Vec3 nb = normalize3(bdir);
-		float du, dv;
-		oct_uv(-nb.x, -nb.y, -nb.z, du, dv);
-		const float trx = o0 ? alpha.x : 1.0f - alpha.x;
-		const float try_ = o1 ? alpha.y : 1.0f - alpha.y;
-		const float trz = o2 ? alpha.z : 1.0f - alpha.z;
-		float weight = 1.0f;
-		const float bfw = fmaxf(0.0001f, (dot3(dtp, normal) + 1.0f) * 0.5f);
-		weight *= bfw * bfw + 0.2f;
-		float fbw = weight;
-		// Chebyshev visibility from the bilinear depth moments
-		int dx0, dy0, dx1, dy1;
-		float dwx, dwy;
-		tile_coords<16>(du, dv, dx0, dy0, dx1, dy1, dwx, dwy);
-		const float2* dt = dep + size_t(lin) * 256;
-		const float2 m00 = dt[dy0 * 16 + dx0], m10 = dt[dy0 * 16 + dx1], m01 = dt[dy1 * 16 + dx0], m11 = dt[dy1 * 16 + dx1];
-		const float mean = (m00.x * (1.0f - dwx) + m10.x * dwx) * (1.0f - dwy) + (m01.x * (1.0f - dwx) + m11.x * dwx) * dwy;
-		const float m2 = (m00.y * (1.0f - dwx) + m10.y * dwx) * (1.0f - dwy) + (m01.y * (1.0f - dwx) + m11.y * dwx) * dwy;
-		const float variance = fabsf(mean * mean - m2);
-		const float bdist = sqrtf(dot3(bdir, bdir));
-		const float dm = fmaxf(bdist - mean, 0.0001f);
-		float cheb = variance / (variance + dm * dm);
-		cheb = fmaxf(cheb * cheb * cheb, 0.0f);
-		weight *= (bdist <= mean) ? 1.0f : cheb;
-		weight = fmaxf(0.000001f, weight);
-		if(weight < 0.2f) weight *= weight * weight * (1.0f / (0.2f * 0.2f));
-		const float trw = trx * try_ * trz + 0.001f;
-		weight *= trw;
-		fbw *= trw;
+		uint32_t lin;
+		float weight, fbw;
+		if(!cage_probe(g, states, dep, position, normal, biased, emin, alpha, f0, f1, f2, i, lin, weight, fbw)) continue;
 		const float4* ct = irr + size_t(lin) * 64;
 		const float4 c00 = ct[cy0 * 8 + cx0], c10 = ct[cy0 * 8 + cx1], c01 = ct[cy1 * 8 + cx0], c11 = ct[cy1 * 8 + cx1];
 		const float cr = (c00.x * (1.0f - cwx) + c10.x * cwx) * (1.0f - cwy) + (c01.x * (1.0f - cwx) + c11.x * cwx) * cwy;
@@ -316,6 +358,133 @@ __device__ __forceinline__ Vec3 sample_probes(const GridConst& g, const uint32_t
 	fb = fb * fb;
 	const float k = 8.0f * fminf(fmaxf(tw, 0.0f), 1.0f / 8.0f);
 	return v3(fb.x * (1.0f - k) + fin.x * k, fb.y * (1.0f - k) + fin.y * k, fb.z * (1.0f - k) + fin.z * k);
+}
+
+// The irradiance-independent part of a sampleProbes call, split out for the adjoint: per cage
+// slot i (0..7) the probe, its weight and fallback weight; the colour tap footprint (the same
+// tile coordinates for all 8 probes: they depend on the normal only); the weight totals.
+struct SampleCollect {
+	uint32_t mask;  // bit i: cage probe i contributes
+	uint32_t lin[8];
+	float w[8], fw[8];
+	int cx0, cy0, cx1, cy1;
+	float cwx, cwy, tw, tfb;
+	bool outside;
+};
+
+__device__ __forceinline__ void sample_collect(const GridConst& g, const uint32_t* __restrict__ states,
+											   const float2* __restrict__ dep, Vec3 position, Vec3 normal,
+											   Vec3 to_camera, SampleCollect& c) {
+	c.mask = 0;
+	c.tw = c.tfb = 0.0f;
+	c.outside = true;
+	const Vec3 acell = v3(fabsf(g.cell[0]), fabsf(g.cell[1]), fabsf(g.cell[2]));
+	const Vec3 emin = v3(g.emin[0], g.emin[1], g.emin[2]);
+	const Vec3 gc = v3((position.x - emin.x) / acell.x, (position.y - emin.y) / acell.y, (position.z - emin.z) / acell.z);
+	if(gc.x < 0.0f || gc.y < 0.0f || gc.z < 0.0f) return;
+	c.outside = false;
+	const Vec3 biased = position + (normal + to_camera) * g.shadow_bias;
+	const int f0 = int(gc.x), f1 = int(gc.y), f2 = int(gc.z);
+	const Vec3 p0 = v3(float(f0) * g.cell[0] + emin.x, float(f1) * g.cell[1] + emin.y, float(f2) * g.cell[2] + emin.z);
+	Vec3 alpha = v3((position.x - p0.x) / acell.x, (position.y - p0.y) / acell.y, (position.z - p0.z) / acell.z);
+	alpha = v3(fminf(fmaxf(alpha.x, 0.0f), 1.0f), fminf(fmaxf(alpha.y, 0.0f), 1.0f), fminf(fmaxf(alpha.z, 0.0f), 1.0f));
+	float cu, cv;
+	oct_uv(normal.x, normal.y, normal.z, cu, cv);
+	tile_coords<8>(cu, cv, c.cx0, c.cy0, c.cx1, c.cy1, c.cwx, c.cwy);
+#pragma unroll
+	for(int i = 0; i < 8; ++i) {
+		c.lin[i] = 0;
+		c.w[i] = c.fw[i] = 0.0f;
+		uint32_t lin;
+		float w, fw;
+		if(!cage_probe(g, states, dep, position, normal, biased, emin, alpha, f0, f1, f2, i, lin, w, fw)) continue;
+		c.mask |= 1u << i;
+		c.lin[i] = lin;
+		c.w[i] = w;
+		c.fw[i] = fw;
+		c.tw += w;
+		c.tfb += fw;
+	}
+}
+
+__device__ __forceinline__ float tap3(const float4* __restrict__ irr, const SampleCollect& c, uint32_t lin, int ch) {
+	const float4* ct = irr + size_t(lin) * 64;
+	const float4 c00 = ct[c.cy0 * 8 + c.cx0], c10 = ct[c.cy0 * 8 + c.cx1], c01 = ct[c.cy1 * 8 + c.cx0],
+				 c11 = ct[c.cy1 * 8 + c.cx1];
+	const float a00 = ch == 0 ? c00.x : (ch == 1 ? c00.y : c00.z), a10 = ch == 0 ? c10.x : (ch == 1 ? c10.y : c10.z);
+	const float a01 = ch == 0 ? c01.x : (ch == 1 ? c01.y : c01.z), a11 = ch == 0 ? c11.x : (ch == 1 ? c11.y : c11.z);
+	return (a00 * (1.0f - c.cwx) + a10 * c.cwx) * (1.0f - c.cwy) + (a01 * (1.0f - c.cwx) + a11 * c.cwx) * c.cwy;
+}
+
+__device__ __forceinline__ Vec3 sample_eval(const SampleCollect& c, const float4* __restrict__ irr) {
+	if(c.outside) return v3(0.0f, 0.0f, 0.0f);
+	float out[3];
+#pragma unroll
+	for(int ch = 0; ch < 3; ++ch) {
+		float fin = 0.0f, fb = 0.0f;
+#pragma unroll
+		for(int i = 0; i < 8; ++i) {
+			if(!(c.mask & (1u << i))) continue;
+			const float s = sqrtf(tap3(irr, c, c.lin[i], ch));
+			fin += s * c.w[i];
+			fb += s * c.fw[i];
+		}
+		if(c.tw > 1e-3f) fin *= 1.0f / c.tw;
+		if(c.tfb > 1e-3f) fb *= 1.0f / c.tfb;
+		const float k = 8.0f * fminf(fmaxf(c.tw, 0.0f), 1.0f / 8.0f);
+		out[ch] = fb * fb * (1.0f - k) + fin * fin * k;
+	}
+	return v3(out[0], out[1], out[2]);
+}
+
+// Order-independent (hence deterministic) accumulation: contributions are added as 64-bit
+// fixed-point integers with 2^-50 resolution.
+constexpr double kFixScale = 1125899906842624.0;  // 2^50
+__device__ __forceinline__ long long to_fixed(float x) {
+	const float s = x * 1125899906842624.0f;
+	const float lim = 9.0e18f;
+	return __float2ll_rn(fminf(fmaxf(s, -lim), lim));
+}
+
+// Adjoint of sample_eval with respect to the irradiance atlas.  acc[i*192 + texel*3 + ch] is the
+// fixed-point gradient accumulator of cage slot i's tile (64 texels x rgb).  sqrt'(x) is taken at
+// max(x, 1e-12).
+__device__ __forceinline__ void sample_adjoint(const SampleCollect& c, const float4* __restrict__ irr, Vec3 g_out,
+											   unsigned long long* acc) {
+	if(c.outside || c.mask == 0u) return;
+	const float k = 8.0f * fminf(fmaxf(c.tw, 0.0f), 1.0f / 8.0f);
+	const float inv_w = c.tw > 1e-3f ? 1.0f / c.tw : 1.0f;
+	const float inv_fb = c.tfb > 1e-3f ? 1.0f / c.tfb : 1.0f;
+	const float b00 = (1.0f - c.cwx) * (1.0f - c.cwy), b10 = c.cwx * (1.0f - c.cwy), b01 = (1.0f - c.cwx) * c.cwy,
+				b11 = c.cwx * c.cwy;
+	const int t00 = c.cy0 * 8 + c.cx0, t10 = c.cy0 * 8 + c.cx1, t01 = c.cy1 * 8 + c.cx0, t11 = c.cy1 * 8 + c.cx1;
+#pragma unroll
+	for(int ch = 0; ch < 3; ++ch) {
+		const float go = ch == 0 ? g_out.x : (ch == 1 ? g_out.y : g_out.z);
+		float s[8];
+		float fin = 0.0f, fb = 0.0f;
+#pragma unroll
+		for(int i = 0; i < 8; ++i) {
+			s[i] = 0.0f;
+			if(!(c.mask & (1u << i))) continue;
+			s[i] = sqrtf(tap3(irr, c, c.lin[i], ch));
+			fin += s[i] * c.w[i];
+			fb += s[i] * c.fw[i];
+		}
+		fin *= inv_w;
+		fb *= inv_fb;
+#pragma unroll
+		for(int i = 0; i < 8; ++i) {
+			if(!(c.mask & (1u << i))) continue;
+			const float dds = (1.0f - k) * 2.0f * fb * c.fw[i] * inv_fb + k * 2.0f * fin * c.w[i] * inv_w;
+			const float gcol = go * dds * 0.5f / sqrtf(fmaxf(s[i] * s[i], 1e-12f));
+			unsigned long long* a = acc + i * 192 + ch;
+			atomicAdd(a + t00 * 3, (unsigned long long)to_fixed(gcol * b00));
+			atomicAdd(a + t10 * 3, (unsigned long long)to_fixed(gcol * b10));
+			atomicAdd(a + t01 * 3, (unsigned long long)to_fixed(gcol * b01));
+			atomicAdd(a + t11 * 3, (unsigned long long)to_fixed(gcol * b11));
+		}
+	}
 }
 
 // pbrMetallicRoughness (src/shaders/pbrMetallicRoughness.glsl:43-84)

@@ -83,6 +83,9 @@ struct TraceFrameArgs {
 	// optional (statistics runs only): [0]/[1] primary node / triangle fetches, [2]/[3] shadow
 	// node / triangle fetches, [4] shadow rays
 	unsigned long long* visit_counters;
+	// optional (differentiable prepare): 4 float4 per ray, rad = c0 + A a + (B a + C) L:
+	// (c0, material bits), (A, lit bits), (B, 0), (C, 0); material 0xFFFFFFFF for misses/back faces
+	float4* factors;
 };
 
 struct SelectState {
@@ -107,7 +110,9 @@ struct BlendArgs {
 	float4* irr_out;
 	float2* dep_out;
 	uint32_t max_slots;
+	uint32_t flags;             // kBlendIrradianceOnly: no depth stores, no state update
 };
+constexpr uint32_t kBlendIrradianceOnly = 0x1;
 
 // ---- launchers (return hipGetLastError of the launch) ----
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s);

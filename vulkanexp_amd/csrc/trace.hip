@@ -74,11 +74,21 @@ __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 	const uint32_t code = __float_as_uint(hit.y);
 	if(code == 0xFFFFFFFFu) {  // miss.rmiss: sky radiance, depth -1
 		a.rays[ri] = make_float4(a.light.sky[0], a.light.sky[1], a.light.sky[2], -1.0f);
+		if(a.factors) {
+			float4* f = a.factors + ri * 4;
+			f[0] = make_float4(a.light.sky[0], a.light.sky[1], a.light.sky[2], __uint_as_float(0xFFFFFFFFu));
+			f[1] = f[2] = f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
 		return;
 	}
 	const float t = hit.x;
 	if(code & 0x80000000u) {  // back face: closesthit.glsl:137-141
 		a.rays[ri] = make_float4(0.0f, 0.0f, 0.0f, t * 0.8f);
+		if(a.factors) {
+			float4* f = a.factors + ri * 4;
+			f[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
+			f[1] = f[2] = f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
 		return;
 	}
 	const uint32_t tslot = code & 0x7FFFFFFFu;
@@ -131,16 +141,38 @@ __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 		atomicAdd(&a.visit_counters[3], (unsigned long long)nt);
 		atomicAdd(&a.visit_counters[4], 1ull);
 	}
-	if(sh.slot == 0xFFFFFFFFu) {
+	const bool lit = sh.slot == 0xFFFFFFFFu;
+	float fade = 1.0f;
+	if(lit) {
 		const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
 		color = color + pbr_metallic_roughness(normal, normalize3(to_cam), v3(a.light.color[0], a.light.color[1], a.light.color[2]),
 											   L, albedo, metal, rough);
 		if(L.y < 0.0f) {
-			const float fade = 1.0f - fminf(fmaxf(-L.y, 0.0f), 0.1f) / 0.1f;
+			fade = 1.0f - fminf(fmaxf(-L.y, 0.0f), 0.1f) / 0.1f;
 			color = color * fade;
 		}
 	}
 	a.rays[ri] = make_float4(color.x, color.y, color.z, t);
+	if(a.factors) {
+		// affine decomposition in (albedo, light radiance) for non-metallic materials
+		const Vec3 em = v3(m[5], m[6], m[7]);
+		const Vec3 c0 = (em + spec * reflection) * fade;
+		const Vec3 A = indirect * ((1.0f - f0) * (1.0f - metal) * fade);
+		Vec3 B = v3(0.0f, 0.0f, 0.0f), C = v3(0.0f, 0.0f, 0.0f);
+		if(lit) {
+			const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
+			const Vec3 one = v3(1.0f, 1.0f, 1.0f), zero = v3(0.0f, 0.0f, 0.0f);
+			const Vec3 with = pbr_metallic_roughness(normal, normalize3(to_cam), one, L, one, metal, rough);
+			const Vec3 without = pbr_metallic_roughness(normal, normalize3(to_cam), one, L, zero, metal, rough);
+			B = (with - without) * fade;
+			C = without * fade;
+		}
+		float4* f = a.factors + ri * 4;
+		f[0] = make_float4(c0.x, c0.y, c0.z, __uint_as_float(mat));
+		f[1] = make_float4(A.x, A.y, A.z, __uint_as_float(lit ? 1u : 0u));
+		f[2] = make_float4(B.x, B.y, B.z, 0.0f);
+		f[3] = make_float4(C.x, C.y, C.z, 0.0f);
+	}
 }
 
 // ---- arbitrary ray queries (public ddgi_hit records) ----
