@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blend", choices=["fp32", "bf16", "fp16"], default="fp32")
+    ap.add_argument("--no-blend-compare", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 3 elementwise copies of 256 MiB (torch.add(x, 0)) so that profiler byte "
                          "counters can be scaled against a kernel of known traffic")
@@ -68,6 +70,85 @@ def cpu_baseline(scene, grid, light, sky, R, n_probes, threads):
             "sample": f"{n_probes} of {og.count} probes x {n} rays, one frame (trace+shade+shadow+blend+border), "
                       f"FP64 oracle, {used} threads, median of 3 = {t * 1e3:.0f} ms",
             "update_ms_extrapolated": round(og.count * n / (rate * 1e6) * 1e3, 1)}
+
+
+def _event_ms(fn, reps=10):
+    import torch
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    return float(np.median(times))
+
+
+def blend_compare(probes, R, lib):
+    """FP32 / BF16 / FP16 blends of one ray buffer (16,384 probes x 256 rays): kernel time (the
+    blend phase of the update: directions, weights, blend) and error of BF16/FP16 against FP32."""
+    import torch
+    state = probes.save_state()
+    probes.update(R, lib.UPDATE_ALL_PROBES)
+    probes.synchronize()
+    rays, idx = probes.last_update_rays()
+    t_rays = torch.from_numpy(rays).cuda()
+    t_idx = torch.from_numpy(idx.view(np.int32).copy()).cuda()
+    out = {}
+    ref = None
+    for mode, flag in (("fp32", 0), ("bf16", lib.UPDATE_BLEND_BF16), ("fp16", lib.UPDATE_BLEND_FP16)):
+        ms = []
+        for _ in range(10):
+            probes.load_state(state)
+            probes.blend_rays(t_rays, t_idx, R, flag)
+            ms.append(probes.timings()["blend_ms"])
+        probes.synchronize()
+        irr = probes.irradiance(lib.LAYOUT_TILED)[..., :3].astype(np.float64)
+        if ref is None:
+            ref = irr
+        err = np.abs(irr - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+        out[mode] = {"blend_ms": round(float(np.median(ms)), 4), "max_rel_err_vs_fp32": float(f"{err.max():.3e}"),
+                     "mean_rel_err_vs_fp32": float(f"{err.mean():.3e}")}
+    probes.load_state(state)
+    out["bf16_speedup_vs_fp32"] = round(out["fp32"]["blend_ms"] / out["bf16"]["blend_ms"], 2)
+    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["blend_ms"] / out["fp16"]["blend_ms"], 2)
+    return out
+
+
+def diff_bench(lib):
+    """Differentiable probes: room + boxes scene (open top), 8x4x8 probes x 128 rays, a 1920x1080
+    G-buffer traced on the GPU, 256 albedo slots + sun; forward and backward time."""
+    import torch
+    from vulkanexp_amd import IrradianceProbes, gbuffer, scenes
+    from vulkanexp_amd.autograd import DiffContext
+    scene, grid = scenes.config_small(seed=1, open_top=True)
+    light = scenes.sun_light()
+    p = IrradianceProbes(grid, max_rays_per_probe=128)
+    p.set_scene(scene)
+    p.set_light(light)
+    p.set_sky(scenes.SKY_RADIANCE)
+    Rs = scenes.orientations(41, 4)
+    for R in Rs[:3]:
+        p.update(R, lib.UPDATE_ALL_PROBES)
+    rays = gbuffer.camera_rays(1920, 1080, eye=(2.0, 7.0, 2.0), look_at=(12.0, 1.0, 12.0))
+    hits = gbuffer.hits_from_device(p.trace_rays(torch.from_numpy(rays).cuda()).cpu().numpy())
+    gb = gbuffer.build(scene, rays, hits)
+    t = lambda a, dt=torch.float32: torch.from_numpy(np.ascontiguousarray(a)).cuda().to(dt)
+    dctx = DiffContext(p, Rs[3], t(gb["position"]), t(gb["normal"]), t(gb["to_camera"]),
+                       t(gb["material"].astype(np.int32), torch.int32), 256)
+    albedo, albedo_t, sun, sun_t = gbuffer.perturbed_params(scene.materials["base_color"], light["color"][:3], seed=5)
+    A, S = t(albedo), t(sun)
+    _, target = dctx.forward(t(albedo_t), t(sun_t), torch.zeros((dctx.N, 3), device="cuda"))
+    target = target.clone()
+    one = torch.ones(1, device="cuda")
+    dctx.forward(A, S, target)
+    dctx.backward(A, S, target, one)
+    fwd = _event_ms(lambda: dctx.forward(A, S, target))
+    bwd = _event_ms(lambda: dctx.backward(A, S, target, one))
+    p.destroy()
+    return {"gbuffer_texels": int(dctx.N), "params": 256 * 3 + 3, "forward_ms": round(fwd, 4),
+            "backward_ms": round(bwd, 4), "bwd_over_fwd": round(bwd / fwd, 3)}
 
 
 def main():
@@ -144,6 +225,9 @@ def main():
     trace_stats = {"nodes_per_primary_ray": st["primary_nodes"] / rays, "tris_per_primary_ray": st["primary_tris"] / rays,
                    "shadow_rays": st["shadow_rays"], "nodes_per_shadow_ray": st["shadow_nodes"] / max(1, st["shadow_rays"])}
 
+    blend = None if args.no_blend_compare else blend_compare(probes, Rs[-1], _lib)
+    diff = None if args.no_diff else diff_bench(_lib)
+
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -167,6 +251,7 @@ def main():
         "phases_ms": {k: round(v, 4) for k, v in avg.items()},
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
+        "blend_precision": blend, "differentiable": diff,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
