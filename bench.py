@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--blend", choices=["fp32", "bf16", "fp16"], default="fp32")
     ap.add_argument("--no-blend-compare", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 3 elementwise copies of 256 MiB (torch.add(x, 0)) so that profiler byte "
                          "counters can be scaled against a kernel of known traffic")
@@ -151,6 +152,81 @@ def diff_bench(lib):
             "backward_ms": round(bwd, 4), "bwd_over_fwd": round(bwd / fwd, 3)}
 
 
+def batch_bench(lib, frames=5):
+    """Batched volumes: 24 volumes over 6 scenes (780 .. 262,144 triangles), grids 4x4x4 ..
+    32x16x32, 64..512 rays per probe (83,859 probes, 24.1 M rays per update).  One graph launch per
+    update (after the autotuner picked its configuration) against the same updates issued as a
+    sequence of per-volume ddgi_update calls on one stream."""
+    import torch
+    from vulkanexp_amd import IrradianceProbes, ProbeBatch, scenes
+    scene_list, specs = scenes.batched_config(seed=5)
+    light = scenes.sun_light()
+    vols = []
+    for si, g in specs:
+        p = IrradianceProbes(g, max_rays_per_probe=int(g["rays_per_probe"]))
+        p.set_scene(scene_list[si])
+        p.set_light(light)
+        p.set_sky(scenes.SKY_RADIANCE)
+        vols.append(p)
+    for k, p in enumerate(vols):
+        p.init_probes(scenes.orientations(200 + k, 1)[0])
+    nv = len(vols)
+    rays = sum(p.get_probe_count() * int(p.GridParameters["rays_per_probe"]) for p in vols)
+    Rs = [np.stack([scenes.orientations(10_000 * f + v, 1)[0] for v in range(nv)]) for f in range(frames + 1)]
+    flags = lib.UPDATE_ALL_PROBES | lib.UPDATE_NO_TIMING
+    for p, R in zip(vols, Rs[0]):
+        p.update(R, flags)
+    torch.cuda.synchronize()
+    seq_total, seq_dur, seq_done = [], [], []
+    for f in range(frames):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(nv + 1)]
+        ev[0].record()
+        for v, (p, R) in enumerate(zip(vols, Rs[1 + f])):
+            p.update(R, flags)
+            ev[v + 1].record()
+        ev[-1].synchronize()
+        seq_dur += [ev[v].elapsed_time(ev[v + 1]) for v in range(nv)]
+        seq_done += [ev[0].elapsed_time(ev[v + 1]) for v in range(nv)]
+        seq_total.append(ev[0].elapsed_time(ev[-1]))
+    batch = ProbeBatch(vols)
+    batch.update(Rs[0])
+    torch.cuda.synchronize()
+    default_ms = []
+    for f in range(frames):
+        batch.update(Rs[1 + f])
+        default_ms.append(batch.timings()[0])
+    t0 = time.perf_counter()
+    best, records = batch.autotune(Rs[0], iterations=3)
+    tune_s = time.perf_counter() - t0
+    batch.update(Rs[0])
+    torch.cuda.synchronize()
+    b_total, b_done = [], []
+    for f in range(frames):
+        batch.update(Rs[1 + f])
+        total, per = batch.timings()
+        b_total.append(total)
+        b_done += list(per)
+    batch.destroy()
+    for p in vols:
+        p.destroy()
+    pct = lambda a, q: round(float(np.percentile(np.asarray(a), q)), 4)
+    seq_ms, bat_ms = float(np.median(seq_total)), float(np.median(b_total))
+    return {
+        "volumes": nv, "probes": int(sum(p for p in [int(np.prod(g["resolution"])) for _, g in specs])),
+        "rays_per_update": int(rays), "triangles": [s.triangle_count for s in scene_list],
+        "sequential": {"total_ms": round(seq_ms, 4), "Mrays/s": round(rays / seq_ms / 1e3, 1),
+                       "volume_update_ms_p50": pct(seq_dur, 50), "volume_update_ms_p99": pct(seq_dur, 99),
+                       "volume_done_ms_p50": pct(seq_done, 50), "volume_done_ms_p99": pct(seq_done, 99)},
+        "batched": {"total_ms": round(bat_ms, 4), "Mrays/s": round(rays / bat_ms / 1e3, 1),
+                    "volume_done_ms_p50": pct(b_done, 50), "volume_done_ms_p99": pct(b_done, 99),
+                    "default_config_total_ms": round(float(np.median(default_ms)), 4)},
+        "speedup_vs_sequential": round(seq_ms / bat_ms, 3),
+        "autotune": {"best": best, "seconds": round(tune_s, 2),
+                     "records": [[r["trace_block"], r["trace_item_blocks"], r["blend_probes"], round(r["ms"], 4)]
+                                 for r in records]},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -227,6 +303,7 @@ def main():
 
     blend = None if args.no_blend_compare else blend_compare(probes, Rs[-1], _lib)
     diff = None if args.no_diff else diff_bench(_lib)
+    batched = None if args.no_batch else batch_bench(_lib)
 
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -251,7 +328,7 @@ def main():
         "phases_ms": {k: round(v, 4) for k, v in avg.items()},
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
-        "blend_precision": blend, "differentiable": diff,
+        "blend_precision": blend, "differentiable": diff, "batched_volumes": batched,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

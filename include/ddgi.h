@@ -280,6 +280,49 @@ ddgi_status ddgi_load_state(void* handle, const void* host_src, size_t bytes);
 /* Block the host until all work queued on the handle's last stream is done. */
 ddgi_status ddgi_synchronize(void* handle);
 
+/* ---- many probe volumes per update ----
+ * No counterpart in the reference, which updates one IrradianceProbes per scene
+ * (src/IrradianceProbes.cpp:426-594).  A batch groups handles (same device, each with a scene, no
+ * handle twice) and updates all of them with ONE hipGraph launch: frame directions and blend
+ * weights of every volume, then persistent work-queue kernels for the primary traversal, the
+ * shading + shadow rays and the blend, each over the work items of all volumes.  The atlases,
+ * states and ray records are bit-identical to ddgi_update(h, R_h, DDGI_UPDATE_ALL_PROBES | mode)
+ * on each handle.  The handles stay usable on their own; ddgi_destroy is refused while a batch
+ * holds the handle.  Errors of the batch calls are reported by ddgi_last_error(NULL). */
+typedef struct ddgi_batch_config {
+	uint32_t trace_block;       /* traversal workgroup size: 64, 128 or 256 threads */
+	uint32_t trace_item_blocks; /* work item = trace_block * trace_item_blocks consecutive rays of one volume (1..64) */
+	uint32_t blend_probes;      /* FP32 blend tile: probes per workgroup, 4, 8, 16 or 32 (MFMA blends use 16) */
+	uint32_t reserved[5];
+} ddgi_batch_config;
+
+typedef struct ddgi_batch_tune_record {
+	ddgi_batch_config config;
+	float ms;                   /* mean time of one batched update with this configuration */
+	uint32_t phase;             /* 0: traversal candidates, 1: blend candidates */
+} ddgi_batch_tune_record;
+
+/* config NULL = defaults (128, 4, 16). */
+ddgi_status ddgi_batch_create(void* const* handles, uint32_t count, const ddgi_batch_config* config, void** out_batch);
+ddgi_status ddgi_batch_destroy(void* batch);
+ddgi_status ddgi_batch_set_config(void* batch, const ddgi_batch_config* config);
+ddgi_status ddgi_batch_get_config(void* batch, ddgi_batch_config* out);
+/* orientations: host, count * 9 floats (row-major 3x3 per volume, in handle order).  flags must
+ * contain DDGI_UPDATE_ALL_PROBES; DDGI_UPDATE_BLEND_BF16 / _FP16 / _HYSTERESIS_RAMP apply as in
+ * ddgi_update; DDGI_UPDATE_COUNT_VISITS is not supported. */
+ddgi_status ddgi_batch_update(void* batch, const float* orientations, uint32_t flags, void* stream);
+/* Times of the last batched update: total (HIP events around the graph launch) and, per volume,
+ * the time from the start of the update on the device until the volume's last blend tile ended
+ * (device wall clock).  Waits for the update. volume_ms may be NULL. */
+ddgi_status ddgi_batch_get_timings(void* batch, float* total_ms, float* volume_ms, uint32_t count);
+/* Project autotuner: times traversal configurations (workgroup size x work-item size), then FP32
+ * blend tiles, each over `iterations` real batched updates, keeps the fastest, and restores the
+ * volumes' atlases, states and hysteresis afterwards.  records (may be NULL) receives up to
+ * max_records measurements; *record_count the number written. */
+ddgi_status ddgi_batch_autotune(void* batch, const float* orientations, uint32_t flags, uint32_t iterations, void* stream,
+								ddgi_batch_config* best, ddgi_batch_tune_record* records, uint32_t max_records,
+								uint32_t* record_count);
+
 #ifdef __cplusplus
 }
 #endif

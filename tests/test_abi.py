@@ -35,7 +35,25 @@ def test_record_layouts_match_header():
     assert fields("ddgi_timings") == [f for f, _ in _lib.Timings._fields_]
     assert fields("ddgi_device_view") == [f for f, _ in _lib.DeviceView._fields_]
     assert fields("ddgi_options") == [f for f, _ in _lib.Options._fields_]
+    assert fields("ddgi_batch_config") == [f for f, _ in _lib.BatchConfig._fields_]
+    assert fields("ddgi_batch_tune_record") == [f for f, _ in _lib.BatchTuneRecord._fields_]
     assert C.sizeof(_lib.Timings) == 40 and C.sizeof(_lib.Options) == 32
+    assert C.sizeof(_lib.BatchConfig) == 32 and C.sizeof(_lib.BatchTuneRecord) == 40
+
+
+def test_batch_argument_errors_without_device():
+    lib = _lib.load()
+    out = C.c_void_p()
+    assert lib.ddgi_batch_create(None, 1, None, C.byref(out)) == 1
+    arr = (C.c_void_p * 1)(None)
+    assert lib.ddgi_batch_create(C.cast(arr, C.c_void_p), 0, None, C.byref(out)) == 1
+    assert lib.ddgi_batch_create(C.cast(arr, C.c_void_p), 1, None, C.byref(out)) == 1  # null handle
+    assert lib.ddgi_batch_create(C.cast(arr, C.c_void_p), 300, None, C.byref(out)) == 4
+    assert lib.ddgi_batch_destroy(None) == 1
+    assert lib.ddgi_batch_update(None, None, 1, None) == 1
+    assert lib.ddgi_batch_set_config(None, None) == 1
+    assert lib.ddgi_batch_get_timings(None, None, None, 0) == 1
+    assert b"null" in lib.ddgi_last_error(None)
 
 
 def test_status_strings():

@@ -2,6 +2,7 @@
 include/ddgi.h) behind a host mirror of VulkanExp's IrradianceProbes interface."""
 from . import scenes
 from ._lib import DdgiError, load
+from .batch import ProbeBatch
 from .probes import IrradianceProbes
 
-__all__ = ["IrradianceProbes", "DdgiError", "load", "scenes"]
+__all__ = ["IrradianceProbes", "ProbeBatch", "DdgiError", "load", "scenes"]

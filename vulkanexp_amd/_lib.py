@@ -54,6 +54,19 @@ class Options(C.Structure):
                 ("target_hysteresis", C.c_float), ("device", C.c_int32), ("reserved", C.c_uint32 * 4)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("trace_block", C.c_uint32), ("trace_item_blocks", C.c_uint32), ("blend_probes", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
+
+    def as_dict(self) -> dict:
+        return {"trace_block": self.trace_block, "trace_item_blocks": self.trace_item_blocks,
+                "blend_probes": self.blend_probes}
+
+
+class BatchTuneRecord(C.Structure):
+    _fields_ = [("config", BatchConfig), ("ms", C.c_float), ("phase", C.c_uint32)]
+
+
 _P = C.c_void_p
 _U32, _U64, _SZ = C.c_uint32, C.c_uint64, C.c_size_t
 _SIGNATURES = {
@@ -86,6 +99,13 @@ _SIGNATURES = {
     "ddgi_diff_prepare": (C.c_int, [_P, _P, _P, _P, _P, _P, _U64, _U32, _P]),
     "ddgi_diff_forward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "ddgi_diff_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "ddgi_batch_create": (C.c_int, [_P, _U32, _P, C.POINTER(C.c_void_p)]),
+    "ddgi_batch_destroy": (C.c_int, [_P]),
+    "ddgi_batch_set_config": (C.c_int, [_P, _P]),
+    "ddgi_batch_get_config": (C.c_int, [_P, _P]),
+    "ddgi_batch_update": (C.c_int, [_P, _P, _U32, _P]),
+    "ddgi_batch_get_timings": (C.c_int, [_P, _P, _P, _U32]),
+    "ddgi_batch_autotune": (C.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32, _P]),
 }
 
 _lib = None

@@ -16,17 +16,12 @@
 #include <string>
 #include <vector>
 
-#include "bvh.hpp"
-#include "diff.hpp"
-#include "kernels.hpp"
-
-namespace ddgi {
-hipError_t launch_set_count(SelectState* st, uint32_t count, hipStream_t s);
-}
+#include "handle.hpp"
 
 using namespace ddgi;
 
-namespace {
+namespace ddgi {
+namespace host {
 
 constexpr double kPiD = 3.1415926538;  // src/shaders/common.glsl:4
 constexpr uint32_t kStateMagic = 0x49474444u;  // "DDGI"
@@ -34,93 +29,10 @@ constexpr uint32_t kStateVersion = 1u;
 
 thread_local std::string g_last_error;
 
-struct DeviceBuffer {
-	void* p = nullptr;
-	size_t bytes = 0;
-};
-
-struct Handle {
-	ddgi_grid_params params{};
-	ddgi_options opts{};
-	int device = 0;
-	uint32_t P = 0;
-	uint32_t max_rays = 0;
-	// atlases: [cur] is exposed, [cur ^ 1] receives the next update
-	float4* irr[2] = {nullptr, nullptr};
-	float2* dep[2] = {nullptr, nullptr};
-	int cur = 0;
-	uint32_t* states = nullptr;
-	uint32_t* indices = nullptr;
-	SelectState* sel = nullptr;
-	bool indices_are_iota = false;
-	float4* rays = nullptr;
-	float4* hits = nullptr;
-	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
-	uint16_t* perm = nullptr;   // kMaxRays
-	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512
-	float4* dirs = nullptr;     // world directions, kMaxRays
-	uint32_t fib_n = 0;
-	float* wt = nullptr;
-	float* inv_sum = nullptr;
-	void* w_hi = nullptr;
-	void* w_lo = nullptr;
-	unsigned long long* visit = nullptr;  // traversal counters (statistics runs)
-	// scene
-	std::vector<DeviceBuffer> scene_bufs;
-	SceneDev sd{};
-	bool has_scene = false;
-	BvhHost bvh_meta;
-	size_t bvh_bytes = 0;
-	LightConst light{};
-	// timing
-	hipEvent_t ev[6] = {};
-	bool timing_pending = false;
-	ddgi_timings last{};
-	uint32_t last_count = 0xFFFFFFFFu;
-	uint32_t updated_probes = 0;  // hysteresis ramp bookkeeping (src/IrradianceProbes.cpp:463)
-	float uploaded_hysteresis = 0.0f;  // the value the kernels use (the reference's UBO copy)
-	hipStream_t last_stream = nullptr;
-	std::string err;
-	bool has_metallic = false;  // any scene material with metallic != 0
-	uint32_t scene_materials = 0;
-	// differentiable update (ddgi_diff_*): buffers owned while prepared
-	struct Diff {
-		bool ready = false;
-		std::vector<DeviceBuffer> bufs;
-		DiffArgs d{};
-		float4* prev = nullptr;     // irradiance before the prepared update
-		float4* irr = nullptr;      // differentiated irradiance (forward output)
-		float4* rad = nullptr;
-		float* wt = nullptr;        // blend weights of the prepared frame (ray_order order)
-		float* inv_sum = nullptr;
-		float4* dirs = nullptr;
-		uint32_t* count = nullptr;  // = P (all probes)
-		float2* dep_scratch = nullptr;
-		uint32_t loss_parts = 0;
-	} diff;
-};
-
 ddgi_status fail(Handle* h, ddgi_status s, const std::string& msg) {
 	if(h) h->err = msg;
 	g_last_error = msg;
 	return s;
-}
-
-#define HIP_TRY(h, expr)                                                                               \
-	do {                                                                                               \
-		hipError_t e_ = (expr);                                                                        \
-		if(e_ != hipSuccess) {                                                                         \
-			return fail((h), e_ == hipErrorOutOfMemory ? DDGI_ERROR_OUT_OF_MEMORY : DDGI_ERROR_HIP,    \
-						std::string(#expr) + ": " + hipGetErrorString(e_));                            \
-		}                                                                                              \
-	} while(0)
-
-template <class T>
-hipError_t dalloc(T** p, size_t bytes) {
-	void* q = nullptr;
-	hipError_t e = hipMalloc(&q, std::max<size_t>(bytes, 16));
-	*p = static_cast<T*>(q);
-	return e;
 }
 
 bool finite_all(const float* v, int n) {
@@ -292,8 +204,6 @@ void image_to_tiled(const float* img, float* tiled, const Handle& h, uint32_t re
 	}
 }
 
-Handle* H(void* p) { return static_cast<Handle*>(p); }
-
 void free_diff(Handle* h) {
 	for(auto& b : h->diff.bufs) (void)hipFree(b.p);
 	h->diff = Handle::Diff{};
@@ -348,7 +258,27 @@ ddgi_status enqueue_blend_and_publish(Handle* h, const Rot& R, const float4* ray
 	return DDGI_OK;
 }
 
-}  // namespace
+// src/IrradianceProbes.cpp:462-476 with its order: in the first branch the uniforms are uploaded
+// BEFORE the value moves, so the change reaches the kernels one ramp step later.
+void advance_hysteresis(Handle* h, uint32_t updated) {
+	if(h->updated_probes >= h->P) {
+		const float target = h->opts.target_hysteresis;
+		if(std::fabs(h->params.hysteresis - target) > 0.05f) {
+			h->uploaded_hysteresis = h->params.hysteresis;
+			h->params.hysteresis += 0.1f * (target - h->params.hysteresis);
+		} else if(h->params.hysteresis != target) {
+			h->params.hysteresis = target;
+			h->uploaded_hysteresis = target;
+		}
+		h->updated_probes -= h->P;
+	}
+	h->updated_probes += updated;
+}
+
+}  // namespace host
+}  // namespace ddgi
+
+using namespace ddgi::host;
 
 extern "C" {
 
@@ -471,6 +401,7 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 ddgi_status ddgi_destroy(void* handle) {
 	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
 	Handle* h = H(handle);
+	if(h->batch_members) return fail(h, DDGI_ERROR_INVALID_STATE, "handle is part of a batch: destroy the batch first");
 	set_device(h);
 	(void)hipDeviceSynchronize();
 	destroy_handle(h);
@@ -664,8 +595,6 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	}
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[1], s));
 	if(flags & DDGI_UPDATE_HYSTERESIS_RAMP) {
-		// src/IrradianceProbes.cpp:462-476 with its order: in the first branch the uniforms are
-		// uploaded BEFORE the value moves, so the change reaches the kernels one ramp step later
 		uint32_t cnt = h->P;
 		if(!all) {
 			SelectState ss;
@@ -673,18 +602,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 			HIP_TRY(h, hipStreamSynchronize(s));
 			cnt = ss.count;
 		}
-		if(h->updated_probes >= h->P) {
-			const float target = h->opts.target_hysteresis;
-			if(std::fabs(h->params.hysteresis - target) > 0.05f) {
-				h->uploaded_hysteresis = h->params.hysteresis;
-				h->params.hysteresis += 0.1f * (target - h->params.hysteresis);
-			} else if(h->params.hysteresis != target) {
-				h->params.hysteresis = target;
-				h->uploaded_hysteresis = target;
-			}
-			h->updated_probes -= h->P;
-		}
-		h->updated_probes += cnt;
+		advance_hysteresis(h, cnt);
 	}
 	const GridConst g = make_grid_const(*h);
 	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s));

@@ -25,12 +25,11 @@ constexpr int kRayChunk = 32;
 // One block per padded weight row (256 rows: irradiance texels at rows 0..35, depth texels at rows
 // 48..243, the rest zero): the weight of every ray, their sum, and for the MFMA blends the 16-bit
 // copies in [row][n_pad] layout (BF16: hi/lo split; FP16: one copy).
-__global__ __launch_bounds__(256) void k_blend_weights(GridConst g, const float4* __restrict__ dirs,
-													   const uint16_t* __restrict__ order, float* __restrict__ wt,
-													   float* __restrict__ inv_sum, void* __restrict__ w_hi_v,
-													   void* __restrict__ w_lo_v, uint32_t mode, uint32_t n_pad) {
+__device__ __forceinline__ void blend_weight_row(const GridConst& g, const float4* __restrict__ dirs,
+												 const uint16_t* __restrict__ order, float* __restrict__ wt,
+												 float* __restrict__ inv_sum, void* __restrict__ w_hi_v,
+												 void* __restrict__ w_lo_v, uint32_t mode, uint32_t n_pad, uint32_t row) {
 	__shared__ float red[256];
-	const uint32_t row = blockIdx.x;
 	const bool irr_row = row < kIrrTexels;
 	const bool dep_row = row >= 48u && row < 48u + kDepTexels;
 	const bool valid = irr_row || dep_row;
@@ -71,6 +70,13 @@ __global__ __launch_bounds__(256) void k_blend_weights(GridConst g, const float4
 		__syncthreads();
 	}
 	if(threadIdx.x == 0 && valid) inv_sum[t] = red[0] > 1e-3f ? 1.0f / red[0] : 1.0f;
+}
+
+__global__ __launch_bounds__(256) void k_blend_weights(GridConst g, const float4* __restrict__ dirs,
+													   const uint16_t* __restrict__ order, float* __restrict__ wt,
+													   float* __restrict__ inv_sum, void* __restrict__ w_hi_v,
+													   void* __restrict__ w_lo_v, uint32_t mode, uint32_t n_pad) {
+	blend_weight_row(g, dirs, order, wt, inv_sum, w_hi_v, w_lo_v, mode, n_pad, blockIdx.x);
 }
 
 // Border destinations mirroring interior texel (x, y) of an R x R tile (probesCopyBorders.comp
@@ -167,33 +173,36 @@ __device__ __forceinline__ void update_state(uint32_t* states, uint32_t probe, f
 	states[probe] = st;
 }
 
-// FP32 VALU blend: a block owns 16 probe slots; thread t owns texel t for all 16 probes and
-// accumulates 3 channels (rgb for irradiance, (d, d^2, 0) for depth) over ray chunks staged in LDS.
-__global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
-	__shared__ float4 s_irr[kProbesPerBlock][kRayChunk];
-	__shared__ float4 s_dep[kProbesPerBlock][kRayChunk];
-	__shared__ uint32_t s_oor[kProbesPerBlock];
-	__shared__ uint32_t s_maxc[kProbesPerBlock];
-	const uint32_t count = *a.count;
-	const uint32_t slot0 = blockIdx.x * kProbesPerBlock;
+// FP32 VALU blend: a block owns NP probe slots from slot0; thread t owns texel t for all NP probes
+// and accumulates 3 channels (rgb for irradiance, (d, d^2, 0) for depth) over ray chunks staged in
+// LDS.  Each probe's sums run over the rays in order, so the result does not depend on NP.
+template <int NP>
+__device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slot0) {
+	__shared__ float4 s_irr[NP][kRayChunk];
+	__shared__ float4 s_dep[NP][kRayChunk];
+	__shared__ uint32_t s_oor[NP];
+	__shared__ uint32_t s_maxc[NP];
+	// count and n steer loops that contain barriers: make them provably uniform (count is read
+	// through a pointer the compiler cannot prove to be global, which it would treat as divergent)
+	const uint32_t count = __builtin_amdgcn_readfirstlane(*a.count);
 	if(slot0 >= count) return;
-	const uint32_t np = min(uint32_t(kProbesPerBlock), count - slot0);
-	const uint32_t n = a.grid.n;
+	const uint32_t np = min(uint32_t(NP), count - slot0);
+	const uint32_t n = __builtin_amdgcn_readfirstlane(a.grid.n);
 	const uint32_t t = threadIdx.x;
 	const float cell_len = a.grid.cell_len;
-	if(t < kProbesPerBlock) {
+	if(t < NP) {
 		s_oor[t] = 0;
 		s_maxc[t] = 0;
 	}
-	float acc[kProbesPerBlock][3];
+	float acc[NP][3];
 #pragma unroll
-	for(int p = 0; p < kProbesPerBlock; ++p) acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
+	for(int p = 0; p < NP; ++p) acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
 	const bool is_depth = t >= kIrrTexels;
 	const bool active = t < kBlendTexels;
 	for(uint32_t i0 = 0; i0 < n; i0 += kRayChunk) {
 		__syncthreads();
 		// stage 16 probes x 32 rays
-		for(uint32_t e = t; e < kProbesPerBlock * kRayChunk; e += kBlendThreads) {
+		for(uint32_t e = t; e < NP * kRayChunk; e += kBlendThreads) {
 			const uint32_t p = e / kRayChunk, i = e % kRayChunk;
 			float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 			float4 dd = r;
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 			for(uint32_t i = 0; i < iend; ++i) {
 				const float w = a.wt[size_t(i0 + i) * kBlendTexels + t];
 #pragma unroll
-				for(int p = 0; p < kProbesPerBlock; ++p) {
+				for(int p = 0; p < NP; ++p) {
 					const float4 x = src[p][i];
 					acc[p][0] += w * x.x;
 					acc[p][1] += w * x.y;
@@ -228,7 +237,7 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 	if(active && !(is_depth && irr_only)) {
 		const float inv = a.inv_sum[t];
 #pragma unroll
-		for(int p = 0; p < kProbesPerBlock; ++p) {
+		for(int p = 0; p < NP; ++p) {
 			if(uint32_t(p) < np) {
 				float mc = 0.0f;
 				store_texel(a, t, a.indices[slot0 + p], acc[p], inv, &mc);
@@ -238,6 +247,10 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 	}
 	__syncthreads();
 	if(t < np && !irr_only) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
+}
+
+__global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
+	blend_fp32_tile<kProbesPerBlock>(a, blockIdx.x * kProbesPerBlock);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -282,7 +295,7 @@ struct MfmaTypes<kBlendFP16> {
 };
 
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) void k_blend_mfma(BlendArgs a, uint32_t n_pad) {
+__device__ __forceinline__ void blend_mfma_tile(const BlendArgs& a, uint32_t slot0, uint32_t n_pad) {
 	typedef MfmaTypes<MODE> M;
 	typedef typename M::T T;
 	typedef typename M::V V;
@@ -297,11 +310,11 @@ __global__ __launch_bounds__(256) void k_blend_mfma(BlendArgs a, uint32_t n_pad)
 	float* s_out_irr = reinterpret_cast<float*>(smem);          // [48][48]
 	float* s_out_dep = s_out_irr + 48 * kIrrCols;               // [208][32]
 
-	const uint32_t count = *a.count;
-	const uint32_t slot0 = blockIdx.x * kProbesPerBlock;
+	const uint32_t count = __builtin_amdgcn_readfirstlane(*a.count);  // uniform: see blend_fp32_tile
 	if(slot0 >= count) return;
 	const uint32_t np = min(uint32_t(kProbesPerBlock), count - slot0);
-	const uint32_t n = a.grid.n;
+	const uint32_t n = __builtin_amdgcn_readfirstlane(a.grid.n);
+	n_pad = __builtin_amdgcn_readfirstlane(n_pad);
 	const float cell_len = a.grid.cell_len;
 	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 	const int lrow = lane & 15, lgrp = lane >> 4;
@@ -408,6 +421,60 @@ __global__ __launch_bounds__(256) void k_blend_mfma(BlendArgs a, uint32_t n_pad)
 	if(uint32_t(t) < np) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
+template <uint32_t MODE>
+__global__ __launch_bounds__(256) void k_blend_mfma(BlendArgs a, uint32_t n_pad) {
+	blend_mfma_tile<MODE>(a, blockIdx.x * kProbesPerBlock, n_pad);
+}
+
+// ---- batched update ----
+// Weight rows of every volume: grid (256 rows, volumes).
+__global__ __launch_bounds__(256) void k_batch_weights(BatchLaunch b) {
+	const BatchVolume& v = b.vols[blockIdx.y];
+	blend_weight_row(v.blend.grid, v.dirs, v.perm, v.wt, v.inv_sum, v.w_hi, v.w_lo, b.blend_mode, v.n_pad, blockIdx.x);
+}
+
+// Persistent blend over the tiles of all volumes (queue head next[2]).  Every workgroup leaves once
+// the head passes the item count.  Thread 0 does all of the workgroup's queue traffic in one place
+// at the top of the loop: it stamps the completion time of the tile just finished, then takes the
+// next item.  (A second thread-0 block at the bottom of the loop can be merged with this one across
+// the back edge by the compiler's control-flow structurizer, which then leaves the other threads
+// looping over the barriers with a stale item: keep exactly one.)
+template <uint32_t MODE, int NP>
+__global__ __launch_bounds__(256) void k_batch_blend(BatchLaunch b) {
+	__shared__ uint32_t s_item;
+	uint32_t prev = 0xFFFFFFFFu;  // volume of the tile this workgroup finished last
+	// bounded: a workgroup can take at most every item once, plus the fetch that ends the loop
+	for(uint32_t round = 0; round <= b.blend_item_count; ++round) {
+		if(threadIdx.x == 0) {
+			if(prev != 0xFFFFFFFFu) atomicMax(&b.ctl->done[prev], (unsigned long long)wall_clock64());
+			s_item = atomicAdd(&b.ctl->next[2], 1u);
+		}
+		__syncthreads();
+		const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+		__syncthreads();
+		if(it >= b.blend_item_count) break;
+		const uint32_t vol = __builtin_amdgcn_readfirstlane(b.blend_items[it].volume);
+		const uint32_t first = __builtin_amdgcn_readfirstlane(b.blend_items[it].first);
+		const BatchVolume& v = b.vols[vol];
+		if constexpr(MODE == kBlendFP32) blend_fp32_tile<NP>(v.blend, first);
+		else blend_mfma_tile<MODE>(v.blend, first, v.n_pad);
+		prev = vol;
+		__syncthreads();
+	}
+}
+
+const void* batch_blend_kernel(uint32_t mode, uint32_t probes) {
+	if(mode == kBlendBF16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend<kBlendBF16, 16>) : nullptr;
+	if(mode == kBlendFP16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend<kBlendFP16, 16>) : nullptr;
+	switch(probes) {
+		case 4: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 4>);
+		case 8: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 8>);
+		case 16: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 16>);
+		case 32: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 32>);
+	}
+	return nullptr;
+}
+
 }  // namespace
 
 hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, const uint16_t* order, float* wt, float* inv_sum,
@@ -426,6 +493,25 @@ hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s) {
 	else if(mode == kBlendFP16) hipLaunchKernelGGL(k_blend_mfma<kBlendFP16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
 	else hipLaunchKernelGGL(k_blend_fp32, dim3(blocks), dim3(kBlendThreads), 0, s, a);
 	return hipGetLastError();
+}
+
+hipError_t launch_batch_weights(const BatchLaunch& b, hipStream_t s) {
+	hipLaunchKernelGGL(k_batch_weights, dim3(kWRows, b.volume_count), dim3(256), 0, s, b);
+	return hipGetLastError();
+}
+
+hipError_t batch_blend_resident(uint32_t mode, uint32_t probes, int* per_cu) {
+	const void* f = batch_blend_kernel(mode, probes);
+	if(!f) return hipErrorInvalidValue;
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, 256, 0);
+}
+
+hipError_t launch_batch_blend(const BatchLaunch& b, uint32_t grid, hipStream_t s) {
+	const void* f = batch_blend_kernel(b.blend_mode, b.blend_probes);
+	if(!f || grid == 0) return hipErrorInvalidValue;
+	BatchLaunch arg = b;
+	void* args[] = {&arg};
+	return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, s);
 }
 
 }  // namespace ddgi

@@ -1,0 +1,119 @@
+// Host-side state of one probe volume (the object behind a `void*` handle of include/ddgi.h) and
+// the helpers shared by the single-volume entry points (api.hip) and the batched update
+// (batch.hip).  Internal to libddgi.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "bvh.hpp"
+#include "diff.hpp"
+#include "kernels.hpp"
+
+namespace ddgi {
+namespace host {
+
+struct DeviceBuffer {
+	void* p = nullptr;
+	size_t bytes = 0;
+};
+
+struct Handle {
+	ddgi_grid_params params{};
+	ddgi_options opts{};
+	int device = 0;
+	uint32_t P = 0;
+	uint32_t max_rays = 0;
+	// atlases: [cur] is exposed, [cur ^ 1] receives the next update
+	float4* irr[2] = {nullptr, nullptr};
+	float2* dep[2] = {nullptr, nullptr};
+	int cur = 0;
+	uint32_t* states = nullptr;
+	uint32_t* indices = nullptr;
+	SelectState* sel = nullptr;
+	bool indices_are_iota = false;
+	float4* rays = nullptr;
+	float4* hits = nullptr;
+	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
+	uint16_t* perm = nullptr;   // kMaxRays
+	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512
+	float4* dirs = nullptr;     // world directions, kMaxRays
+	uint32_t fib_n = 0;
+	float* wt = nullptr;
+	float* inv_sum = nullptr;
+	void* w_hi = nullptr;
+	void* w_lo = nullptr;
+	unsigned long long* visit = nullptr;  // traversal counters (statistics runs)
+	// scene
+	std::vector<DeviceBuffer> scene_bufs;
+	SceneDev sd{};
+	bool has_scene = false;
+	BvhHost bvh_meta;
+	size_t bvh_bytes = 0;
+	LightConst light{};
+	// timing
+	hipEvent_t ev[6] = {};
+	bool timing_pending = false;
+	ddgi_timings last{};
+	uint32_t last_count = 0xFFFFFFFFu;
+	uint32_t updated_probes = 0;  // hysteresis ramp bookkeeping (src/IrradianceProbes.cpp:463)
+	float uploaded_hysteresis = 0.0f;  // the value the kernels use (the reference's UBO copy)
+	hipStream_t last_stream = nullptr;
+	std::string err;
+	bool has_metallic = false;  // any scene material with metallic != 0
+	uint32_t scene_materials = 0;
+	uint32_t batch_members = 0;  // batches holding this handle (destroy is refused while > 0)
+	// differentiable update (ddgi_diff_*): buffers owned while prepared
+	struct Diff {
+		bool ready = false;
+		std::vector<DeviceBuffer> bufs;
+		DiffArgs d{};
+		float4* prev = nullptr;     // irradiance before the prepared update
+		float4* irr = nullptr;      // differentiated irradiance (forward output)
+		float4* rad = nullptr;
+		float* wt = nullptr;        // blend weights of the prepared frame (ray_order order)
+		float* inv_sum = nullptr;
+		float4* dirs = nullptr;
+		uint32_t* count = nullptr;  // = P (all probes)
+		float2* dep_scratch = nullptr;
+		uint32_t loss_parts = 0;
+	} diff;
+};
+
+// Records `msg` as the handle's (and the thread's) last error and returns `s`.
+ddgi_status fail(Handle* h, ddgi_status s, const std::string& msg);
+
+#define HIP_TRY(h, expr)                                                                               \
+	do {                                                                                               \
+		hipError_t e_ = (expr);                                                                        \
+		if(e_ != hipSuccess) {                                                                         \
+			return ::ddgi::host::fail((h), e_ == hipErrorOutOfMemory ? DDGI_ERROR_OUT_OF_MEMORY : DDGI_ERROR_HIP, \
+									  std::string(#expr) + ": " + hipGetErrorString(e_));              \
+		}                                                                                              \
+	} while(0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t bytes) {
+	void* q = nullptr;
+	hipError_t e = hipMalloc(&q, bytes < 16 ? 16 : bytes);
+	*p = static_cast<T*>(q);
+	return e;
+}
+
+inline Handle* H(void* p) { return static_cast<Handle*>(p); }
+
+// Grid constants of the kernels (cell size, ray tmax, the hysteresis the kernels use).
+GridConst make_grid_const(const Handle& h);
+// Makes the handle's device current.
+ddgi_status set_device(Handle* h);
+// Uploads the spherical-Fibonacci tables and the ray order for the current rays_per_probe.
+ddgi_status ensure_fib(Handle* h);
+// Validates a row-major 3x3 orientation.
+ddgi_status make_rotation(const float* o, Rot& R);
+// Hysteresis ramp of src/IrradianceProbes.cpp:462-476 after an update of `updated` probes.
+void advance_hysteresis(Handle* h, uint32_t updated);
+
+}  // namespace host
+}  // namespace ddgi

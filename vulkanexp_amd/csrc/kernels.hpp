@@ -125,6 +125,7 @@ hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const floa
 hipError_t launch_select(const uint32_t* states, uint32_t P, uint32_t cap, SelectState* st, uint32_t* indices,
 						 hipStream_t s);
 hipError_t launch_iota(uint32_t* indices, uint32_t P, SelectState* st, hipStream_t s);
+hipError_t launch_set_count(SelectState* st, uint32_t count, hipStream_t s);
 hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float4* irr, const float2* dep,
 						 const float* pos, const float* nrm, const float* tocam, uint64_t count, float* out,
 						 hipStream_t s);
@@ -138,5 +139,59 @@ hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s);
 constexpr uint32_t kBlendFP32 = 0;
 constexpr uint32_t kBlendBF16 = 1;
 constexpr uint32_t kBlendFP16 = 2;
+
+// ---- batched update of many probe volumes (one graph launch per update) ----
+constexpr uint32_t kMaxBatchVolumes = 256;
+constexpr uint32_t kMfmaProbes = 16;   // probes per MFMA blend tile
+
+// Everything the kernels of one volume read during a batched update.
+struct BatchVolume {
+	TraceFrameArgs trace;  // trace.dirs == dirs
+	BlendArgs blend;       // blend.wt / inv_sum / w_hi16 / w_lo16 == the buffers below
+	Rot R;                 // orientation of this update
+	const float4* fib;     // fib(i, n)
+	float4* dirs;          // world directions, written by k_batch_begin
+	const uint16_t* perm;  // ray order of the traversal threads
+	float* wt;
+	float* inv_sum;
+	void* w_hi;
+	void* w_lo;
+	uint32_t threads;      // traversal threads: slots * rays
+	uint32_t n_pad;        // rays rounded up to the MFMA chunk
+};
+
+struct BatchItem {
+	uint32_t volume;
+	uint32_t first;  // first traversal thread (trace items) or first slot (blend items)
+};
+
+struct BatchControl {
+	uint32_t next[4];        // work-queue heads: primary, shade, blend
+	unsigned long long t0;   // wall clock when the update started on the device
+	unsigned long long pad;
+	unsigned long long done[kMaxBatchVolumes];  // wall clock when a volume's last blend tile ended
+};
+
+struct BatchLaunch {
+	const BatchVolume* vols;
+	const BatchItem* trace_items;
+	const BatchItem* blend_items;
+	BatchControl* ctl;
+	uint32_t volume_count;
+	uint32_t trace_item_count;
+	uint32_t trace_item_threads;  // traversal threads per item
+	uint32_t blend_item_count;
+	uint32_t blend_probes;        // slots per blend item
+	uint32_t blend_mode;
+};
+
+hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s);
+hipError_t launch_batch_weights(const BatchLaunch& b, hipStream_t s);
+hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t stack, uint32_t block, bool shade, uint32_t grid,
+							  hipStream_t s);
+hipError_t launch_batch_blend(const BatchLaunch& b, uint32_t grid, hipStream_t s);
+// Resident workgroups per CU of the persistent kernels (for sizing their grids).
+hipError_t batch_trace_resident(uint32_t stack, uint32_t block, bool shade, int* per_cu);
+hipError_t batch_blend_resident(uint32_t mode, uint32_t probes, int* per_cu);
 
 }  // namespace ddgi

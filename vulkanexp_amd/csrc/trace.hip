@@ -26,21 +26,24 @@ __device__ __forceinline__ float4 pack_hit(const HitResult& h) {
 }
 
 // ---- per-frame world directions: dirs[i] = R * fib[i] (FP32, no FMA) ----
+__device__ __forceinline__ float4 frame_dir(const Rot& R, float4 f) {
+	float d[3];
+	for(int k = 0; k < 3; ++k) d[k] = R.r[k * 3 + 0] * f.x + R.r[k * 3 + 1] * f.y + R.r[k * 3 + 2] * f.z;
+	return make_float4(d[0], d[1], d[2], 0.0f);
+}
+
 __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= count) return;
-	const float4 f = fib[i];
-	float d[3];
-	for(int k = 0; k < 3; ++k) d[k] = R.r[k * 3 + 0] * f.x + R.r[k * 3 + 1] * f.y + R.r[k * 3 + 2] * f.z;
-	dirs[i] = make_float4(d[0], d[1], d[2], 0.0f);
+	dirs[i] = frame_dir(R, fib[i]);
 }
 
 // ---- primary probe rays: closest hit, mask STATIC|DYNAMIC, tmin 0.01, tmax = |max - min| ----
+// Traversal thread `gtid` = slot * n + j traces ray perm[j] of the probe in `slot`; `stack` is the
+// thread's column of an LDS stack with `stride` columns.
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a) {
-	__shared__ uint32_t stack[STACK * kTraceBlock];
+__device__ __forceinline__ void trace_primary_ray(const TraceFrameArgs& a, uint32_t gtid, uint32_t* stack, int stride) {
 	const uint32_t n = a.grid.n;
-	const uint32_t gtid = xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x;
 	const uint32_t slot = gtid / n, j = gtid - slot * n;
 	if(slot >= *a.count) return;
 	const uint32_t i = a.perm[j];
@@ -51,8 +54,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a)
 	ray_prepare(o[0], o[1], o[2], d.x, d.y, d.z, r);
 	uint32_t nn = 0, nt = 0;
 	const HitResult h = traverse<false, STACK, COUNT>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
-													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack + threadIdx.x,
-													  kTraceBlock, nn, nt);
+													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack, stride, nn, nt);
 	a.hits[size_t(slot) * n + j] = pack_hit(h);  // thread order: coalesced stores
 	if(COUNT) {
 		atomicAdd(&a.visit_counters[0], (unsigned long long)nn);
@@ -60,12 +62,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a)
 	}
 }
 
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	trace_primary_ray<STACK, COUNT>(a, xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x,
+									stack + threadIdx.x, kTraceBlock);
+}
+
 // ---- shading of the primary hits + shadow ray (one thread per ray) ----
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
-	__shared__ uint32_t stack[STACK * kTraceBlock];
+__device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t gtid, uint32_t* stack, int stride) {
 	const uint32_t n = a.grid.n;
-	const uint32_t gtid = xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x;
 	const uint32_t slot = gtid / n, j = gtid - slot * n;
 	if(slot >= *a.count) return;
 	const uint32_t i = a.perm[j];
@@ -134,8 +141,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 	RayPrep sr;
 	ray_prepare(position.x, position.y, position.z, a.light.dir[0], a.light.dir[1], a.light.dir[2], sr);
 	uint32_t nn = 0, nt = 0;
-	const HitResult sh = traverse<true, STACK, COUNT>(a.scene.nodes, a.scene.tris, sr, 0.1f, 10000.0f, 0xFFu,
-													  stack + threadIdx.x, kTraceBlock, nn, nt);
+	const HitResult sh = traverse<true, STACK, COUNT>(a.scene.nodes, a.scene.tris, sr, 0.1f, 10000.0f, 0xFFu, stack,
+													  stride, nn, nt);
 	if(COUNT) {
 		atomicAdd(&a.visit_counters[2], (unsigned long long)nn);
 		atomicAdd(&a.visit_counters[3], (unsigned long long)nt);
@@ -172,6 +179,53 @@ __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 		f[1] = make_float4(A.x, A.y, A.z, __uint_as_float(lit ? 1u : 0u));
 		f[2] = make_float4(B.x, B.y, B.z, 0.0f);
 		f[3] = make_float4(C.x, C.y, C.z, 0.0f);
+	}
+}
+
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	shade_ray<STACK, COUNT>(a, xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x, stack + threadIdx.x,
+							kTraceBlock);
+}
+
+// ---- batched update of many volumes ----
+// Start of a batched update: block v writes volume v's frame directions; block 0 also resets the
+// work-queue heads and completion stamps and records the start time.
+__global__ __launch_bounds__(512) void k_batch_begin(BatchLaunch b) {
+	const BatchVolume& v = b.vols[blockIdx.x];
+	const uint32_t n = v.trace.grid.n;
+	for(uint32_t i = threadIdx.x; i < n; i += 512) v.dirs[i] = frame_dir(v.R, v.fib[i]);
+	if(blockIdx.x == 0) {
+		if(threadIdx.x < 4) b.ctl->next[threadIdx.x] = 0u;
+		for(uint32_t k = threadIdx.x; k < kMaxBatchVolumes; k += 512) b.ctl->done[k] = 0ull;
+		if(threadIdx.x == 0) b.ctl->t0 = wall_clock64();
+	}
+}
+
+// Persistent traversal over the work items of all volumes: each workgroup takes the next item
+// (a run of consecutive traversal threads of one volume) from a queue until the queue is empty.
+// Every workgroup leaves once the head passes the item count, so the grid always drains.
+template <int STACK, int BLOCK, bool SHADE>
+__global__ __launch_bounds__(BLOCK) void k_batch_trace(BatchLaunch b) {
+	__shared__ uint32_t stack[STACK * BLOCK];
+	__shared__ uint32_t s_item;
+	uint32_t* head = &b.ctl->next[SHADE ? 1 : 0];
+	// bounded: a workgroup can take at most every item once, plus the fetch that ends the loop
+	for(uint32_t round = 0; round <= b.trace_item_count; ++round) {
+		if(threadIdx.x == 0) s_item = atomicAdd(head, 1u);
+		__syncthreads();
+		const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+		__syncthreads();
+		if(it >= b.trace_item_count) break;
+		const uint32_t vol = __builtin_amdgcn_readfirstlane(b.trace_items[it].volume);
+		const uint32_t first = __builtin_amdgcn_readfirstlane(b.trace_items[it].first);
+		const BatchVolume& v = b.vols[vol];
+		const uint32_t end = min(first + b.trace_item_threads, v.threads);
+		for(uint32_t g = first + threadIdx.x; g < end; g += BLOCK) {
+			if(SHADE) shade_ray<STACK, false>(v.trace, g, stack + threadIdx.x, BLOCK);
+			else trace_primary_ray<STACK, false>(v.trace, g, stack + threadIdx.x, BLOCK);
+		}
 	}
 }
 
@@ -423,6 +477,49 @@ hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float
 	if(count == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_sample, dim3(blocks_for(count, 256)), dim3(256), 0, s, g, states, irr, dep, pos, nrm, tocam, count, out);
 	return hipGetLastError();
+}
+
+// ---- batched update ----
+namespace {
+
+template <bool SHADE>
+const void* batch_trace_kernel(uint32_t stack, uint32_t block) {
+	if(stack <= 32) {
+		switch(block) {
+			case 64: return reinterpret_cast<const void*>(&k_batch_trace<32, 64, SHADE>);
+			case 128: return reinterpret_cast<const void*>(&k_batch_trace<32, 128, SHADE>);
+			case 256: return reinterpret_cast<const void*>(&k_batch_trace<32, 256, SHADE>);
+		}
+	} else {
+		switch(block) {
+			case 64: return reinterpret_cast<const void*>(&k_batch_trace<64, 64, SHADE>);
+			case 128: return reinterpret_cast<const void*>(&k_batch_trace<64, 128, SHADE>);
+			case 256: return reinterpret_cast<const void*>(&k_batch_trace<64, 256, SHADE>);
+		}
+	}
+	return nullptr;
+}
+
+}  // namespace
+
+hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s) {
+	hipLaunchKernelGGL(k_batch_begin, dim3(b.volume_count), dim3(512), 0, s, b);
+	return hipGetLastError();
+}
+
+hipError_t batch_trace_resident(uint32_t stack, uint32_t block, bool shade, int* per_cu) {
+	const void* f = shade ? batch_trace_kernel<true>(stack, block) : batch_trace_kernel<false>(stack, block);
+	if(!f) return hipErrorInvalidValue;
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, int(block), 0);
+}
+
+hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t stack, uint32_t block, bool shade, uint32_t grid,
+							  hipStream_t s) {
+	const void* f = shade ? batch_trace_kernel<true>(stack, block) : batch_trace_kernel<false>(stack, block);
+	if(!f || grid == 0) return hipErrorInvalidValue;
+	BatchLaunch arg = b;
+	void* args[] = {&arg};
+	return hipLaunchKernel(f, dim3(grid), dim3(block), args, 0, s);
 }
 
 }  // namespace ddgi

@@ -303,3 +303,42 @@ def config_large(seed=2, rays=256):
 
 
 SKY_RADIANCE = np.array([0.5, 0.5, 0.5], np.float32)
+
+
+BATCH_SCENE_INSTANCES = (None, 2, 8, 25, 80, 256)  # None: the 780-triangle room
+
+
+def batched_scenes(seed=5):
+    """Six scenes at log-spaced sizes: the room + 64 boxes (780 triangles), then clustered scenes of
+    2, 8, 25, 80 and 256 instances of 1024 triangles (2,048 ... 262,144 triangles)."""
+    out = []
+    for k, n in enumerate(BATCH_SCENE_INSTANCES):
+        out.append(room_boxes_scene(seed + k) if n is None else clustered_scene(seed + k, n_instances=n))
+    return out
+
+
+def batched_volumes(seed=5, count=24, n_scenes=6):
+    """Volume specs of the batched workload: volume i uses scene i % n_scenes; its grid is
+    (4*8^s, 4*4^s, 4*8^s) probes rounded, s ~ U[0, 1] (log-uniform between 4x4x4 and 32x16x32;
+    volume 0 is the largest, volume 1 the smallest grid), rays per probe drawn from
+    {64, 128, 256, 512}.  Returns [(scene index, resolution, rays)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = rng.uniform(0.0, 1.0, count)
+    s[0], s[1] = 1.0, 0.0
+    rays = rng.choice(np.array([64, 128, 256, 512]), count)
+    out = []
+    for i in range(count):
+        res = (int(round(4 * 8 ** s[i])), int(round(4 * 4 ** s[i])), int(round(4 * 8 ** s[i])))
+        out.append((i % n_scenes, res, int(rays[i])))
+    return out
+
+
+def batched_config(seed=5, count=24):
+    """Batched volumes: ([scene], [(scene index, grid params)]) with each grid spanning its scene's
+    bounds inset by 0.5 m."""
+    scenes = batched_scenes(seed)
+    vols = []
+    for si, res, n in batched_volumes(seed, count, len(scenes)):
+        lo, hi = inset_extent(scenes[si])
+        vols.append((si, grid_params(lo, hi, res, n)))
+    return scenes, vols
