@@ -216,8 +216,8 @@ void destroy_handle(Handle* h) {
 		(void)hipFree(h->irr[i]);
 		(void)hipFree(h->dep[i]);
 	}
-	void* bufs[] = {h->states, h->indices, h->sel, h->rays, h->hits, h->fib, h->perm, h->fib512, h->dirs,
-					h->wt, h->inv_sum, h->w_hi, h->w_lo, h->visit};
+	void* bufs[] = {h->states, h->indices, h->sel,    h->rays, h->hits,  h->shade_list, h->shade_count, h->fib,
+					h->perm,   h->fib512,  h->dirs,   h->wt,   h->inv_sum, h->w_hi,      h->w_lo,        h->visit};
 	for(void* b : bufs) (void)hipFree(b);
 	for(auto& e : h->ev)
 		if(e) (void)hipEventDestroy(e);
@@ -355,6 +355,8 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->sel, sizeof(SelectState));
 	if(e == hipSuccess) e = dalloc(&h->rays, P * R * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->hits, P * R * sizeof(float4));
+	if(e == hipSuccess) e = dalloc(&h->shade_list, P * R * sizeof(uint32_t));
+	if(e == hipSuccess) e = dalloc(&h->shade_count, 16);
 	if(e == hipSuccess) e = dalloc(&h->fib, kMaxRays * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->perm, kMaxRays * sizeof(uint16_t));
 	if(e == hipSuccess) e = dalloc(&h->fib512, 512 * sizeof(float4));
@@ -605,7 +607,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 		advance_hysteresis(h, cnt);
 	}
 	const GridConst g = make_grid_const(*h);
-	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s));
+	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s, h->shade_count));
 	TraceFrameArgs a{};
 	a.scene = h->sd;
 	a.grid = g;
@@ -619,6 +621,8 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	a.dep = h->dep[h->cur];
 	a.hits = h->hits;
 	a.rays = h->rays;
+	a.shade_list = h->shade_list;
+	a.shade_count = h->shade_count;
 	a.max_slots = max_slots;
 	a.visit_counters = nullptr;
 	a.factors = factors;

@@ -208,6 +208,8 @@ void fill_volume(const Handle& h, const Rot& R, BatchVolume& v) {
 	a.dep = h.dep[h.cur];
 	a.hits = h.hits;
 	a.rays = h.rays;
+	a.shade_list = h.shade_list;
+	a.shade_count = h.shade_count;
 	a.max_slots = h.P;
 	BlendArgs& bl = v.blend;
 	bl.grid = g;

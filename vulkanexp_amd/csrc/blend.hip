@@ -179,7 +179,7 @@ __device__ __forceinline__ void update_state(uint32_t* states, uint32_t probe, f
 template <int NP>
 __device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slot0) {
 	__shared__ float4 s_irr[NP][kRayChunk];
-	__shared__ float4 s_dep[NP][kRayChunk];
+	__shared__ float2 s_dep[NP][kRayChunk];
 	__shared__ uint32_t s_oor[NP];
 	__shared__ uint32_t s_maxc[NP];
 	// count and n steer loops that contain barriers: make them provably uniform (count is read
@@ -200,17 +200,24 @@ __device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slo
 	const bool is_depth = t >= kIrrTexels;
 	const bool active = t < kBlendTexels;
 	for(uint32_t i0 = 0; i0 < n; i0 += kRayChunk) {
+		const uint32_t iend = min(uint32_t(kRayChunk), n - i0);
+		// this chunk's weights of texel t, fetched before the barrier so that the loads overlap the
+		// staging instead of stalling each step of the accumulation
+		float w[kRayChunk];
+#pragma unroll
+		for(int i = 0; i < int(kRayChunk); ++i)
+			w[i] = (active && uint32_t(i) < iend) ? a.wt[size_t(i0 + i) * kBlendTexels + t] : 0.0f;
 		__syncthreads();
-		// stage 16 probes x 32 rays
+		// stage NP probes x 32 rays: (r, g, b) for irradiance, (d, d^2) for depth
 		for(uint32_t e = t; e < NP * kRayChunk; e += kBlendThreads) {
 			const uint32_t p = e / kRayChunk, i = e % kRayChunk;
 			float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-			float4 dd = r;
+			float2 dd = make_float2(0.0f, 0.0f);
 			if(p < np && i0 + i < n) {
 				r = a.rays[size_t(slot0 + p) * n + i0 + i];
 				float dc = fminf(cell_len, r.w);
 				if(dc < 0.0f) dc = cell_len;
-				dd = make_float4(dc, dc * dc, 0.0f, 0.0f);
+				dd = make_float2(dc, dc * dc);
 				if(r.w < 0.0f || r.w > cell_len) atomicAdd(&s_oor[p], 1u);
 				r.w = 0.0f;
 			}
@@ -218,17 +225,31 @@ __device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slo
 			s_dep[p][i] = dd;
 		}
 		__syncthreads();
-		if(active) {
-			const uint32_t iend = min(uint32_t(kRayChunk), n - i0);
-			const float4(*src)[kRayChunk] = is_depth ? s_dep : s_irr;
-			for(uint32_t i = 0; i < iend; ++i) {
-				const float w = a.wt[size_t(i0 + i) * kBlendTexels + t];
+		// rays in order for every (texel, probe, channel): the sums do not depend on NP.  Waves 1..3
+		// hold depth texels only and take the two-channel path as a whole.
+		if(active && is_depth) {
 #pragma unroll
-				for(int p = 0; p < NP; ++p) {
-					const float4 x = src[p][i];
-					acc[p][0] += w * x.x;
-					acc[p][1] += w * x.y;
-					acc[p][2] += w * x.z;
+			for(int i = 0; i < int(kRayChunk); ++i) {
+				if(uint32_t(i) < iend) {
+#pragma unroll
+					for(int p = 0; p < NP; ++p) {
+						const float2 x = s_dep[p][i];
+						acc[p][0] += w[i] * x.x;
+						acc[p][1] += w[i] * x.y;
+					}
+				}
+			}
+		} else if(active) {
+#pragma unroll
+			for(int i = 0; i < int(kRayChunk); ++i) {
+				if(uint32_t(i) < iend) {
+#pragma unroll
+					for(int p = 0; p < NP; ++p) {
+						const float4 x = s_irr[p][i];
+						acc[p][0] += w[i] * x.x;
+						acc[p][1] += w[i] * x.y;
+						acc[p][2] += w[i] * x.z;
+					}
 				}
 			}
 		}

@@ -37,6 +37,22 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t block, uint32_t nblocks) 
 	return x * q + min(x, r) + k;
 }
 
+// List append from a wavefront: every active lane with `pred` gets a distinct position in the list
+// whose length is *counter, with one atomic per wave (issued by the lowest such lane).  Explicit, so
+// that it does not depend on the compiler proving `counter` uniform.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+	const uint64_t mask = __ballot(pred);
+	const uint32_t lane = __lane_id();
+	const uint32_t before = uint32_t(__popcll(mask & ((uint64_t(1) << lane) - 1u)));
+	uint32_t base = 0;
+	if(mask != 0u) {
+		const int leader = __ffsll((unsigned long long)mask) - 1;
+		if(int(lane) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+		base = __shfl(base, leader);
+	}
+	return base + before;
+}
+
 __device__ __forceinline__ void ray_prepare(float ox, float oy, float oz, float dx, float dy, float dz, RayPrep& r) {
 	const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
 	const int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);

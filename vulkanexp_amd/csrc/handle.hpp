@@ -36,6 +36,8 @@ struct Handle {
 	bool indices_are_iota = false;
 	float4* rays = nullptr;
 	float4* hits = nullptr;
+	uint32_t* shade_list = nullptr;   // P * max_rays entries
+	uint32_t* shade_count = nullptr;
 	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
 	uint16_t* perm = nullptr;   // kMaxRays
 	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512

@@ -79,6 +79,8 @@ struct TraceFrameArgs {
 	const float2* dep;         // exposed depth atlas (tiled)
 	float4* hits;
 	float4* rays;              // (rgb, depth)
+	uint32_t* shade_list;      // ray positions (slot * n + k) of the front-face hits, filled by the primary pass
+	uint32_t* shade_count;     // its length (cleared before the primary pass)
 	uint32_t max_slots;        // grid sized for this many slots
 	// optional (statistics runs only): [0]/[1] primary node / triangle fetches, [2]/[3] shadow
 	// node / triangle fetches, [4] shadow rays
@@ -115,7 +117,8 @@ struct BlendArgs {
 constexpr uint32_t kBlendIrradianceOnly = 0x1;
 
 // ---- launchers (return hipGetLastError of the launch) ----
-hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s);
+hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
+							 uint32_t* zero = nullptr);
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,

@@ -5,6 +5,10 @@
 // (src/IrradianceProbes.cpp:396-424) and standalone sampleProbes queries.
 //
 // Built with -ffp-contract=off (see device_common.hpp).
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "device_common.hpp"
 
 namespace ddgi {
@@ -32,10 +36,27 @@ __device__ __forceinline__ float4 frame_dir(const Rot& R, float4 f) {
 	return make_float4(d[0], d[1], d[2], 0.0f);
 }
 
-__global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count) {
+// `zero` (optional): a counter the next kernels start from, cleared here so that no separate
+// memset is needed.
+__global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count,
+							 uint32_t* zero) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(zero && i == 0) *zero = 0u;
 	if(i >= count) return;
 	dirs[i] = frame_dir(R, fib[i]);
+}
+
+// Ray records of rays that need no shading: misses.rmiss (sky radiance, depth -1) and back faces
+// (closesthit.glsl:137-141: black, depth 0.8 t); with `factors` also their (constant) factor rows.
+__device__ __forceinline__ void write_unshaded(const TraceFrameArgs& a, size_t ri, bool miss, float t) {
+	const float4 rec = miss ? make_float4(a.light.sky[0], a.light.sky[1], a.light.sky[2], -1.0f)
+							: make_float4(0.0f, 0.0f, 0.0f, t * 0.8f);
+	a.rays[ri] = rec;
+	if(a.factors) {
+		float4* f = a.factors + ri * 4;
+		f[0] = make_float4(rec.x, rec.y, rec.z, __uint_as_float(0xFFFFFFFFu));
+		f[1] = f[2] = f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	}
 }
 
 // ---- primary probe rays: closest hit, mask STATIC|DYNAMIC, tmin 0.01, tmax = |max - min| ----
@@ -55,7 +76,14 @@ __device__ __forceinline__ void trace_primary_ray(const TraceFrameArgs& a, uint3
 	uint32_t nn = 0, nt = 0;
 	const HitResult h = traverse<false, STACK, COUNT>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
 													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack, stride, nn, nt);
-	a.hits[size_t(slot) * n + j] = pack_hit(h);  // thread order: coalesced stores
+	const size_t ri = size_t(slot) * n + j;  // thread order: coalesced stores
+	const float4 packed = pack_hit(h);
+	a.hits[ri] = packed;
+	const uint32_t code = __float_as_uint(packed.y);
+	const bool front = code != 0xFFFFFFFFu && !(code & 0x80000000u);
+	const uint32_t pos = wave_append(a.shade_count, front);  // front-face hits are queued for shading
+	if(front) a.shade_list[pos] = uint32_t(ri);
+	else write_unshaded(a, ri, code == 0xFFFFFFFFu, h.t);
 	if(COUNT) {
 		atomicAdd(&a.visit_counters[0], (unsigned long long)nn);
 		atomicAdd(&a.visit_counters[1], (unsigned long long)nt);
@@ -69,35 +97,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a)
 									stack + threadIdx.x, kTraceBlock);
 }
 
-// ---- shading of the primary hits + shadow ray (one thread per ray) ----
+// ---- shading of the front-face hits + shadow ray (closesthit.glsl:143-289) ----
+// Entry k of the frame's shade list (the front-face hits queued by the primary traversal), so that
+// every lane of a wave shades and casts a shadow ray.  Each ray's result depends only on its own
+// record: the order of the list does not change any output.
 template <int STACK, bool COUNT>
-__device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t gtid, uint32_t* stack, int stride) {
+__device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, uint32_t* stack, int stride) {
+	if(k >= *a.shade_count) return;
 	const uint32_t n = a.grid.n;
-	const uint32_t slot = gtid / n, j = gtid - slot * n;
-	if(slot >= *a.count) return;
+	const uint32_t ri = a.shade_list[k];  // hits and rays are stored in thread (ray_order) order
+	const uint32_t slot = ri / n, j = ri - slot * n;
 	const uint32_t i = a.perm[j];
-	const size_t ri = size_t(slot) * n + j;  // hits and rays are stored in thread (ray_order) order
 	const float4 hit = a.hits[ri];
 	const uint32_t code = __float_as_uint(hit.y);
-	if(code == 0xFFFFFFFFu) {  // miss.rmiss: sky radiance, depth -1
-		a.rays[ri] = make_float4(a.light.sky[0], a.light.sky[1], a.light.sky[2], -1.0f);
-		if(a.factors) {
-			float4* f = a.factors + ri * 4;
-			f[0] = make_float4(a.light.sky[0], a.light.sky[1], a.light.sky[2], __uint_as_float(0xFFFFFFFFu));
-			f[1] = f[2] = f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		}
-		return;
-	}
 	const float t = hit.x;
-	if(code & 0x80000000u) {  // back face: closesthit.glsl:137-141
-		a.rays[ri] = make_float4(0.0f, 0.0f, 0.0f, t * 0.8f);
-		if(a.factors) {
-			float4* f = a.factors + ri * 4;
-			f[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
-			f[1] = f[2] = f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		}
-		return;
-	}
 	const uint32_t tslot = code & 0x7FFFFFFFu;
 	float o[3];
 	probe_origin(a.grid, a.indices[slot], o);
@@ -182,11 +195,14 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t gtid
 	}
 }
 
+// Grid-stride over the shade list (its length is known only on the device).
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
-	shade_ray<STACK, COUNT>(a, xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x, stack + threadIdx.x,
-							kTraceBlock);
+	const uint32_t count = *a.shade_count;
+	const uint32_t stride = gridDim.x * kTraceBlock;
+	for(uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x; k < count; k += stride)
+		shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kTraceBlock);
 }
 
 // ---- batched update of many volumes ----
@@ -196,6 +212,7 @@ __global__ __launch_bounds__(512) void k_batch_begin(BatchLaunch b) {
 	const BatchVolume& v = b.vols[blockIdx.x];
 	const uint32_t n = v.trace.grid.n;
 	for(uint32_t i = threadIdx.x; i < n; i += 512) v.dirs[i] = frame_dir(v.R, v.fib[i]);
+	if(threadIdx.x == 0) *v.trace.shade_count = 0u;
 	if(blockIdx.x == 0) {
 		if(threadIdx.x < 4) b.ctl->next[threadIdx.x] = 0u;
 		for(uint32_t k = threadIdx.x; k < kMaxBatchVolumes; k += 512) b.ctl->done[k] = 0ull;
@@ -395,10 +412,42 @@ __global__ void k_sample(GridConst g, const uint32_t* __restrict__ states, const
 
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
 
+// Workgroups of `kernel` that fit on the current device at once (CUs x resident per CU), cached per
+// (device, kernel): the occupancy query costs more host time than a small launch.
+hipError_t resident_blocks(const void* kernel, uint32_t block, uint64_t* out) {
+	struct Entry {
+		int device;
+		const void* kernel;
+		uint64_t blocks;
+	};
+	static std::mutex mu;
+	static std::vector<Entry> cache;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return e;
+	{
+		std::lock_guard<std::mutex> lock(mu);
+		for(const Entry& c : cache)
+			if(c.device == dev && c.kernel == kernel) {
+				*out = c.blocks;
+				return hipSuccess;
+			}
+	}
+	int cus = 0, per_cu = 0;
+	e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	if(e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, int(block), 0);
+	if(e != hipSuccess) return e;
+	*out = uint64_t(std::max(per_cu, 1)) * uint64_t(std::max(cus, 1));
+	std::lock_guard<std::mutex> lock(mu);
+	cache.push_back(Entry{dev, kernel, *out});
+	return hipSuccess;
+}
+
 }  // namespace
 
-hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s) {
-	hipLaunchKernelGGL(k_frame_dirs, dim3(blocks_for(count, 256)), dim3(256), 0, s, R, fib, dirs, count);
+hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
+							 uint32_t* zero) {
+	hipLaunchKernelGGL(k_frame_dirs, dim3(blocks_for(count, 256)), dim3(256), 0, s, R, fib, dirs, count, zero);
 	return hipGetLastError();
 }
 
@@ -418,18 +467,20 @@ hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
-	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
+	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;  // upper bound of the shade list
 	if(threads == 0) return hipSuccess;
-	const dim3 grid(blocks_for(threads, kTraceBlock));
 	const bool cnt = a.visit_counters != nullptr;
-	if(a.scene.stack_depth <= 32) {
-		if(cnt) hipLaunchKernelGGL((k_shade<32, true>), grid, dim3(kTraceBlock), 0, s, a);
-		else hipLaunchKernelGGL((k_shade<32, false>), grid, dim3(kTraceBlock), 0, s, a);
-	} else {
-		if(cnt) hipLaunchKernelGGL((k_shade<64, true>), grid, dim3(kTraceBlock), 0, s, a);
-		else hipLaunchKernelGGL((k_shade<64, false>), grid, dim3(kTraceBlock), 0, s, a);
-	}
-	return hipGetLastError();
+	const void* f = a.scene.stack_depth <= 32
+						? (cnt ? reinterpret_cast<const void*>(&k_shade<32, true>) : reinterpret_cast<const void*>(&k_shade<32, false>))
+						: (cnt ? reinterpret_cast<const void*>(&k_shade<64, true>) : reinterpret_cast<const void*>(&k_shade<64, false>));
+	// grid-stride: as many workgroups as fit on the device at once (the list length is on the device)
+	uint64_t resident = 0;
+	const hipError_t e = resident_blocks(f, kTraceBlock, &resident);
+	if(e != hipSuccess) return e;
+	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(threads, kTraceBlock))));
+	TraceFrameArgs arg = a;
+	void* args[] = {&arg};
+	return hipLaunchKernel(f, grid, dim3(kTraceBlock), args, 0, s);
 }
 
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
