@@ -50,7 +50,6 @@ struct Batch {
 	hipGraphExec_t exec = nullptr;
 	bool exec_valid = false;
 	uint32_t exec_mode = 0;
-	uint32_t exec_stack = 0;
 	std::vector<uint64_t> exec_shape;  // (P, n) per volume the work items were built for
 	hipStream_t cap = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -82,12 +81,6 @@ ddgi_status check_config(const ddgi_batch_config& c) {
 
 uint32_t mode_of(uint32_t flags) {
 	return (flags & DDGI_UPDATE_BLEND_BF16) ? kBlendBF16 : ((flags & DDGI_UPDATE_BLEND_FP16) ? kBlendFP16 : kBlendFP32);
-}
-
-uint32_t batch_stack(const Batch& b) {
-	uint32_t s = 32;
-	for(const Handle* h : b.vols) s = std::max(s, h->sd.stack_depth);
-	return s;
 }
 
 std::vector<uint64_t> batch_shape(const Batch& b) {
@@ -148,20 +141,19 @@ ddgi_status build_items(Batch* b, uint32_t mode, BatchLaunch& L) {
 	return DDGI_OK;
 }
 
-// (Re)captures the graph when the configuration, blend mode, BVH stack size or a volume's shape
-// changed since the last capture.
+// (Re)captures the graph when the configuration, blend mode or a volume's shape changed since the
+// last capture.
 ddgi_status ensure_graph(Batch* b, uint32_t mode) {
-	const uint32_t stack = batch_stack(*b);
 	const std::vector<uint64_t> shape = batch_shape(*b);
-	if(b->exec_valid && b->exec_mode == mode && b->exec_stack == stack && b->exec_shape == shape) return DDGI_OK;
+	if(b->exec_valid && b->exec_mode == mode && b->exec_shape == shape) return DDGI_OK;
 	BHIP_TRY(hipDeviceSynchronize());  // the item buffers may be in use by an earlier launch
 	destroy_exec(b);
 	BatchLaunch L{};
 	ddgi_status st = build_items(b, mode, L);
 	if(st != DDGI_OK) return st;
 	int r_primary = 0, r_shade = 0, r_blend = 0;
-	BHIP_TRY(batch_trace_resident(stack, b->cfg.trace_block, false, &r_primary));
-	BHIP_TRY(batch_trace_resident(stack, b->cfg.trace_block, true, &r_shade));
+	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, false, &r_primary));
+	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, true, &r_shade));
 	BHIP_TRY(batch_blend_resident(mode, L.blend_probes, &r_blend));
 	const uint32_t g_primary = uint32_t(std::max(1, r_primary) * b->cus);
 	const uint32_t g_shade = uint32_t(std::max(1, r_shade) * b->cus);
@@ -169,8 +161,8 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	BHIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
 	hipError_t e = launch_batch_begin(L, b->cap);
 	if(e == hipSuccess) e = launch_batch_weights(L, b->cap);
-	if(e == hipSuccess) e = launch_batch_trace(L, stack, b->cfg.trace_block, false, std::min(g_primary, L.trace_item_count), b->cap);
-	if(e == hipSuccess) e = launch_batch_trace(L, stack, b->cfg.trace_block, true, std::min(g_shade, L.trace_item_count), b->cap);
+	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, false, std::min(g_primary, L.trace_item_count), b->cap);
+	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, true, std::min(g_shade, L.trace_item_count), b->cap);
 	if(e == hipSuccess) e = launch_batch_blend(L, std::min(g_blend, L.blend_item_count), b->cap);
 	hipGraph_t graph = nullptr;
 	const hipError_t e_end = hipStreamEndCapture(b->cap, &graph);
@@ -186,7 +178,6 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	}
 	b->exec_valid = true;
 	b->exec_mode = mode;
-	b->exec_stack = stack;
 	b->exec_shape = shape;
 	return DDGI_OK;
 }

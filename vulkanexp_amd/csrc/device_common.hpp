@@ -132,21 +132,64 @@ struct HitResult {
 	float V, W, det;
 };
 
-// Closest-hit (ANY = false; ties -> lowest global id) or any-hit traversal of the BVH, with a
-// per-thread stack in LDS laid out [entry][thread] (conflict-free).  `stack` points at this
-// thread's column; `stride` is the block size.  With COUNT, n_nodes / n_tris receive the number
-// of 64-byte node and 48-byte triangle fetches (the algorithmic bytes of the traversal).
+// Closest-hit (ANY = false; ties -> lowest global id) or any-hit traversal of the BVH.
+//
+// Stack: the newest STACK entries live in an LDS ring laid out [entry][thread] (conflict-free;
+// `stack` points at this thread's column, `stride` is the block size); when the ring is full its
+// oldest entry moves to a private spill array, and pops drain the ring first, then the spill, so
+// the order stays LIFO.  Capacity STACK + kSpill >= the deepest BVH accepted (63 levels), while the
+// LDS footprint stays small enough for full occupancy.
+//
+// Leaves are tested two triangles at a time, each triangle's 48 bytes loaded in one go, so that
+// two loads are in flight and the mask word costs no extra round trip.
+//
+// With COUNT, n_nodes / n_tris receive the number of 64-byte node and 48-byte triangle fetches
+// (the algorithmic bytes of the traversal).
+constexpr int kLdsStack = 16;  // LDS ring entries per thread
+constexpr int kSpill = 48;      // private spill entries (kLdsStack + kSpill = 64)
+
+template <bool ANY>
+__device__ __forceinline__ bool test_triangle(const RayPrep& r, float tmin, uint32_t mask, uint32_t slot, float4 ta,
+											  float4 tb, float4 tc, HitResult& best) {
+	if((__float_as_uint(tc.w) & mask) == 0u) return false;
+	float t, V, W, det;
+	if(!tri_intersect(r, ta, tb, tc, tmin, best.t, t, V, W, det)) return false;
+	const uint32_t gid = __float_as_uint(ta.w);
+	if(best.slot != 0xFFFFFFFFu && !(t < best.t || gid < best.gid)) return false;
+	best.t = t;
+	best.slot = slot;
+	best.gid = gid;
+	best.meta = __float_as_uint(tc.w);
+	best.V = V;
+	best.W = W;
+	best.det = det;
+	return true;
+}
+
 template <bool ANY, int STACK, bool COUNT = false>
 __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
 											  const RayPrep& r, float tmin, float tmax, uint32_t mask,
 											  uint32_t* stack, int stride, uint32_t& n_nodes_out, uint32_t& n_tris_out) {
+	static_assert((STACK & (STACK - 1)) == 0, "the LDS ring size must be a power of two");
 	HitResult best;
 	best.t = tmax;
 	best.slot = 0xFFFFFFFFu;
 	best.gid = 0xFFFFFFFFu;
 	best.meta = 0u;
 	best.V = best.W = best.det = 0.0f;
-	int sp = 0;
+	uint32_t spill[kSpill];
+	int head = 0, count = 0, nspill = 0;  // ring write position, ring entries, spilled entries
+// pop into `v`, or run `on_empty` when ring and spill are both empty
+#define DDGI_STACK_POP(v, on_empty)                    \
+	if(count > 0) {                                    \
+		head = (head - 1) & (STACK - 1);               \
+		--count;                                       \
+		v = stack[head * stride];                      \
+	} else if(nspill > 0) {                            \
+		v = spill[--nspill];                           \
+	} else {                                           \
+		on_empty;                                      \
+	}
 	uint32_t node = 0;  // root (always an inner node)
 	uint32_t n_nodes = 0, n_tris = 0;
 	for(;;) {
@@ -162,45 +205,44 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 			if(h0 && h1) {
 				const bool swap = t1 < t0;
 				node = swap ? c1 : c0;
-				stack[sp * stride] = swap ? c0 : c1;
-				++sp;
+				if(count == STACK) {  // ring full: its oldest entry (at `head`) moves to the spill array
+					spill[nspill++] = stack[head * stride];
+					--count;
+				}
+				stack[head * stride] = swap ? c0 : c1;
+				head = (head + 1) & (STACK - 1);
+				++count;
 			} else if(h0) {
 				node = c0;
 			} else if(h1) {
 				node = c1;
 			} else {
-				if(sp == 0) goto done;
-				--sp;
-				node = stack[sp * stride];
+				DDGI_STACK_POP(node, goto done)
 			}
 		}
-		// leaf: up to 8 triangles
+		// leaf: up to 8 triangles, two per step
 		{
 			const uint32_t first = (node & ~kLeafFlag) >> 3, cnt = (node & 7u) + 1u;
-			for(uint32_t k = 0; k < cnt; ++k) {
+			for(uint32_t k = 0; k < cnt; k += 2) {
 				const float4* tr = tris + size_t(first + k) * 3;
 				const float4 ta = tr[0], tb = tr[1], tc = tr[2];
-				++n_tris;
-				if((__float_as_uint(tc.w) & mask) == 0u) continue;
-				float t, V, W, det;
-				if(!tri_intersect(r, ta, tb, tc, tmin, best.t, t, V, W, det)) continue;
-				const uint32_t gid = __float_as_uint(ta.w);
-				if(best.slot != 0xFFFFFFFFu && !(t < best.t || gid < best.gid)) continue;
-				best.t = t;
-				best.slot = first + k;
-				best.gid = gid;
-				best.meta = __float_as_uint(tc.w);
-				best.V = V;
-				best.W = W;
-				best.det = det;
-				if(ANY) goto done;
+				const bool two = k + 1 < cnt;
+				float4 ua = ta, ub = tb, uc = tc;
+				if(two) {
+					ua = tr[3];
+					ub = tr[4];
+					uc = tr[5];
+				}
+				n_tris += two ? 2u : 1u;
+				// in index order, so that equal-t hits resolve exactly as a sequential scan would
+				if(test_triangle<ANY>(r, tmin, mask, first + k, ta, tb, tc, best) && ANY) goto done;
+				if(two && test_triangle<ANY>(r, tmin, mask, first + k + 1, ua, ub, uc, best) && ANY) goto done;
 			}
 		}
-		if(sp == 0) break;
-		--sp;
-		node = stack[sp * stride];
+		DDGI_STACK_POP(node, break)
 	}
 done:
+#undef DDGI_STACK_POP
 	if(COUNT) {
 		n_nodes_out = n_nodes;
 		n_tris_out = n_tris;

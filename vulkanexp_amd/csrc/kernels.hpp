@@ -190,11 +190,10 @@ struct BatchLaunch {
 
 hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s);
 hipError_t launch_batch_weights(const BatchLaunch& b, hipStream_t s);
-hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t stack, uint32_t block, bool shade, uint32_t grid,
-							  hipStream_t s);
+hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, bool shade, uint32_t grid, hipStream_t s);
 hipError_t launch_batch_blend(const BatchLaunch& b, uint32_t grid, hipStream_t s);
 // Resident workgroups per CU of the persistent kernels (for sizing their grids).
-hipError_t batch_trace_resident(uint32_t stack, uint32_t block, bool shade, int* per_cu);
+hipError_t batch_trace_resident(uint32_t block, bool shade, int* per_cu);
 hipError_t batch_blend_resident(uint32_t mode, uint32_t probes, int* per_cu);
 
 }  // namespace ddgi

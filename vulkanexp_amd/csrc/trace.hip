@@ -456,13 +456,8 @@ hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 	if(threads == 0) return hipSuccess;
 	const dim3 grid(blocks_for(threads, kTraceBlock));
 	const bool cnt = a.visit_counters != nullptr;
-	if(a.scene.stack_depth <= 32) {
-		if(cnt) hipLaunchKernelGGL((k_trace_primary<32, true>), grid, dim3(kTraceBlock), 0, s, a);
-		else hipLaunchKernelGGL((k_trace_primary<32, false>), grid, dim3(kTraceBlock), 0, s, a);
-	} else {
-		if(cnt) hipLaunchKernelGGL((k_trace_primary<64, true>), grid, dim3(kTraceBlock), 0, s, a);
-		else hipLaunchKernelGGL((k_trace_primary<64, false>), grid, dim3(kTraceBlock), 0, s, a);
-	}
+	if(cnt) hipLaunchKernelGGL((k_trace_primary<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, a);
+	else hipLaunchKernelGGL((k_trace_primary<kLdsStack, false>), grid, dim3(kTraceBlock), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -470,9 +465,8 @@ hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
 	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;  // upper bound of the shade list
 	if(threads == 0) return hipSuccess;
 	const bool cnt = a.visit_counters != nullptr;
-	const void* f = a.scene.stack_depth <= 32
-						? (cnt ? reinterpret_cast<const void*>(&k_shade<32, true>) : reinterpret_cast<const void*>(&k_shade<32, false>))
-						: (cnt ? reinterpret_cast<const void*>(&k_shade<64, true>) : reinterpret_cast<const void*>(&k_shade<64, false>));
+	const void* f = cnt ? reinterpret_cast<const void*>(&k_shade<kLdsStack, true>)
+						: reinterpret_cast<const void*>(&k_shade<kLdsStack, false>);
 	// grid-stride: as many workgroups as fit on the device at once (the list length is on the device)
 	uint64_t resident = 0;
 	const hipError_t e = resident_blocks(f, kTraceBlock, &resident);
@@ -487,23 +481,15 @@ hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit*
 							 bool any_hit, hipStream_t s) {
 	if(count == 0) return hipSuccess;
 	const dim3 grid(blocks_for(count, kTraceBlock));
-	if(sc.stack_depth <= 32) {
-		if(any_hit) hipLaunchKernelGGL((k_trace_rays<32, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
-		else hipLaunchKernelGGL((k_trace_rays<32, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
-	} else {
-		if(any_hit) hipLaunchKernelGGL((k_trace_rays<64, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
-		else hipLaunchKernelGGL((k_trace_rays<64, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
-	}
+	if(any_hit) hipLaunchKernelGGL((k_trace_rays<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+	else hipLaunchKernelGGL((k_trace_rays<kLdsStack, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
 	return hipGetLastError();
 }
 
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
 							  uint32_t* backface_counts, hipStream_t s) {
 	if(g.probe_count == 0) return hipSuccess;
-	if(sc.stack_depth <= 32)
-		hipLaunchKernelGGL(k_init_probes<32>, dim3(g.probe_count), dim3(kTraceBlock), 0, s, sc, g, dirs512, states, backface_counts);
-	else
-		hipLaunchKernelGGL(k_init_probes<64>, dim3(g.probe_count), dim3(kTraceBlock), 0, s, sc, g, dirs512, states, backface_counts);
+	hipLaunchKernelGGL(k_init_probes<kLdsStack>, dim3(g.probe_count), dim3(kTraceBlock), 0, s, sc, g, dirs512, states, backface_counts);
 	return hipGetLastError();
 }
 
@@ -534,19 +520,11 @@ hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float
 namespace {
 
 template <bool SHADE>
-const void* batch_trace_kernel(uint32_t stack, uint32_t block) {
-	if(stack <= 32) {
-		switch(block) {
-			case 64: return reinterpret_cast<const void*>(&k_batch_trace<32, 64, SHADE>);
-			case 128: return reinterpret_cast<const void*>(&k_batch_trace<32, 128, SHADE>);
-			case 256: return reinterpret_cast<const void*>(&k_batch_trace<32, 256, SHADE>);
-		}
-	} else {
-		switch(block) {
-			case 64: return reinterpret_cast<const void*>(&k_batch_trace<64, 64, SHADE>);
-			case 128: return reinterpret_cast<const void*>(&k_batch_trace<64, 128, SHADE>);
-			case 256: return reinterpret_cast<const void*>(&k_batch_trace<64, 256, SHADE>);
-		}
+const void* batch_trace_kernel(uint32_t block) {
+	switch(block) {
+		case 64: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 64, SHADE>);
+		case 128: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 128, SHADE>);
+		case 256: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 256, SHADE>);
 	}
 	return nullptr;
 }
@@ -558,15 +536,14 @@ hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s) {
 	return hipGetLastError();
 }
 
-hipError_t batch_trace_resident(uint32_t stack, uint32_t block, bool shade, int* per_cu) {
-	const void* f = shade ? batch_trace_kernel<true>(stack, block) : batch_trace_kernel<false>(stack, block);
+hipError_t batch_trace_resident(uint32_t block, bool shade, int* per_cu) {
+	const void* f = shade ? batch_trace_kernel<true>(block) : batch_trace_kernel<false>(block);
 	if(!f) return hipErrorInvalidValue;
 	return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, int(block), 0);
 }
 
-hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t stack, uint32_t block, bool shade, uint32_t grid,
-							  hipStream_t s) {
-	const void* f = shade ? batch_trace_kernel<true>(stack, block) : batch_trace_kernel<false>(stack, block);
+hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, bool shade, uint32_t grid, hipStream_t s) {
+	const void* f = shade ? batch_trace_kernel<true>(block) : batch_trace_kernel<false>(block);
 	if(!f || grid == 0) return hipErrorInvalidValue;
 	BatchLaunch arg = b;
 	void* args[] = {&arg};
