@@ -7,7 +7,7 @@ irradiance/depth blend with hysteresis + border copy (fused) + publish.
 
 Prints ONE JSON line (rank 0).  `value` = primary probe rays per second of whole updates, in
 Mrays/s.  `roofline` describes the kernel with the largest share of the step; its algorithmic
-bytes come from the traversal's own fetch counters (64-byte nodes, 48-byte triangles) of an
+bytes come from the traversal's own fetch counters (128-byte 4-wide nodes, 48-byte triangles) of an
 untimed counting update.  `cpu_baseline` times the CPU oracle (FP64 shading, FP32 intersection
 replica) through its thread pool on a bounded sample of the same frame.
 """

@@ -169,7 +169,7 @@ typedef struct ddgi_timings {
 
 /* Traversal statistics of the last update run with DDGI_UPDATE_COUNT_VISITS:
  * [0] primary node fetches, [1] primary triangle fetches, [2] shadow node fetches,
- * [3] shadow triangle fetches, [4] shadow rays.  Node = 64 bytes, triangle = 48 bytes. */
+ * [3] shadow triangle fetches, [4] shadow rays.  Node = 128 bytes (4-wide), triangle = 48 bytes. */
 #define DDGI_TRACE_STATS 5
 
 /* Raw device view of a handle's buffers for zero-copy interop (torch). Valid until the next

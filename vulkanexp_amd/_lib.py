@@ -23,7 +23,7 @@ UPDATE_BLEND_FP16 = 0x4
 UPDATE_HYSTERESIS_RAMP = 0x8
 UPDATE_NO_TIMING = 0x10
 UPDATE_COUNT_VISITS = 0x20
-NODE_BYTES, TRIANGLE_BYTES = 64, 48
+NODE_BYTES, TRIANGLE_BYTES = 128, 48  # 4-wide BVH node, triangle record
 ATLAS_IRRADIANCE, ATLAS_DEPTH = 0, 1
 LAYOUT_TILED, LAYOUT_IMAGE = 0, 1
 TRACE_ANY_HIT = 0x1

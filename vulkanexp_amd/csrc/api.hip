@@ -474,7 +474,8 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	if(!build_bvh(vertices, vertex_count, indices, index_count, offsets, offset_count, instances, instance_count, bvh, &why))
 		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, why ? why : "invalid scene");
 	if(bvh.tri_count == 0) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "scene has no triangles");
-	if(bvh.max_depth >= 64) return fail(h, DDGI_ERROR_UNSUPPORTED, "BVH deeper than the 64-entry traversal stack");
+	if(bvh.wide_depth > kMaxWideDepth)
+		return fail(h, DDGI_ERROR_UNSUPPORTED, "BVH deeper than the traversal stack (32 levels of the 4-wide tree)");
 	ddgi_status st = set_device(h);
 	if(st != DDGI_OK) return st;
 	(void)hipDeviceSynchronize();  // the previous scene may still be read by queued work
@@ -523,7 +524,7 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	sd.tri_count = bvh.tri_count;
 	sd.node_count = bvh.node_count;
 	sd.instance_count = uint32_t(instance_count);
-	sd.stack_depth = bvh.max_depth < 32 ? 32 : 64;
+	sd.stack_depth = 3 * bvh.wide_depth;  // entries the traversal can hold at once
 	h->sd = sd;
 	h->bvh_bytes = bvh.nodes.size() * 4 + bvh.tris.size() * 4;
 	bvh.nodes.clear();

@@ -259,21 +259,6 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 	b.build(uint32_t(total));
 	out.max_depth = b.max_depth;
 
-	// ---- emit: inner nodes in depth-first order, children boxes stored in the parent ----
-	std::vector<uint32_t> emitted(b.nodes.size(), 0xFFFFFFFFu);
-	std::vector<int32_t> inner_order;
-	{
-		std::vector<int32_t> st{0};
-		while(!st.empty()) {
-			int32_t n = st.back();
-			st.pop_back();
-			if(b.nodes[n].leaf()) continue;
-			emitted[n] = uint32_t(inner_order.size());
-			inner_order.push_back(n);
-			st.push_back(b.nodes[n].right);
-			st.push_back(b.nodes[n].left);
-		}
-	}
 	// triangles in leaf order
 	out.tris.resize(total * 12);
 	for(size_t i = 0; i < total; ++i) {
@@ -288,48 +273,89 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		d[7] = u2f(inst_of[t]);
 		d[11] = u2f(meta[t]);
 	}
-	auto child_code = [&](int32_t c) -> uint32_t {
-		const BuildNode& n = b.nodes[c];
-		if(n.leaf()) return kLeafFlag | (n.first << 3) | (n.count - 1);
-		return emitted[c];
-	};
-	auto put_box = [](float* rec, int which, const Aabb& bx) {
-		float lo[3], hi[3];
-		for(int k = 0; k < 3; ++k) {
-			lo[k] = pad_down(bx.lo[k]);
-			hi[k] = pad_up(bx.hi[k]);
+	// ---- collapse to a 4-wide tree: each wide node takes the children of a binary node and keeps
+	// opening its largest-area inner child until it has four ----
+	auto leaf_code = [&](const BuildNode& n) { return kLeafFlag | (n.first << 3) | (n.count - 1); };
+	auto wide_children = [&](int32_t n, int32_t ch[4]) -> int {
+		int cnt = 0;
+		if(b.nodes[n].leaf()) {  // the whole scene is one leaf: a root with that leaf as its only child
+			ch[cnt++] = n;
+			return cnt;
 		}
-		float* xy = rec + which * 4;
-		xy[0] = lo[0];
-		xy[1] = hi[0];
-		xy[2] = lo[1];
-		xy[3] = hi[1];
-		rec[8 + which * 2 + 0] = lo[2];
-		rec[8 + which * 2 + 1] = hi[2];
-	};
-	if(inner_order.empty()) {
-		// the whole scene is one leaf: a root whose two children are that same leaf
-		out.nodes.assign(16, 0.0f);
-		put_box(out.nodes.data(), 0, b.nodes[0].box);
-		put_box(out.nodes.data(), 1, b.nodes[0].box);
-		const uint32_t code = kLeafFlag | (b.nodes[0].first << 3) | (b.nodes[0].count - 1);
-		out.nodes[12] = u2f(code);
-		out.nodes[13] = u2f(code);
-		out.node_count = 1;
-	} else {
-		out.nodes.assign(inner_order.size() * 16, 0.0f);
-		for(size_t i = 0; i < inner_order.size(); ++i) {
-			const BuildNode& n = b.nodes[inner_order[i]];
-			float* rec = &out.nodes[i * 16];
-			put_box(rec, 0, b.nodes[n.left].box);
-			put_box(rec, 1, b.nodes[n.right].box);
-			rec[12] = u2f(child_code(n.left));
-			rec[13] = u2f(child_code(n.right));
-			rec[14] = 0.0f;
-			rec[15] = 0.0f;
+		ch[cnt++] = b.nodes[n].left;
+		ch[cnt++] = b.nodes[n].right;
+		while(cnt < 4) {
+			int pick = -1;
+			double area = -1.0;
+			for(int k = 0; k < cnt; ++k) {
+				const BuildNode& c = b.nodes[ch[k]];
+				if(!c.leaf() && c.box.half_area() > area) {
+					area = c.box.half_area();
+					pick = k;
+				}
+			}
+			if(pick < 0) break;
+			const int32_t open = ch[pick];
+			ch[pick] = b.nodes[open].left;
+			ch[cnt++] = b.nodes[open].right;
 		}
-		out.node_count = uint32_t(inner_order.size());
+		return cnt;
+	};
+	struct Wide {
+		int32_t bin;       // binary node it was made from
+		uint32_t depth;
+		int32_t kids[4];   // binary child nodes, -1 = empty slot
+	};
+	// depth first: the children of a node get consecutive indices, then each child's subtree in turn
+	std::vector<Wide> wide{{0, 1, {-1, -1, -1, -1}}};
+	std::vector<uint32_t> todo{0};
+	uint32_t wide_depth = 0;
+	while(!todo.empty()) {
+		const uint32_t w = todo.back();
+		todo.pop_back();
+		int32_t ch[4];
+		const int cnt = wide_children(wide[w].bin, ch);
+		wide_depth = std::max(wide_depth, wide[w].depth);
+		uint32_t created[4];
+		int ncreated = 0;
+		for(int k = 0; k < 4; ++k) {
+			wide[w].kids[k] = k < cnt ? ch[k] : -1;
+			if(k < cnt && !b.nodes[ch[k]].leaf()) {
+				created[ncreated++] = uint32_t(wide.size());
+				wide.push_back({ch[k], wide[w].depth + 1, {-1, -1, -1, -1}});
+			}
+		}
+		for(int k = ncreated - 1; k >= 0; --k) todo.push_back(created[k]);
 	}
+	out.wide_depth = wide_depth;
+	// node indices in the order wide nodes were created
+	std::vector<uint32_t> wide_index(b.nodes.size(), 0xFFFFFFFFu);
+	for(size_t w = 0; w < wide.size(); ++w) wide_index[wide[w].bin] = uint32_t(w);
+	out.nodes.assign(wide.size() * kNodeFloats, 0.0f);
+	for(size_t w = 0; w < wide.size(); ++w) {
+		float* rec = &out.nodes[w * kNodeFloats];
+		for(int k = 0; k < 4; ++k) {
+			const int32_t c = wide[w].kids[k];
+			float lo[3], hi[3];
+			uint32_t code = kEmptyChild;
+			if(c >= 0) {
+				const BuildNode& n = b.nodes[c];
+				for(int a = 0; a < 3; ++a) {
+					lo[a] = pad_down(n.box.lo[a]);
+					hi[a] = pad_up(n.box.hi[a]);
+				}
+				code = n.leaf() ? leaf_code(n) : wide_index[c];
+			} else {
+				for(int a = 0; a < 3; ++a) lo[a] = hi[a] = 1e30f;  // unreachable point box
+			}
+			for(int a = 0; a < 3; ++a) {
+				rec[(2 * a) * 4 + k] = lo[a];
+				rec[(2 * a + 1) * 4 + k] = hi[a];
+			}
+			rec[24 + k] = u2f(code);
+		}
+	}
+	out.node_count = uint32_t(wide.size());
 	// SAH cost of the emitted tree (for reporting)
 	double cost = 0.0;
 	const double root_area = std::max(b.nodes[0].box.half_area(), 1e-30);

@@ -3,18 +3,19 @@
 // so traversal is single-level.
 //
 // Device layout (all float4 / 16-byte aligned, read with 128-bit loads):
-//   node k  : 4 float4 = 64 bytes, holding the boxes of BOTH children, so one node fetch decides
-//             which children a ray enters (layout after Aila & Laine, HPG 2009):
-//               [0] = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
-//               [1] = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
-//               [2] = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
-//               [3] = (child0, child1, -, -) as int bits; child >= 0: inner node index,
-//                     child < 0: leaf = 0x80000000 | first_triangle << 3 | (count - 1)
+//   node k  : 8 float4 = 128 bytes, a 4-wide node collapsed from the binary SAH tree, holding the
+//             boxes of its (up to) four children side by side so that one node fetch tests all four:
+//               [0] = lo.x of children 0..3     [1] = hi.x     [2] = lo.y
+//               [3] = hi.y                      [4] = lo.z     [5] = hi.z
+//               [6] = child codes 0..3 as int bits: inner node index, or
+//                     leaf = 0x80000000 | first_triangle << 3 | (count - 1), or kEmptyChild
+//               [7] = unused
 //   triangle: 3 float4 = 48 bytes, leaf order:
 //               [0] = (v0.xyz, global triangle id)   global id = instance-major flattened index
 //               [1] = (v1.xyz, instance index)
 //               [2] = (v2.xyz, mask | flip_facing << 8)
-// Node boxes are inflated by a few ulps so that FP32 slab tests stay conservative.
+// Node boxes are inflated by a few ulps so that FP32 slab tests stay conservative; an empty child
+// slot has the code kEmptyChild and a point box far outside any scene.
 #pragma once
 
 #include <cstdint>
@@ -25,15 +26,19 @@
 namespace ddgi {
 
 constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr uint32_t kMaxLeafSize = 8;
+constexpr uint32_t kNodeFloats = 32;     // 128-byte wide node
+constexpr uint32_t kMaxWideDepth = 32;   // traversal stack: 3 entries per level (16 LDS + 80 private)
 
 struct BvhHost {
-	std::vector<float> nodes;     // 16 floats per node
+	std::vector<float> nodes;     // kNodeFloats per node
 	std::vector<float> tris;      // 12 floats per triangle (leaf order)
 	std::vector<uint32_t> inst_prim_base;  // per instance: global id of its triangle 0
 	uint32_t node_count = 0;
 	uint32_t tri_count = 0;
-	uint32_t max_depth = 0;
+	uint32_t max_depth = 0;       // binary tree
+	uint32_t wide_depth = 0;      // levels of the 4-wide tree
 	double sah_cost = 0.0;
 };
 

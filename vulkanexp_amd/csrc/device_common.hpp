@@ -134,19 +134,27 @@ struct HitResult {
 
 // Closest-hit (ANY = false; ties -> lowest global id) or any-hit traversal of the BVH.
 //
+// Nodes are 4-wide (bvh.hpp): one 128-byte fetch tests four child boxes.  The children the ray
+// enters are sorted by entry distance (a five-comparator network on registers); the nearest is
+// visited next and the others are pushed farthest first, so the next pop is the nearest remaining.
+// The result does not depend on the visiting order: closest hit is the minimum of (t, global id)
+// over every triangle the conservative box tests let through.
+//
 // Stack: the newest STACK entries live in an LDS ring laid out [entry][thread] (conflict-free;
 // `stack` points at this thread's column, `stride` is the block size); when the ring is full its
 // oldest entry moves to a private spill array, and pops drain the ring first, then the spill, so
-// the order stays LIFO.  Capacity STACK + kSpill >= the deepest BVH accepted (63 levels), while the
-// LDS footprint stays small enough for full occupancy.
+// the order stays LIFO.  At most 3 entries per level of the wide tree are pending, and
+// STACK + kSpill covers the deepest tree accepted (kMaxWideDepth levels), while the LDS footprint
+// stays small enough for full occupancy.
 //
 // Leaves are tested two triangles at a time, each triangle's 48 bytes loaded in one go, so that
 // two loads are in flight and the mask word costs no extra round trip.
 //
-// With COUNT, n_nodes / n_tris receive the number of 64-byte node and 48-byte triangle fetches
+// With COUNT, n_nodes / n_tris receive the number of 128-byte node and 48-byte triangle fetches
 // (the algorithmic bytes of the traversal).
 constexpr int kLdsStack = 16;  // LDS ring entries per thread
-constexpr int kSpill = 48;      // private spill entries (kLdsStack + kSpill = 64)
+constexpr int kSpill = 80;      // private spill entries: kLdsStack + kSpill = 3 * kMaxWideDepth
+static_assert(kLdsStack + kSpill >= 3 * int(kMaxWideDepth), "traversal stack too small for the deepest tree");
 
 template <bool ANY>
 __device__ __forceinline__ bool test_triangle(const RayPrep& r, float tmin, uint32_t mask, uint32_t slot, float4 ta,
@@ -190,35 +198,62 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 	} else {                                           \
 		on_empty;                                      \
 	}
+// push `v`; when the ring is full its oldest entry (at `head`) moves to the spill array first
+#define DDGI_STACK_PUSH(v)                             \
+	{                                                  \
+		if(count == STACK) {                           \
+			spill[nspill++] = stack[head * stride];    \
+			--count;                                   \
+		}                                              \
+		stack[head * stride] = (v);                    \
+		head = (head + 1) & (STACK - 1);               \
+		++count;                                       \
+	}
+// order (key, code) pairs i, j by key
+#define DDGI_CSWAP(i, j)                               \
+	{                                                  \
+		const bool sw_ = key[j] < key[i];              \
+		const float ki_ = key[i], kj_ = key[j];        \
+		const uint32_t ci_ = code[i], cj_ = code[j];   \
+		key[i] = sw_ ? kj_ : ki_;                      \
+		key[j] = sw_ ? ki_ : kj_;                      \
+		code[i] = sw_ ? cj_ : ci_;                     \
+		code[j] = sw_ ? ci_ : cj_;                     \
+	}
 	uint32_t node = 0;  // root (always an inner node)
 	uint32_t n_nodes = 0, n_tris = 0;
+	const float kInf = __int_as_float(0x7f800000);
 	for(;;) {
 		// descend through inner nodes
 		while(!(node & kLeafFlag)) {
-			const float4* nr = nodes + size_t(node) * 4;
-			const float4 a = nr[0], b = nr[1], c = nr[2], d = nr[3];
+			const float4* nr = nodes + size_t(node) * (kNodeFloats / 4);
+			const float4 lx = nr[0], hx = nr[1], ly = nr[2], hy = nr[3], lz = nr[4], hz = nr[5], cc = nr[6];
 			++n_nodes;
-			float t0, t1;
-			const bool h0 = slab(r, a.x, a.y, a.z, a.w, c.x, c.y, tmin, best.t, t0);
-			const bool h1 = slab(r, b.x, b.y, b.z, b.w, c.z, c.w, tmin, best.t, t1);
-			const uint32_t c0 = __float_as_uint(d.x), c1 = __float_as_uint(d.y);
-			if(h0 && h1) {
-				const bool swap = t1 < t0;
-				node = swap ? c1 : c0;
-				if(count == STACK) {  // ring full: its oldest entry (at `head`) moves to the spill array
-					spill[nspill++] = stack[head * stride];
-					--count;
-				}
-				stack[head * stride] = swap ? c0 : c1;
-				head = (head + 1) & (STACK - 1);
-				++count;
-			} else if(h0) {
-				node = c0;
-			} else if(h1) {
-				node = c1;
-			} else {
+			float key[4];
+			uint32_t code[4];
+			code[0] = __float_as_uint(cc.x);
+			code[1] = __float_as_uint(cc.y);
+			code[2] = __float_as_uint(cc.z);
+			code[3] = __float_as_uint(cc.w);
+			float tn;
+			key[0] = (slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, best.t, tn) && code[0] != kEmptyChild) ? tn : kInf;
+			key[1] = (slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, best.t, tn) && code[1] != kEmptyChild) ? tn : kInf;
+			key[2] = (slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, best.t, tn) && code[2] != kEmptyChild) ? tn : kInf;
+			key[3] = (slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, best.t, tn) && code[3] != kEmptyChild) ? tn : kInf;
+			const int nh = int(key[0] != kInf) + int(key[1] != kInf) + int(key[2] != kInf) + int(key[3] != kInf);
+			if(nh == 0) {
 				DDGI_STACK_POP(node, goto done)
+				continue;
 			}
+			DDGI_CSWAP(0, 1)
+			DDGI_CSWAP(2, 3)
+			DDGI_CSWAP(0, 2)
+			DDGI_CSWAP(1, 3)
+			DDGI_CSWAP(1, 2)
+			node = code[0];
+			if(nh > 3) DDGI_STACK_PUSH(code[3])
+			if(nh > 2) DDGI_STACK_PUSH(code[2])
+			if(nh > 1) DDGI_STACK_PUSH(code[1])
 		}
 		// leaf: up to 8 triangles, two per step
 		{
@@ -243,6 +278,8 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 	}
 done:
 #undef DDGI_STACK_POP
+#undef DDGI_STACK_PUSH
+#undef DDGI_CSWAP
 	if(COUNT) {
 		n_nodes_out = n_nodes;
 		n_tris_out = n_tris;

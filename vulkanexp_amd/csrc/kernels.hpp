@@ -61,7 +61,7 @@ struct SceneDev {
 	uint32_t tri_count;
 	uint32_t node_count;
 	uint32_t instance_count;
-	uint32_t stack_depth;      // 32 or 64
+	uint32_t stack_depth;      // most stack entries a traversal needs (3 per level of the 4-wide tree)
 };
 
 // Internal hit record: (t, slot | back << 31, u, v); slot = triangle position in the BVH array,
@@ -117,6 +117,7 @@ struct BlendArgs {
 constexpr uint32_t kBlendIrradianceOnly = 0x1;
 
 // ---- launchers (return hipGetLastError of the launch) ----
+// zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);

@@ -36,8 +36,8 @@ __device__ __forceinline__ float4 frame_dir(const Rot& R, float4 f) {
 	return make_float4(d[0], d[1], d[2], 0.0f);
 }
 
-// `zero` (optional): a counter the next kernels start from, cleared here so that no separate
-// memset is needed.
+// `zero` (optional): a counter the next kernels start from (the shade-list length), cleared here so
+// that no separate memset is needed.
 __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count,
 							 uint32_t* zero) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,6 +60,18 @@ __device__ __forceinline__ void write_unshaded(const TraceFrameArgs& a, size_t r
 }
 
 // ---- primary probe rays: closest hit, mask STATIC|DYNAMIC, tmin 0.01, tmax = |max - min| ----
+// Result of the ray at position ri (= slot * n + k, thread order): its hit record, and either its
+// final ray record (miss, back face) or a shade-list entry (front-face hit).
+__device__ __forceinline__ void finish_primary_ray(const TraceFrameArgs& a, uint32_t ri, const HitResult& h) {
+	const float4 packed = pack_hit(h);
+	a.hits[ri] = packed;
+	const uint32_t code = __float_as_uint(packed.y);
+	const bool front = code != 0xFFFFFFFFu && !(code & 0x80000000u);
+	const uint32_t pos = wave_append(a.shade_count, front);  // front-face hits are queued for shading
+	if(front) a.shade_list[pos] = ri;
+	else write_unshaded(a, ri, code == 0xFFFFFFFFu, h.t);
+}
+
 // Traversal thread `gtid` = slot * n + j traces ray perm[j] of the probe in `slot`; `stack` is the
 // thread's column of an LDS stack with `stride` columns.
 template <int STACK, bool COUNT>
@@ -76,14 +88,7 @@ __device__ __forceinline__ void trace_primary_ray(const TraceFrameArgs& a, uint3
 	uint32_t nn = 0, nt = 0;
 	const HitResult h = traverse<false, STACK, COUNT>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
 													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack, stride, nn, nt);
-	const size_t ri = size_t(slot) * n + j;  // thread order: coalesced stores
-	const float4 packed = pack_hit(h);
-	a.hits[ri] = packed;
-	const uint32_t code = __float_as_uint(packed.y);
-	const bool front = code != 0xFFFFFFFFu && !(code & 0x80000000u);
-	const uint32_t pos = wave_append(a.shade_count, front);  // front-face hits are queued for shading
-	if(front) a.shade_list[pos] = uint32_t(ri);
-	else write_unshaded(a, ri, code == 0xFFFFFFFFu, h.t);
+	finish_primary_ray(a, slot * n + j, h);  // thread order: coalesced stores
 	if(COUNT) {
 		atomicAdd(&a.visit_counters[0], (unsigned long long)nn);
 		atomicAdd(&a.visit_counters[1], (unsigned long long)nt);
@@ -195,9 +200,10 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 	}
 }
 
-// Grid-stride over the shade list (its length is known only on the device).
+// Grid-stride over the shade list (its length is known only on the device).  At least 5 waves per
+// SIMD: the shadow traversal is latency-bound, and the register budget of 5 waves costs no spills.
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_shade(TraceFrameArgs a) {
+__global__ __launch_bounds__(kTraceBlock, 5) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
 	const uint32_t count = *a.shade_count;
 	const uint32_t stride = gridDim.x * kTraceBlock;
@@ -455,8 +461,7 @@ hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
 	if(threads == 0) return hipSuccess;
 	const dim3 grid(blocks_for(threads, kTraceBlock));
-	const bool cnt = a.visit_counters != nullptr;
-	if(cnt) hipLaunchKernelGGL((k_trace_primary<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, a);
+	if(a.visit_counters) hipLaunchKernelGGL((k_trace_primary<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, a);
 	else hipLaunchKernelGGL((k_trace_primary<kLdsStack, false>), grid, dim3(kTraceBlock), 0, s, a);
 	return hipGetLastError();
 }
