@@ -73,6 +73,24 @@ def cpu_baseline(scene, grid, light, sky, R, n_probes, threads):
             "update_ms_extrapolated": round(og.count * n / (rate * 1e6) * 1e3, 1)}
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "ddgi_update_large_rocprof.md")
+PROFILE_KERNELS = {"trace_primary": "k_trace_primary<16, false>", "shade_shadow": "k_shade<16, false>",
+                   "blend": "k_blend_fp32"}
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size) from
+    the committed rocprofv3 summary of this workload; None if the summary does not list it."""
+    name = PROFILE_KERNELS.get(kernel)
+    if name is None or not os.path.exists(PROFILE_SUMMARY):
+        return None
+    for line in open(PROFILE_SUMMARY):
+        parts = line.strip().split(" ")
+        if parts[0] == "traffic_per_launch" and " ".join(parts[1:-1]) == name:
+            return int(parts[-1])
+    return None
+
+
 def _event_ms(fn, reps=10):
     import torch
     times = []
@@ -292,7 +310,8 @@ def main():
     dk = kernels[dom]
     achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom),
+                "traffic_source": os.path.relpath(PROFILE_SUMMARY, ROOT),
                 "algorithmic_bytes_per_launch": int(dk["bytes"]), "avg_ms": round(dk["ms"], 4)}
     for k, v in kernels.items():
         v["GB/s"] = round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
