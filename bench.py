@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-blend-compare", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 3 elementwise copies of 256 MiB (torch.add(x, 0)) so that profiler byte "
                          "counters can be scaled against a kernel of known traffic")
@@ -133,6 +134,23 @@ def blend_compare(probes, R, lib):
     out["bf16_speedup_vs_fp32"] = round(out["fp32"]["blend_ms"] / out["bf16"]["blend_ms"], 2)
     out["fp16_speedup_vs_fp32"] = round(out["fp32"]["blend_ms"] / out["fp16"]["blend_ms"], 2)
     return out
+
+
+def frame_bench(probes, scene, width=1920, height=1080):
+    """Full-frame consumer on the large scene: G-buffer of a 1920x1080 pinhole camera (back faces
+    culled) and the final gather (hard sun shadow ray + sampleProbes + PBR) over it."""
+    import torch
+    from vulkanexp_amd import gbuffer
+    eye = (8.0, 9.0, 6.0)
+    rays = torch.from_numpy(gbuffer.camera_rays(width, height, eye=eye, look_at=(30.0, 2.0, 20.0))).cuda()
+    gb = probes.gbuffer(rays)
+    probes.gather(gb, eye)
+    g_ms = _event_ms(lambda: probes.gbuffer(rays))
+    f_ms = _event_ms(lambda: probes.gather(gb, eye))
+    hit = float((gb[:, 3] > 0).float().mean())
+    px = width * height
+    return {"pixels": px, "hit_fraction": round(hit, 3), "gbuffer_ms": round(g_ms, 4), "gather_ms": round(f_ms, 4),
+            "Mpixels/s": round(px / ((g_ms + f_ms) * 1e-3) / 1e6, 1)}
 
 
 def diff_bench(lib):
@@ -320,6 +338,7 @@ def main():
     trace_stats = {"nodes_per_primary_ray": st["primary_nodes"] / rays, "tris_per_primary_ray": st["primary_tris"] / rays,
                    "shadow_rays": st["shadow_rays"], "nodes_per_shadow_ray": st["shadow_nodes"] / max(1, st["shadow_rays"])}
 
+    frame = None if args.no_frame else frame_bench(probes, scene)
     blend = None if args.no_blend_compare else blend_compare(probes, Rs[-1], _lib)
     diff = None if args.no_diff else diff_bench(_lib)
     batched = None if args.no_batch else batch_bench(_lib)
@@ -347,7 +366,7 @@ def main():
         "phases_ms": {k: round(v, 4) for k, v in avg.items()},
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
-        "blend_precision": blend, "differentiable": diff, "batched_volumes": batched,
+        "full_frame": frame, "blend_precision": blend, "differentiable": diff, "batched_volumes": batched,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -150,7 +150,20 @@ typedef struct ddgi_hit {
 	uint32_t flags;
 } ddgi_hit;
 
-#define DDGI_TRACE_ANY_HIT 0x1u /* terminate on first hit (shadow-ray semantics) */
+#define DDGI_TRACE_ANY_HIT 0x1u        /* terminate on first hit (shadow-ray semantics) */
+#define DDGI_TRACE_CULL_BACK_FACES 0x2u /* ignore back-facing triangles (rasterizer back-face culling) */
+
+/* G-buffer texel (the reference's four attachments, GBuffer.frag:63-66): 64 bytes. */
+typedef struct ddgi_gbuffer_texel {
+	float position[3];  /* world position of the nearest front-facing hit */
+	float depth;        /* its distance from the ray origin; 0 on a miss */
+	float normal[3];    /* normalize(normal matrix * interpolated vertex normal) */
+	float metalness;
+	float albedo[3];    /* interpolated vertex colour * base colour */
+	float roughness;
+	float emissive[3];
+	uint32_t material;  /* material index, 0xFFFFFFFF on a miss */
+} ddgi_gbuffer_texel;
 
 /* Per-phase GPU times of the last completed update, in milliseconds (the reference's timestamp
  * split, src/IrradianceProbes.cpp:441-452). */
@@ -237,6 +250,18 @@ ddgi_status ddgi_trace_rays(void* handle, const ddgi_ray* rays, ddgi_hit* hits, 
 
 /* sampleProbes (src/shaders/irradiance.glsl:145-237) at `count` points: device float3 arrays
  * (packed xyz) position, normal, to_camera; writes device float3 rgb. */
+/* ---- full-frame consumer: G-buffer + final gather (no counterpart in IrradianceProbes; the
+ * reference's GBufferPipeline / GatherPipeline, src/RenderPasses/GBufferPipeline.cpp,
+ * src/RenderPasses/GatherPipeline.cpp:3-383, src/shaders/FinalGather.frag:38-77) ---- */
+/* G-buffer of `count` camera rays (device ddgi_ray array) into device texels: the nearest
+ * front-facing hit (back faces culled like GBufferPipeline.cpp:212), mask 0xFF. */
+ddgi_status ddgi_gbuffer(void* handle, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, void* stream);
+/* Final gather of device G-buffer texels into device rgb (count * 3 floats), with the handle's
+ * exposed atlases, light and sky.  view_origin (host, 3 floats): the camera position.  direct
+ * (device, count floats, may be NULL): sun visibility per texel; NULL = a hard shadow ray toward
+ * the sun.  reflection (device, count * 3 floats, may be NULL = black): the reflection pass. */
+ddgi_status ddgi_gather(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
+						const float* direct, const float* reflection, float* out_rgb, void* stream);
 ddgi_status ddgi_sample(void* handle, const float* position, const float* normal, const float* to_camera,
 						uint64_t count, float* rgb_out, void* stream);
 

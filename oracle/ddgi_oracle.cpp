@@ -230,11 +230,14 @@ static inline bool better(float t, uint32_t id, const Hit32& best) {
 	return !best.hit() || t < best.t || (t == best.t && id < best.tri);
 }
 
-static inline bool try_tri(const Scene& s, const Ray32& r, uint32_t id, uint32_t mask, Hit32& best) {
+// `cull`: back-facing triangles are ignored (the G-buffer rasterizer's VK_CULL_MODE_BACK_BIT,
+// src/RenderPasses/GBufferPipeline.cpp:212).
+static inline bool try_tri(const Scene& s, const Ray32& r, uint32_t id, uint32_t mask, Hit32& best, bool cull) {
 	if((s.inst_mask[s.tri_inst[id]] & mask) == 0) return false;
 	const float* v = &s.tri[uint64_t(id) * 9];
 	float t, U, V, W, det;
 	if(!tri_intersect32(r, v, v + 3, v + 6, t, U, V, W, det)) return false;
+	if(cull && ((det < 0.0f) != (s.tri_flip[id] != 0))) return false;
 	if(!better(t, id, best)) return false;
 	best.t = t;
 	best.tri = id;
@@ -244,12 +247,12 @@ static inline bool try_tri(const Scene& s, const Ray32& r, uint32_t id, uint32_t
 	return true;
 }
 
-Hit32 trace32(const Scene& s, const Ray32& r, uint32_t mask, bool any_hit, bool brute, uint64_t* stats) {
+Hit32 trace32(const Scene& s, const Ray32& r, uint32_t mask, bool any_hit, bool brute, uint64_t* stats, bool cull) {
 	Hit32 best;
 	const size_t n = s.tri_count();
 	if(brute || s.nodes.empty()) {
 		for(size_t i = 0; i < n; ++i) {
-			if(try_tri(s, r, uint32_t(i), mask, best) && any_hit) return best;
+			if(try_tri(s, r, uint32_t(i), mask, best, cull) && any_hit) return best;
 		}
 		if(stats) stats[1] += n;
 		return best;
@@ -275,7 +278,7 @@ Hit32 trace32(const Scene& s, const Ray32& r, uint32_t mask, bool any_hit, bool 
 		if(nd.count > 0) {
 			for(uint32_t k = 0; k < nd.count; ++k) {
 				if(stats) stats[1] += 1;
-				if(try_tri(s, r, s.order[nd.first + k], mask, best) && any_hit) return best;
+				if(try_tri(s, r, s.order[nd.first + k], mask, best, cull) && any_hit) return best;
 			}
 			continue;
 		}
@@ -299,7 +302,7 @@ Hit32 trace32(const Scene& s, const Ray32& r, uint32_t mask, bool any_hit, bool 
 			stack[sp++] = nd.first + 1;
 		}
 		if(sp > 126) {  // cannot happen for depth <= 64 trees; degrade to brute force if it does
-			return trace32(s, r, mask, any_hit, true, stats);
+			return trace32(s, r, mask, any_hit, true, stats, cull);
 		}
 	}
 	return best;
@@ -946,9 +949,11 @@ void orc_scene_world_tris(void* h, float* out) {
 }
 
 // ---- ray queries: rays = n * 8 floats (o.xyz, tmin, d.xyz, tmax); out = n * 6 uint32 records
-//      {t bits, instance, primitive, flags(back), u bits, v bits}; stats = {node visits, tri tests}
-void orc_trace(void* h, const float* rays, uint64_t n, uint32_t mask, int any_hit, int brute, uint32_t* out,
+//      {t bits, instance, primitive, flags(back), u bits, v bits}; stats = {node visits, tri tests};
+//      flags: 1 = any hit, 2 = cull back faces
+void orc_trace(void* h, const float* rays, uint64_t n, uint32_t mask, int flags, int brute, uint32_t* out,
 			   uint64_t* stats) {
+	const bool any_hit = (flags & 1) != 0, cull = (flags & 2) != 0;
 	const Scene& s = static_cast<OracleScene*>(h)->s;
 	std::vector<uint64_t> st(2 * ((n + 255) / 256), 0);
 	ThreadPool::parallel_for(n, 256, [&](uint64_t b, uint64_t e) {
@@ -962,7 +967,7 @@ void orc_trace(void* h, const float* rays, uint64_t n, uint32_t mask, int any_hi
 			r.tmin = rays[i * 8 + 3];
 			r.tmax = rays[i * 8 + 7];
 			ray32_prepare(r);
-			Hit32 hh = trace32(s, r, mask, any_hit != 0, brute != 0, ls);
+			Hit32 hh = trace32(s, r, mask, any_hit, brute != 0, ls, cull);
 			fill_hit_record(s, hh, &out[i * 6]);
 		}
 	});

@@ -202,7 +202,9 @@ bool tri_intersect32(const Ray32& r, const float* v0, const float* v1, const flo
 
 // Closest hit (ties: lowest global triangle id) or any hit, through the oracle BVH or by brute
 // force over every triangle. `stats` (optional) counts node and triangle visits.
-Hit32 trace32(const Scene& s, const Ray32& r, uint32_t mask, bool any_hit, bool brute, uint64_t* stats = nullptr);
+// cull: ignore back-facing triangles (the G-buffer pass culls them)
+Hit32 trace32(const Scene& s, const Ray32& r, uint32_t mask, bool any_hit, bool brute, uint64_t* stats = nullptr,
+			  bool cull = false);
 
 // ---------------------------------------------------------------------------------------------
 // Atlases in the reference's image layout (2D, row-major), in double.

@@ -172,12 +172,15 @@ class OracleScene:
         lib().orc_scene_world_tris(self.h, _p(out))
         return out
 
-    def trace(self, rays: np.ndarray, mask: int = 0xFF, any_hit: bool = False, brute: bool = False):
-        """rays: (n, 8) float32 (origin, tmin, direction, tmax) -> (n, 6) uint32 hit records, stats."""
+    def trace(self, rays: np.ndarray, mask: int = 0xFF, any_hit: bool = False, brute: bool = False,
+              cull_back: bool = False):
+        """rays: (n, 8) float32 (origin, tmin, direction, tmax) -> (n, 6) uint32 hit records, stats.
+        cull_back: ignore back-facing triangles."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         out = np.zeros((len(rays), 6), np.uint32)
         stats = np.zeros(2, np.uint64)
-        lib().orc_trace(self.h, _p(rays), len(rays), mask, int(any_hit), int(brute), _p(out), _p(stats))
+        flags = (1 if any_hit else 0) | (2 if cull_back else 0)
+        lib().orc_trace(self.h, _p(rays), len(rays), mask, flags, int(brute), _p(out), _p(stats))
         return out, stats
 
     def probe_rays(self, grid: Grid, R: np.ndarray, indices: np.ndarray) -> np.ndarray:
@@ -290,4 +293,128 @@ def frame_dirs(R: np.ndarray, n: int) -> np.ndarray:
     out = np.zeros((n, 3), np.float32)
     for k in range(3):
         out[:, k] = (R[k, 0] * f[:, 0] + R[k, 1] * f[:, 1]) + R[k, 2] * f[:, 2]
+    return out
+
+
+# ---- full-frame consumer --------------------------------------------------------------------------
+GBUFFER_COLUMNS = 16  # position(3) depth normal(3) metalness albedo(3) roughness emissive(3) material
+
+
+def gbuffer(scene, rays: np.ndarray, hits: np.ndarray) -> np.ndarray:
+    """G-buffer texels (N, 16) float64 of camera rays in FP64 (material id in column 15, -1 on a miss).
+
+    Follows GBuffer.vert.glsl:35-48 and GBuffer.frag:32-66: position = o + t d of the nearest
+    front-facing hit (``hits`` from OracleScene.trace(..., cull_back=True); back faces are culled,
+    src/RenderPasses/GBufferPipeline.cpp:212), depth = |position - o|, normal = normalize(transpose(
+    inverse(M)) * sum_i b_i n_i), albedo = sum_i b_i colour_i * baseColorFactor, metalness,
+    roughness and emissive factors of the material.  A miss leaves depth 0."""
+    rays = np.asarray(rays, np.float32).reshape(-1, 8)
+    out = np.zeros((len(rays), GBUFFER_COLUMNS))
+    out[:, 15] = -1.0
+    hit = hits[:, 1] != 0xFFFFFFFF
+    idx = np.nonzero(hit)[0]
+    if len(idx) == 0:
+        return out
+    h = hits[idx]
+    t = h[:, 0].view(np.float32).astype(np.float64)
+    u = h[:, 4].view(np.float32).astype(np.float64)
+    v = h[:, 5].view(np.float32).astype(np.float64)
+    o = rays[idx, :3].astype(np.float64)
+    d = rays[idx, 4:7].astype(np.float64)
+    pos = o + t[:, None] * d
+    inst = scene.instances[h[:, 1]]
+    oe = scene.offsets[inst["custom_index"]]
+    base = oe["index_offset"].astype(np.int64) + 3 * h[:, 2].astype(np.int64)
+    tri = scene.indices[base[:, None] + np.arange(3)].astype(np.int64) + oe["vertex_offset"].astype(np.int64)[:, None]
+    b = np.stack([1 - u - v, u, v], 1)[:, :, None]
+    n_obj = (scene.vertices["normal"][tri].astype(np.float64) * b).sum(1)
+    col = (scene.vertices["color"][tri].astype(np.float64) * b).sum(1)
+    M = inst["transform"].astype(np.float64).reshape(-1, 3, 4)[:, :, :3]
+    nmat = np.transpose(np.linalg.inv(M), (0, 2, 1))
+    n = np.einsum("kij,kj->ki", nmat, n_obj)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    mat = scene.materials[oe["material_index"]]
+    out[idx, 0:3] = pos
+    out[idx, 3] = np.linalg.norm(pos - o, axis=1)
+    out[idx, 4:7] = n
+    out[idx, 7] = mat["metallic"]
+    out[idx, 8:11] = col * mat["base_color"].astype(np.float64)
+    out[idx, 11] = mat["roughness"]
+    out[idx, 12:15] = mat["emissive"]
+    out[idx, 15] = oe["material_index"]
+    return out
+
+
+def shadow_direction32(light_dir) -> np.ndarray:
+    """normalize(light direction) in FP32, operation by operation as the library computes it."""
+    L = np.asarray(light_dir, np.float32)[:3]
+    s = np.float32(np.float32(L[0] * L[0]) + np.float32(L[1] * L[1]))
+    s = np.float32(s + np.float32(L[2] * L[2]))
+    return (L / np.sqrt(s)).astype(np.float32)
+
+
+def gather(grid: Grid, states, irr, dep, oscene: OracleScene, light: np.ndarray, sky, gb: np.ndarray, origin,
+           direct=None, reflection=None) -> np.ndarray:
+    """Final gather (FinalGather.frag:38-77) in FP64 over G-buffer texels ``gb`` ((N, 16) float32
+    records as the library writes them).  Miss (depth <= 0): the constant sky.  Hit:
+    visibility * pbrMetallicRoughness(n, view, L colour, L, albedo, metal, rough) + mix(f0, albedo,
+    metal) * reflection + sampleProbes(position, n, view) * albedo (1 - f0)(1 - metal) + emissive,
+    f0 = 0.004, view = normalize(origin - position).  ``direct`` None: visibility from a hard shadow
+    ray (directLight.rgen:66-90 without its jitter): the FP32 ray from the texel position along
+    normalize(L) (tmin 0.01, tmax 10000, any hit) when dot(normalize(L), n) > 0 in FP32; 1 if it
+    escapes (times 1 - clamp(-L.y, 0, 0.1) / 0.1 below the horizon), else 0."""
+    gb = np.asarray(gb, np.float32).reshape(-1, GBUFFER_COLUMNS)
+    N = len(gb)
+    out = np.zeros((N, 3))
+    sky = np.asarray(sky, np.float64)
+    miss = gb[:, 3] <= 0
+    out[miss] = sky
+    idx = np.nonzero(~miss)[0]
+    if len(idx) == 0:
+        return out
+    g = gb[idx]
+    pos32 = g[:, 0:3]
+    n32 = g[:, 4:7]
+    # FP32 unit normal and visibility test, as the kernel computes them
+    ln = np.sqrt((n32[:, 0] * n32[:, 0] + n32[:, 1] * n32[:, 1]) + n32[:, 2] * n32[:, 2])
+    nn32 = (n32 / ln[:, None]).astype(np.float32)
+    sd = shadow_direction32(light["direction"])
+    if direct is None:
+        facing = ((sd[0] * nn32[:, 0] + sd[1] * nn32[:, 1]) + sd[2] * nn32[:, 2]) > 0
+        vis = np.zeros(len(idx))
+        f = np.nonzero(facing)[0]
+        if len(f):
+            rays = np.zeros((len(f), 8), np.float32)
+            rays[:, 0:3] = pos32[f]
+            rays[:, 3] = 0.01
+            rays[:, 4:7] = sd
+            rays[:, 7] = 10000.0
+            hits, _ = oscene.trace(rays, mask=0xFF, any_hit=True)
+            lit = hits[:, 1] == 0xFFFFFFFF
+            fade = 1.0 - min(max(-float(sd[1]), 0.0), 0.1) / 0.1 if sd[1] < 0 else 1.0
+            vis[f[lit]] = fade
+    else:
+        vis = np.asarray(direct, np.float64).reshape(-1)[idx]
+    pos = pos32.astype(np.float64)
+    n = n32.astype(np.float64)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    view = np.asarray(origin, np.float64)[None, :] - pos
+    view /= np.linalg.norm(view, axis=1, keepdims=True)
+    albedo = g[:, 8:11].astype(np.float64)
+    metal = g[:, 7].astype(np.float64)
+    rough = g[:, 11].astype(np.float64)
+    lc = np.asarray(light["color"][:3], np.float64)
+    ld = np.asarray(light["direction"][:3], np.float64)
+    color = np.zeros((len(idx), 3))
+    for k in range(len(idx)):
+        if vis[k] != 0.0:
+            color[k] = vis[k] * pbr(n[k], view[k], lc, ld, albedo[k], metal[k], rough[k])
+    f0 = 0.004
+    diffuse = albedo * (1 - f0) * (1 - metal)[:, None]
+    if reflection is not None:
+        spec = f0 * (1 - metal)[:, None] + albedo * metal[:, None]
+        color += spec * np.asarray(reflection, np.float64).reshape(-1, 3)[idx]
+    color += sample(grid, states, irr, dep, pos, n, view) * diffuse
+    color += g[:, 12:15].astype(np.float64)
+    out[idx] = color
     return out

@@ -211,3 +211,29 @@ def test_thread_count_does_not_change_results():
     orc.set_threads(8)
     b, _ = osc.trace(rays)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("which", ["soup", "room_closed"])
+def test_back_face_culling(which):
+    """Culled traversal = the minimum over front-facing triangles only: BVH and brute force agree,
+    no culled hit is a back face, and where the two-sided closest hit is front-facing it is also the
+    culled hit."""
+    scene = {"soup": lambda: _soup(4), "room_closed": lambda: scenes.room_boxes_scene(3)}[which]()
+    osc = orc.OracleScene(scene)
+    rng = np.random.Generator(np.random.PCG64(21))
+    rays = _random_rays(rng, 20000, scene.bounds_min - 1, scene.bounds_max + 1)
+    culled, _ = osc.trace(rays, cull_back=True)
+    brute, _ = osc.trace(rays, cull_back=True, brute=True)
+    assert np.array_equal(culled, brute)
+    hit = culled[:, 1] != 0xFFFFFFFF
+    assert np.all(culled[hit, 3] == 0)
+    plain, _ = osc.trace(rays)
+    front = (plain[:, 1] != 0xFFFFFFFF) & (plain[:, 3] == 0)
+    assert np.array_equal(plain[front], culled[front])
+    # where the two-sided hit is a back face, culling finds a farther surface or nothing
+    back = (plain[:, 1] != 0xFFFFFFFF) & (plain[:, 3] == 1)
+    assert back.sum() > 100
+    t_back = plain[back, 0].view(np.float32)
+    t_cull = culled[back, 0].view(np.float32)
+    gone = culled[back, 1] == 0xFFFFFFFF
+    assert np.all(t_cull[~gone] >= t_back[~gone])

@@ -27,6 +27,8 @@ NODE_BYTES, TRIANGLE_BYTES = 128, 48  # 4-wide BVH node, triangle record
 ATLAS_IRRADIANCE, ATLAS_DEPTH = 0, 1
 LAYOUT_TILED, LAYOUT_IMAGE = 0, 1
 TRACE_ANY_HIT = 0x1
+TRACE_CULL_BACK_FACES = 0x2
+GBUFFER_FIELDS = ("position", "depth", "normal", "metalness", "albedo", "roughness", "emissive", "material")
 
 
 class DdgiError(RuntimeError):
@@ -99,6 +101,8 @@ _SIGNATURES = {
     "ddgi_diff_prepare": (C.c_int, [_P, _P, _P, _P, _P, _P, _U64, _U32, _P]),
     "ddgi_diff_forward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "ddgi_diff_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "ddgi_gbuffer": (C.c_int, [_P, _P, _U64, _P, _P]),
+    "ddgi_gather": (C.c_int, [_P, _P, _U64, _P, _P, _P, _P, _P]),
     "ddgi_batch_create": (C.c_int, [_P, _U32, _P, C.POINTER(C.c_void_p)]),
     "ddgi_batch_destroy": (C.c_int, [_P]),
     "ddgi_batch_set_config": (C.c_int, [_P, _P]),

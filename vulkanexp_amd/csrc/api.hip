@@ -894,7 +894,53 @@ ddgi_status ddgi_trace_rays(void* handle, const ddgi_ray* rays, ddgi_hit* hits, 
 	ddgi_status st = set_device(h);
 	if(st != DDGI_OK) return st;
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	HIP_TRY(h, launch_trace_rays(h->sd, rays, hits, count, mask, (flags & DDGI_TRACE_ANY_HIT) != 0, s));
+	HIP_TRY(h, launch_trace_rays(h->sd, rays, hits, count, mask, (flags & DDGI_TRACE_ANY_HIT) != 0,
+								 (flags & DDGI_TRACE_CULL_BACK_FACES) != 0, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_gbuffer(void* handle, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_gbuffer");
+	if(count && (!rays || !out)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null rays or output");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, launch_gbuffer(h->sd, rays, count, out, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_gather(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
+						const float* direct, const float* reflection, float* out_rgb, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_gather");
+	if(count && (!gbuffer || !out_rgb)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null G-buffer or output");
+	if(!view_origin || !finite_all(view_origin, 3)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid view origin");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	GatherArgs a{};
+	a.scene = h->sd;
+	a.grid = make_grid_const(*h);
+	a.light = h->light;
+	a.states = h->states;
+	a.irr = h->irr[h->cur];
+	a.dep = h->dep[h->cur];
+	a.gbuffer = gbuffer;
+	a.direct = direct;
+	a.reflection = reflection;
+	a.out = out_rgb;
+	a.count = count;
+	for(int k = 0; k < 3; ++k) a.origin[k] = view_origin[k];
+	// the shadow ray direction: normalize(light direction) in FP32 (directLight.rgen:72)
+	const float* L = h->light.dir;
+	const float len = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+	for(int k = 0; k < 3; ++k) a.sdir[k] = L[k] / len;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, launch_gather(a, s));
 	h->last_stream = s;
 	return DDGI_OK;
 }

@@ -156,12 +156,14 @@ constexpr int kLdsStack = 16;  // LDS ring entries per thread
 constexpr int kSpill = 80;      // private spill entries: kLdsStack + kSpill = 3 * kMaxWideDepth
 static_assert(kLdsStack + kSpill >= 3 * int(kMaxWideDepth), "traversal stack too small for the deepest tree");
 
-template <bool ANY>
+// CULL: back-facing triangles are ignored (rasterizer back-face culling).
+template <bool ANY, bool CULL>
 __device__ __forceinline__ bool test_triangle(const RayPrep& r, float tmin, uint32_t mask, uint32_t slot, float4 ta,
 											  float4 tb, float4 tc, HitResult& best) {
 	if((__float_as_uint(tc.w) & mask) == 0u) return false;
 	float t, V, W, det;
 	if(!tri_intersect(r, ta, tb, tc, tmin, best.t, t, V, W, det)) return false;
+	if(CULL && ((det < 0.0f) != ((__float_as_uint(tc.w) & 0x100u) != 0u))) return false;
 	const uint32_t gid = __float_as_uint(ta.w);
 	if(best.slot != 0xFFFFFFFFu && !(t < best.t || gid < best.gid)) return false;
 	best.t = t;
@@ -174,7 +176,7 @@ __device__ __forceinline__ bool test_triangle(const RayPrep& r, float tmin, uint
 	return true;
 }
 
-template <bool ANY, int STACK, bool COUNT = false>
+template <bool ANY, int STACK, bool COUNT = false, bool CULL = false>
 __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
 											  const RayPrep& r, float tmin, float tmax, uint32_t mask,
 											  uint32_t* stack, int stride, uint32_t& n_nodes_out, uint32_t& n_tris_out) {
@@ -270,8 +272,8 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 				}
 				n_tris += two ? 2u : 1u;
 				// in index order, so that equal-t hits resolve exactly as a sequential scan would
-				if(test_triangle<ANY>(r, tmin, mask, first + k, ta, tb, tc, best) && ANY) goto done;
-				if(two && test_triangle<ANY>(r, tmin, mask, first + k + 1, ua, ub, uc, best) && ANY) goto done;
+				if(test_triangle<ANY, CULL>(r, tmin, mask, first + k, ta, tb, tc, best) && ANY) goto done;
+				if(two && test_triangle<ANY, CULL>(r, tmin, mask, first + k + 1, ua, ub, uc, best) && ANY) goto done;
 			}
 		}
 		DDGI_STACK_POP(node, break)

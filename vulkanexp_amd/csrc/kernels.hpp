@@ -116,6 +116,23 @@ struct BlendArgs {
 };
 constexpr uint32_t kBlendIrradianceOnly = 0x1;
 
+// Final gather over a G-buffer (FinalGather.frag:38-77).
+struct GatherArgs {
+	SceneDev scene;
+	GridConst grid;
+	LightConst light;
+	const uint32_t* states;
+	const float4* irr;
+	const float2* dep;
+	const ddgi_gbuffer_texel* gbuffer;
+	const float* direct;      // optional, one value per texel
+	const float* reflection;  // optional, rgb per texel
+	float* out;               // rgb per texel
+	uint64_t count;
+	float origin[3];          // camera position (the view vectors' origin)
+	float sdir[3];            // normalize(light direction), computed on the host
+};
+
 // ---- launchers (return hipGetLastError of the launch) ----
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
@@ -123,7 +140,9 @@ hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
-							 bool any_hit, hipStream_t s);
+							 bool any_hit, bool cull, hipStream_t s);
+hipError_t launch_gbuffer(const SceneDev& sc, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, hipStream_t s);
+hipError_t launch_gather(const GatherArgs& a, hipStream_t s);
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
 							  uint32_t* backface_counts, hipStream_t s);
 hipError_t launch_select(const uint32_t* states, uint32_t P, uint32_t cap, SelectState* st, uint32_t* indices,

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_trace(BatchLaunch b) {
 }
 
 // ---- arbitrary ray queries (public ddgi_hit records) ----
-template <int STACK, bool ANY>
+template <int STACK, bool ANY, bool CULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneDev sc, const ddgi_ray* __restrict__ rays,
 															ddgi_hit* __restrict__ hits, uint64_t count, uint32_t mask) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
@@ -263,7 +263,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneDev sc, const d
 	RayPrep r;
 	ray_prepare(q.origin[0], q.origin[1], q.origin[2], q.direction[0], q.direction[1], q.direction[2], r);
 	uint32_t nn, nt;
-	const HitResult h = traverse<ANY, STACK>(sc.nodes, sc.tris, r, q.tmin, q.tmax, mask, stack + threadIdx.x, kTraceBlock, nn, nt);
+	const HitResult h = traverse<ANY, STACK, false, CULL>(sc.nodes, sc.tris, r, q.tmin, q.tmax, mask, stack + threadIdx.x,
+														   kTraceBlock, nn, nt);
 	ddgi_hit out;
 	if(h.slot == 0xFFFFFFFFu) {
 		out.t = -1.0f;
@@ -281,6 +282,133 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneDev sc, const d
 		out.flags = (__float_as_uint(p.y) & 0x80000000u) ? 1u : 0u;
 	}
 	hits[id] = out;
+}
+
+// ---- G-buffer of camera rays (GBuffer.vert.glsl:35-48, GBuffer.frag:32-66) ----
+// The nearest front-facing hit (the G-buffer pass culls back faces,
+// src/RenderPasses/GBufferPipeline.cpp:212): world position, its distance to the ray origin,
+// normal = normalize(normal matrix * interpolated vertex normal), albedo = interpolated vertex
+// colour * base colour, metalness, roughness, emissive, material.  A miss leaves depth 0.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_gbuffer(SceneDev sc, const ddgi_ray* __restrict__ rays, uint64_t count,
+														 ddgi_gbuffer_texel* __restrict__ out) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint64_t id = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+	if(id >= count) return;
+	const ddgi_ray q = rays[id];
+	RayPrep r;
+	ray_prepare(q.origin[0], q.origin[1], q.origin[2], q.direction[0], q.direction[1], q.direction[2], r);
+	uint32_t nn, nt;
+	const HitResult h = traverse<false, STACK, false, true>(sc.nodes, sc.tris, r, q.tmin, q.tmax, 0xFFu,
+															 stack + threadIdx.x, kTraceBlock, nn, nt);
+	ddgi_gbuffer_texel g;
+	if(h.slot == 0xFFFFFFFFu) {
+		for(int k = 0; k < 3; ++k) g.position[k] = g.normal[k] = g.albedo[k] = g.emissive[k] = 0.0f;
+		g.depth = g.metalness = g.roughness = 0.0f;
+		g.material = 0xFFFFFFFFu;
+		out[id] = g;
+		return;
+	}
+	const float t = h.t;
+	const Vec3 position = v3(q.direction[0] * t + q.origin[0], q.direction[1] * t + q.origin[1],
+							 q.direction[2] * t + q.origin[2]);
+	const Vec3 rel = position - v3(q.origin[0], q.origin[1], q.origin[2]);
+	const float b1 = h.V / h.det, b2 = h.W / h.det, b0 = 1.0f - b1 - b2;
+	const float4* tr = sc.tris + size_t(h.slot) * 3;
+	const uint32_t gid = __float_as_uint(tr[0].w), inst = __float_as_uint(tr[1].w);
+	const InstanceDev I = sc.inst[inst];
+	const uint32_t* oe = sc.offsets + size_t(I.custom_index) * 3;
+	const uint32_t mat = oe[0], voff = oe[1], ib = oe[2] + 3u * (gid - I.prim_base);
+	Vec3 nsum = v3(0.0f, 0.0f, 0.0f), csum = v3(0.0f, 0.0f, 0.0f);
+	const float b[3] = {b0, b1, b2};
+	for(int k = 0; k < 3; ++k) {
+		const float4* vr = sc.vertices + size_t(voff + sc.indices[ib + k]) * 4;  // 64-byte vertex record
+		const float4 q0 = vr[0], q1 = vr[1], q2 = vr[2];
+		nsum = nsum + v3(q1.z, q1.w, q2.x) * b[k];
+		csum = csum + v3(q0.w, q1.x, q1.y) * b[k];
+	}
+	const Vec3 normal = normalize3(v3(I.nm[0] * nsum.x + I.nm[1] * nsum.y + I.nm[2] * nsum.z,
+									  I.nm[3] * nsum.x + I.nm[4] * nsum.y + I.nm[5] * nsum.z,
+									  I.nm[6] * nsum.x + I.nm[7] * nsum.y + I.nm[8] * nsum.z));
+	const float* m = sc.materials + size_t(mat) * 12;
+	g.position[0] = position.x;
+	g.position[1] = position.y;
+	g.position[2] = position.z;
+	g.depth = sqrtf(dot3(rel, rel));
+	g.normal[0] = normal.x;
+	g.normal[1] = normal.y;
+	g.normal[2] = normal.z;
+	g.metalness = m[0];
+	g.albedo[0] = csum.x * m[2];
+	g.albedo[1] = csum.y * m[3];
+	g.albedo[2] = csum.z * m[4];
+	g.roughness = m[1];
+	g.emissive[0] = m[5];
+	g.emissive[1] = m[6];
+	g.emissive[2] = m[7];
+	g.material = mat;
+	out[id] = g;
+}
+
+// ---- final gather (FinalGather.frag:38-77) over a G-buffer ----
+// Miss: the constant sky.  Hit: sun visibility * pbrMetallicRoughness + mix(f0, albedo, metal) *
+// reflection + sampleProbes(position, normal, view) * albedo (1 - f0)(1 - metal) + emissive, with
+// f0 = 0.004.  Sun visibility: `direct[i]` when given, else a hard shadow toward the sun as
+// directLight.rgen:66-90 without its jitter: when dot(sdir, n) > 0, an any-hit ray from the
+// position (tmin 0.01, tmax 10000) that escapes gives 1, faded below the horizon; otherwise 0.
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_gather(GatherArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint64_t id = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+	if(id >= a.count) return;
+	const ddgi_gbuffer_texel g = a.gbuffer[id];
+	float* o = a.out + id * 3;
+	if(g.depth <= 0.0f) {
+		o[0] = a.light.sky[0];
+		o[1] = a.light.sky[1];
+		o[2] = a.light.sky[2];
+		return;
+	}
+	const Vec3 position = v3(g.position[0], g.position[1], g.position[2]);
+	const Vec3 normal = normalize3(v3(g.normal[0], g.normal[1], g.normal[2]));
+	const Vec3 albedo = v3(g.albedo[0], g.albedo[1], g.albedo[2]);
+	const float metal = g.metalness;
+	const Vec3 view = normalize3(v3(a.origin[0], a.origin[1], a.origin[2]) - position);
+	float visible = 0.0f;
+	if(a.direct) {
+		visible = a.direct[id];
+	} else {
+		const Vec3 sd = v3(a.sdir[0], a.sdir[1], a.sdir[2]);
+		if(dot3(sd, normal) > 0.0f) {
+			RayPrep sr;
+			ray_prepare(position.x, position.y, position.z, sd.x, sd.y, sd.z, sr);
+			uint32_t nn, nt;
+			const HitResult sh = traverse<true, STACK>(a.scene.nodes, a.scene.tris, sr, 0.01f, 10000.0f, 0xFFu,
+														stack + threadIdx.x, kTraceBlock, nn, nt);
+			if(sh.slot == 0xFFFFFFFFu) {
+				visible = 1.0f;
+				if(sd.y < 0.0f) visible *= 1.0f - fminf(fmaxf(-sd.y, 0.0f), 0.1f) / 0.1f;
+			}
+		}
+	}
+	Vec3 color = pbr_metallic_roughness(normal, view, v3(a.light.color[0], a.light.color[1], a.light.color[2]),
+										v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]), albedo, metal, g.roughness) *
+				 visible;
+	const float f0 = 0.004f;
+	const Vec3 diffuse = albedo * (1.0f - f0) * (1.0f - metal);
+	if(a.reflection) {
+		const Vec3 spec = v3(f0 * (1.0f - metal) + albedo.x * metal, f0 * (1.0f - metal) + albedo.y * metal,
+							 f0 * (1.0f - metal) + albedo.z * metal);
+		const float* rf = a.reflection + id * 3;
+		color = color + spec * v3(rf[0], rf[1], rf[2]);
+	}
+	const Vec3 indirect = sample_probes(a.grid, a.states, a.irr, a.dep, position, normal, view);
+	color = color + indirect * diffuse;
+	color = color + v3(g.emissive[0], g.emissive[1], g.emissive[2]);
+	o[0] = color.x;
+	o[1] = color.y;
+	o[2] = color.z;
 }
 
 // ---- probe-state init: one block per probe, 512 rays (probesInit.rgen:31-64) ----
@@ -483,11 +611,29 @@ hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
-							 bool any_hit, hipStream_t s) {
+							 bool any_hit, bool cull, hipStream_t s) {
 	if(count == 0) return hipSuccess;
 	const dim3 grid(blocks_for(count, kTraceBlock));
-	if(any_hit) hipLaunchKernelGGL((k_trace_rays<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
-	else hipLaunchKernelGGL((k_trace_rays<kLdsStack, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+	if(any_hit) {
+		if(cull) hipLaunchKernelGGL((k_trace_rays<kLdsStack, true, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+		else hipLaunchKernelGGL((k_trace_rays<kLdsStack, true, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+	} else {
+		if(cull) hipLaunchKernelGGL((k_trace_rays<kLdsStack, false, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+		else hipLaunchKernelGGL((k_trace_rays<kLdsStack, false, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_gbuffer(const SceneDev& sc, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, hipStream_t s) {
+	if(count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_gbuffer<kLdsStack>, dim3(blocks_for(count, kTraceBlock)), dim3(kTraceBlock), 0, s, sc, rays, count,
+					   out);
+	return hipGetLastError();
+}
+
+hipError_t launch_gather(const GatherArgs& a, hipStream_t s) {
+	if(a.count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_gather<kLdsStack>, dim3(blocks_for(a.count, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
 	return hipGetLastError();
 }
 

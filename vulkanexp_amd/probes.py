@@ -112,13 +112,41 @@ class IrradianceProbes:
         check(self._lib.ddgi_blend_rays(self.handle, _tensor_ptr(rays), _tensor_ptr(probe_indices),
                                         int(probe_indices.numel()), _p(R), int(flags), _stream_ptr(stream)), self.handle)
 
-    def trace_rays(self, rays, mask: int = 0xFF, any_hit: bool = False, stream=None):
+    def gbuffer(self, rays, stream=None):
+        """G-buffer of camera rays (a (N, 8) float32 CUDA tensor of ddgi_ray records) -> (N, 16) float32
+        CUDA tensor of ddgi_gbuffer_texel records (position, depth, normal, metalness, albedo,
+        roughness, emissive, material bits)."""
+        import torch
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.shape[-1] == 8
+        rays = rays.contiguous()
+        out = torch.empty((rays.shape[0], 16), dtype=torch.float32, device=rays.device)
+        check(self._lib.ddgi_gbuffer(self.handle, _tensor_ptr(rays), int(rays.shape[0]), _tensor_ptr(out),
+                                     _stream_ptr(stream)), self.handle)
+        return out
+
+    def gather(self, gbuffer, view_origin, direct=None, reflection=None, stream=None):
+        """Final gather (FinalGather.frag:38-77) of a G-buffer -> (N, 3) float32 CUDA tensor."""
+        import torch
+        gbuffer = gbuffer.contiguous()
+        n = int(gbuffer.shape[0])
+        out = torch.empty((n, 3), dtype=torch.float32, device=gbuffer.device)
+        origin = np.ascontiguousarray(view_origin, np.float32).reshape(3)
+        d = direct.contiguous() if direct is not None else None
+        r = reflection.contiguous() if reflection is not None else None
+        check(self._lib.ddgi_gather(self.handle, _tensor_ptr(gbuffer), n, _p(origin),
+                                    _tensor_ptr(d) if d is not None else None,
+                                    _tensor_ptr(r) if r is not None else None, _tensor_ptr(out),
+                                    _stream_ptr(stream)), self.handle)
+        return out
+
+    def trace_rays(self, rays, mask: int = 0xFF, any_hit: bool = False, stream=None, cull_back: bool = False):
         """rays: CUDA float32 tensor (N, 8) = (origin, tmin, direction, tmax).  Returns a CUDA int32
         tensor (N, 6): t bits, u bits, v bits, instance, primitive, flags."""
         import torch
         hits = torch.empty((rays.shape[0], 6), dtype=torch.int32, device=rays.device)
         check(self._lib.ddgi_trace_rays(self.handle, _tensor_ptr(rays), _tensor_ptr(hits), int(rays.shape[0]), mask,
-                                        _lib.TRACE_ANY_HIT if any_hit else 0, _stream_ptr(stream)), self.handle)
+                                        (_lib.TRACE_ANY_HIT if any_hit else 0) |
+                                        (_lib.TRACE_CULL_BACK_FACES if cull_back else 0), _stream_ptr(stream)), self.handle)
         return hits
 
     def sample(self, position, normal, to_camera, stream=None):

@@ -71,10 +71,10 @@ class SceneBuilder:
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def add_mesh(self, pos: np.ndarray, nrm: np.ndarray, tris: np.ndarray, material: int) -> int:
+    def add_mesh(self, pos: np.ndarray, nrm: np.ndarray, tris: np.ndarray, material: int, color=None) -> int:
         v = np.zeros(len(pos), VERTEX_DTYPE)
         v["pos"], v["normal"] = pos, nrm
-        v["color"] = 1.0
+        v["color"] = 1.0 if color is None else color
         v["tangent"] = (1.0, 0.0, 0.0, 1.0)
         self.vertices.append(v)
         self.indices.append(np.ascontiguousarray(tris, np.uint32).reshape(-1))
