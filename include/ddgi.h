@@ -226,6 +226,17 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 						   size_t index_count, const ddgi_offset_entry* offsets, size_t offset_count,
 						   const ddgi_instance* instances, size_t instance_count, const ddgi_material* materials,
 						   size_t material_count);
+/* Moves instances [first, first + count) of the current scene to new row-major 3x4 transforms
+ * (12 floats each, host memory, copied), as Renderer::updateAccelerationStructureInstances +
+ * updateTLAS (src/Renderer.cpp:671-742: new instance transforms, then an acceleration-structure
+ * update in place).  The moved instances' triangles are re-transformed on the device exactly as
+ * ddgi_set_scene would flatten them, and the BVH boxes are refitted bottom-up (the tree's topology
+ * is kept, as a TLAS update keeps it), so every query afterwards returns the hits a scene built
+ * from the new transforms returns.  Masks, offset entries and mesh data are kept.  Asynchronous on
+ * `stream`; the transform array may be reused when the call returns.  The probe states keep their
+ * values: ddgi_init_probes re-classifies them for the moved geometry (the "Update State" action,
+ * src/UI.cpp:991-993). */
+ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, const float* transforms, void* stream);
 /* Node / triangle counts and bytes of the uploaded BVH. */
 ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes);
 

@@ -83,6 +83,7 @@ _SIGNATURES = {
     "ddgi_set_sky": (C.c_int, [_P, _P]),
     "ddgi_set_scene": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ]),
     "ddgi_get_bvh_info": (C.c_int, [_P, _P, _P, _P]),
+    "ddgi_update_instances": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "ddgi_init_probes": (C.c_int, [_P, _P, _P]),
     "ddgi_update": (C.c_int, [_P, _P, _U32, _P]),
     "ddgi_blend_rays": (C.c_int, [_P, _P, _P, _U32, _P, _U32, _P]),

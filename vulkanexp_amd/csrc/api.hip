@@ -156,21 +156,48 @@ ddgi_status set_device(Handle* h) {
 }
 
 void free_scene(Handle* h) {
+	if(h->move_pending) (void)hipEventSynchronize(h->move_done);
+	h->move_pending = false;
+	(void)hipHostFree(h->move_host);
+	h->move_host = nullptr;
+	h->move_xf = nullptr;
+	h->move_meta = nullptr;
+	h->obj_tris = nullptr;
+	h->level_nodes = nullptr;
+	h->instances.clear();
 	for(auto& b : h->scene_bufs) (void)hipFree(b.p);
 	h->scene_bufs.clear();
 	h->has_scene = false;
 	h->sd = SceneDev{};
 }
 
+// A scene buffer of `count` elements, filled from `src` when given.
 template <class T>
 ddgi_status upload(Handle* h, const T* src, size_t count, const T** dst) {
 	DeviceBuffer b;
 	b.bytes = std::max<size_t>(count * sizeof(T), 16);
 	HIP_TRY(h, hipMalloc(&b.p, b.bytes));
 	h->scene_bufs.push_back(b);
-	if(count) HIP_TRY(h, hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice));
-	*dst = static_cast<const T*>(b.p);
+	if(count && src) HIP_TRY(h, hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+	if(dst) *dst = static_cast<const T*>(b.p);
 	return DDGI_OK;
+}
+
+// Device record of an instance with row-major 3x4 transform M: the normal matrix (inverse
+// transpose of the linear part, from cofactors in double), the offset entry, the global id of its
+// first triangle and its mask.  False for a singular transform.
+bool instance_dev(const ddgi_instance& in, const float* M, uint32_t prim_base, InstanceDev& out) {
+	const double a[9] = {M[0], M[1], M[2], M[4], M[5], M[6], M[8], M[9], M[10]};
+	const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+	if(det == 0.0) return false;
+	const double cof[9] = {a[4] * a[8] - a[5] * a[7], -(a[3] * a[8] - a[5] * a[6]), a[3] * a[7] - a[4] * a[6],
+						   -(a[1] * a[8] - a[2] * a[7]), a[0] * a[8] - a[2] * a[6], -(a[0] * a[7] - a[1] * a[6]),
+						   a[1] * a[5] - a[2] * a[4], -(a[0] * a[5] - a[2] * a[3]), a[0] * a[4] - a[1] * a[3]};
+	for(int q = 0; q < 9; ++q) out.nm[q] = float(cof[q] / det);
+	out.custom_index = in.custom_index;
+	out.prim_base = prim_base;
+	out.mask = in.mask;
+	return true;
 }
 
 ddgi_status make_rotation(const float* o, Rot& R) {
@@ -221,6 +248,7 @@ void destroy_handle(Handle* h) {
 	for(void* b : bufs) (void)hipFree(b);
 	for(auto& e : h->ev)
 		if(e) (void)hipEventDestroy(e);
+	if(h->move_done) (void)hipEventDestroy(h->move_done);
 	delete h;
 }
 
@@ -482,19 +510,9 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	free_scene(h);
 	SceneDev sd{};
 	std::vector<InstanceDev> inst(instance_count);
-	for(size_t k = 0; k < instance_count; ++k) {
-		const float* M = instances[k].transform;
-		const double a[9] = {M[0], M[1], M[2], M[4], M[5], M[6], M[8], M[9], M[10]};
-		const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
-		if(det == 0.0) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "singular instance transform");
-		const double cof[9] = {a[4] * a[8] - a[5] * a[7], -(a[3] * a[8] - a[5] * a[6]), a[3] * a[7] - a[4] * a[6],
-							   -(a[1] * a[8] - a[2] * a[7]), a[0] * a[8] - a[2] * a[6], -(a[0] * a[7] - a[1] * a[6]),
-							   a[1] * a[5] - a[2] * a[4], -(a[0] * a[5] - a[2] * a[3]), a[0] * a[4] - a[1] * a[3]};
-		for(int q = 0; q < 9; ++q) inst[k].nm[q] = float(cof[q] / det);
-		inst[k].custom_index = instances[k].custom_index;
-		inst[k].prim_base = bvh.inst_prim_base[k];
-		inst[k].mask = instances[k].mask;
-	}
+	for(size_t k = 0; k < instance_count; ++k)
+		if(!instance_dev(instances[k], instances[k].transform, bvh.inst_prim_base[k], inst[k]))
+			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "singular instance transform");
 	std::vector<float> mats(material_count * 12);
 	for(size_t k = 0; k < material_count; ++k) std::memcpy(&mats[k * 12], &materials[k], 48);
 	const float4* nodes = nullptr;
@@ -510,10 +528,26 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	   (st = upload(h, reinterpret_cast<const uint32_t*>(offsets), offset_count * 3, &op)) != DDGI_OK ||
 	   (st = upload(h, indices, index_count, &ixp)) != DDGI_OK ||
 	   (st = upload(h, reinterpret_cast<const float4*>(vertices), vertex_count * 4, &vp)) != DDGI_OK ||
-	   (st = upload(h, mats.data(), mats.size(), &mp)) != DDGI_OK) {
+	   (st = upload(h, mats.data(), mats.size(), &mp)) != DDGI_OK ||
+	   (st = upload(h, bvh.obj_tris.data(), bvh.obj_tris.size(), &h->obj_tris)) != DDGI_OK ||
+	   (st = upload(h, bvh.level_nodes.data(), bvh.level_nodes.size(), &h->level_nodes)) != DDGI_OK ||
+	   (st = upload<float>(h, nullptr, instance_count * 13, nullptr)) != DDGI_OK) {
 		free_scene(h);
 		return st;
 	}
+	h->move_xf = static_cast<float*>(h->scene_bufs.back().p);
+	h->move_meta = reinterpret_cast<uint32_t*>(h->move_xf + instance_count * 12);
+	if(!h->move_done) HIP_TRY(h, hipEventCreateWithFlags(&h->move_done, hipEventDisableTiming));
+	{
+		const hipError_t e = hipHostMalloc(&h->move_host, std::max<size_t>(instance_count * (13 * 4 + sizeof(InstanceDev)), 16),
+										   hipHostMallocDefault);
+		if(e != hipSuccess) {
+			h->move_host = nullptr;
+			free_scene(h);
+			return fail(h, DDGI_ERROR_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+		}
+	}
+	h->instances.assign(instances, instances + instance_count);
 	sd.nodes = nodes;
 	sd.tris = tris;
 	sd.inst = ip;
@@ -529,11 +563,63 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	h->bvh_bytes = bvh.nodes.size() * 4 + bvh.tris.size() * 4;
 	bvh.nodes.clear();
 	bvh.tris.clear();
+	bvh.obj_tris.clear();
+	bvh.obj_tris.shrink_to_fit();
 	h->bvh_meta = std::move(bvh);
 	h->has_scene = true;
 	h->has_metallic = metallic;
 	h->scene_materials = uint32_t(material_count);
 	free_diff(h);  // prepared factors refer to the previous scene
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, const float* transforms, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_update_instances");
+	const size_t n = h->instances.size();
+	if(size_t(first) > n || size_t(count) > n - first)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "instance range outside the scene");
+	if(count == 0) return DDGI_OK;
+	if(!transforms) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null transforms");
+	for(size_t q = 0; q < size_t(count) * 12; ++q)
+		if(!std::isfinite(transforms[q])) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "non-finite instance transform");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	if(h->move_pending) HIP_TRY(h, hipEventSynchronize(h->move_done));  // staging still read by the last call
+	h->move_pending = false;
+	float* xf = static_cast<float*>(h->move_host);
+	uint32_t* meta = reinterpret_cast<uint32_t*>(xf + n * 12);
+	InstanceDev* inst = reinterpret_cast<InstanceDev*>(meta + n);
+	for(uint32_t k = 0; k < count; ++k) {
+		const ddgi_instance& in = h->instances[first + k];
+		const float* M = transforms + size_t(k) * 12;
+		if(!instance_dev(in, M, h->bvh_meta.inst_prim_base[first + k], inst[k]))
+			return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "singular instance transform");
+		const double det = double(M[0]) * (double(M[5]) * M[10] - double(M[6]) * M[9]) -
+						   double(M[1]) * (double(M[4]) * M[10] - double(M[6]) * M[8]) +
+						   double(M[2]) * (double(M[4]) * M[9] - double(M[5]) * M[8]);
+		const bool flip = (det < 0.0) != ((in.flags & DDGI_INSTANCE_FLIP_FACING) != 0);
+		meta[k] = (in.mask & 0xFFu) | (flip ? 0x100u : 0u);
+	}
+	std::memcpy(xf, transforms, size_t(count) * 48);
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, hipMemcpyAsync(h->move_xf, xf, size_t(count) * 48, hipMemcpyHostToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(h->move_meta, meta, size_t(count) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(const_cast<InstanceDev*>(h->sd.inst) + first, inst, size_t(count) * sizeof(InstanceDev),
+							  hipMemcpyHostToDevice, s));
+	float4* tris = const_cast<float4*>(h->sd.tris);
+	float4* nodes = const_cast<float4*>(h->sd.nodes);
+	HIP_TRY(h, launch_retransform(tris, h->obj_tris, h->sd.tri_count, h->move_xf, h->move_meta, first, count, s));
+	const std::vector<uint32_t>& lv = h->bvh_meta.level_offsets;
+	for(size_t d = lv.size() - 1; d >= 1; --d)
+		HIP_TRY(h, launch_refit_level(nodes, tris, h->level_nodes + lv[d - 1], lv[d] - lv[d - 1], s));
+	// the staging area and move_xf / move_meta are reused once this has run
+	HIP_TRY(h, hipEventRecord(h->move_done, s));
+	h->move_pending = true;
+	for(uint32_t k = 0; k < count; ++k) std::memcpy(h->instances[first + k].transform, transforms + size_t(k) * 12, 48);
+	free_diff(h);  // prepared factors hold hits of the old geometry
+	h->last_stream = s;
 	return DDGI_OK;
 }
 

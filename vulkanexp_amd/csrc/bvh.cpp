@@ -151,19 +151,6 @@ struct Builder {
 	}
 };
 
-inline float pad_down(float v) {
-	double d = double(v) - (std::fabs(double(v)) * 1e-6 + 1e-6);
-	float f = float(d);
-	if(double(f) > d) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
-	return f;
-}
-inline float pad_up(float v) {
-	double d = double(v) + (std::fabs(double(v)) * 1e-6 + 1e-6);
-	float f = float(d);
-	if(double(f) < d) f = std::nextafter(f, std::numeric_limits<float>::infinity());
-	return f;
-}
-
 inline uint32_t f2u(float f) {
 	uint32_t u;
 	std::memcpy(&u, &f, 4);
@@ -176,6 +163,20 @@ inline float u2f(uint32_t u) {
 }
 
 }  // namespace
+
+float pad_down(float v) {
+	double d = double(v) - (std::fabs(double(v)) * 1e-6 + 1e-6);
+	float f = float(d);
+	if(double(f) > d) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+	return f;
+}
+
+float pad_up(float v) {
+	double d = double(v) + (std::fabs(double(v)) * 1e-6 + 1e-6);
+	float f = float(d);
+	if(double(f) < d) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+	return f;
+}
 
 void transform_point(const float* M, const float* p, float out[3]) {
 	for(int r = 0; r < 3; ++r) {
@@ -198,7 +199,7 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		*err = "scene exceeds 2^28 triangles";
 		return false;
 	}
-	std::vector<float> world(total * 9);
+	std::vector<float> world(total * 9), object(total * 9);
 	std::vector<uint32_t> inst_of(total), meta(total);
 	out.inst_prim_base.resize(instance_count);
 	size_t g = 0;
@@ -238,6 +239,7 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 					return false;
 				}
 				transform_point(M, pos, &world[g * 9 + v * 3]);
+				for(int q = 0; q < 3; ++q) object[g * 9 + v * 3 + q] = pos[q];
 			}
 			inst_of[g] = uint32_t(k);
 			meta[g] = (in.mask & 0xFFu) | (flip ? 0x100u : 0u);
@@ -261,8 +263,10 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 
 	// triangles in leaf order
 	out.tris.resize(total * 12);
+	out.obj_tris.resize(total * 9);
 	for(size_t i = 0; i < total; ++i) {
 		const uint32_t t = b.order[i];
+		std::memcpy(&out.obj_tris[i * 9], &object[t * 9], 9 * sizeof(float));
 		float* d = &out.tris[i * 12];
 		for(int v = 0; v < 3; ++v) {
 			d[v * 4 + 0] = world[t * 9 + v * 3 + 0];
@@ -356,6 +360,15 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		}
 	}
 	out.node_count = uint32_t(wide.size());
+	// nodes by level, for bottom-up refits
+	out.level_offsets.assign(wide_depth + 1, 0);
+	for(const Wide& w : wide) ++out.level_offsets[w.depth];
+	for(uint32_t d = 1; d <= wide_depth; ++d) out.level_offsets[d] += out.level_offsets[d - 1];
+	out.level_nodes.resize(wide.size());
+	{
+		std::vector<uint32_t> fill(out.level_offsets.begin(), out.level_offsets.end() - 1);
+		for(size_t w = 0; w < wide.size(); ++w) out.level_nodes[fill[wide[w].depth - 1]++] = uint32_t(w);
+	}
 	// SAH cost of the emitted tree (for reporting)
 	double cost = 0.0;
 	const double root_area = std::max(b.nodes[0].box.half_area(), 1e-30);

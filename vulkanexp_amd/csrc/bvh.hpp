@@ -40,7 +40,18 @@ struct BvhHost {
 	uint32_t max_depth = 0;       // binary tree
 	uint32_t wide_depth = 0;      // levels of the 4-wide tree
 	double sah_cost = 0.0;
+	// for refits after instance moves: object-space vertices of each triangle slot (9 floats, leaf
+	// order) and the wide nodes grouped by depth (level d = nodes level_nodes[level_offsets[d-1] ..
+	// level_offsets[d]), root at level 1)
+	std::vector<float> obj_tris;
+	std::vector<uint32_t> level_nodes;
+	std::vector<uint32_t> level_offsets;
 };
+
+// Box padding of the device layout: the float just below v - (|v| 1e-6 + 1e-6) and just above
+// v + (|v| 1e-6 + 1e-6), computed in double (the traversal's slab tests stay conservative).
+float pad_down(float v);
+float pad_up(float v);
 
 // World-space FP32 vertex of instance row-major 3x4 transform M applied to object position p:
 // each row evaluated in double left to right and rounded once to float.

@@ -54,6 +54,18 @@ struct Handle {
 	bool has_scene = false;
 	BvhHost bvh_meta;
 	size_t bvh_bytes = 0;
+	// instance moves (ddgi_update_instances): the scene's instance records, the triangles'
+	// object-space vertices in BVH slot order and the wide nodes by level; the moved instances'
+	// transforms (12 floats), triangle meta words and InstanceDev records pass through a pinned
+	// staging area (`move_host`, reused once `move_done` has fired) to `move_xf` / `move_meta`
+	std::vector<ddgi_instance> instances;
+	const float* obj_tris = nullptr;
+	const uint32_t* level_nodes = nullptr;
+	float* move_xf = nullptr;
+	uint32_t* move_meta = nullptr;
+	void* move_host = nullptr;
+	hipEvent_t move_done = nullptr;
+	bool move_pending = false;
 	LightConst light{};
 	// timing
 	hipEvent_t ev[6] = {};

@@ -411,6 +411,92 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(GatherArgs a) {
 	o[2] = color.z;
 }
 
+// ---- instance moves (the TLAS instance update + refit of src/Renderer.cpp:671-742) ----
+// World vertices of the triangles of instances [first, first + count): each row of the row-major
+// 3x4 transform evaluated in double left to right and rounded once to float, exactly as the host
+// flattening (bvh.cpp transform_point), so a moved scene traces like a freshly built one.  `meta`
+// holds mask | flip << 8 per moved instance.
+__global__ void k_retransform(float4* __restrict__ tris, const float* __restrict__ obj, uint32_t tri_count,
+							  const float* __restrict__ xf, const uint32_t* __restrict__ meta, uint32_t first,
+							  uint32_t count) {
+	const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+	if(slot >= tri_count) return;
+	float4* t = tris + size_t(slot) * 3;
+	const uint32_t k = __float_as_uint(t[1].w) - first;  // wraps for instances below `first`
+	if(k >= count) return;
+	const float* M = xf + size_t(k) * 12;
+	for(int v = 0; v < 3; ++v) {
+		const float* p = obj + size_t(slot) * 9 + v * 3;
+		float w[3];
+		for(int r = 0; r < 3; ++r) {
+			double acc = double(M[r * 4 + 0]) * double(p[0]);
+			acc = acc + double(M[r * 4 + 1]) * double(p[1]);
+			acc = acc + double(M[r * 4 + 2]) * double(p[2]);
+			acc = acc + double(M[r * 4 + 3]);
+			w[r] = float(acc);
+		}
+		const float4 old = t[v];
+		t[v] = make_float4(w[0], w[1], w[2], v == 2 ? __uint_as_float(meta[k]) : old.w);
+	}
+}
+
+__device__ __forceinline__ float pad_down_dev(float v) {
+	const double d = double(v) - (fabs(double(v)) * 1e-6 + 1e-6);
+	float f = float(d);
+	if(double(f) > d) f = nextafterf(f, -__int_as_float(0x7f800000));
+	return f;
+}
+__device__ __forceinline__ float pad_up_dev(float v) {
+	const double d = double(v) + (fabs(double(v)) * 1e-6 + 1e-6);
+	float f = float(d);
+	if(double(f) < d) f = nextafterf(f, __int_as_float(0x7f800000));
+	return f;
+}
+
+// Refit of one level of the wide tree (deeper levels already refitted): every child box of the
+// level's nodes becomes the padded bounds of its triangles (leaf) or of its child node's boxes.
+__global__ void k_refit_level(float4* __restrict__ nodes, const float4* __restrict__ tris,
+							  const uint32_t* __restrict__ level_nodes, uint32_t count) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= count) return;
+	float* rec = reinterpret_cast<float*>(nodes + size_t(level_nodes[i]) * (kNodeFloats / 4));
+	for(int k = 0; k < 4; ++k) {
+		const uint32_t code = __float_as_uint(rec[24 + k]);
+		if(code == kEmptyChild) continue;
+		float lo[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+		float hi[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+		if(code & kLeafFlag) {
+			const uint32_t firstt = (code & ~kLeafFlag) >> 3, cnt = (code & 7u) + 1u;
+			for(uint32_t q = 0; q < cnt; ++q)
+				for(int v = 0; v < 3; ++v) {
+					const float4 p = tris[size_t(firstt + q) * 3 + v];
+					const float c[3] = {p.x, p.y, p.z};
+					for(int a = 0; a < 3; ++a) {
+						lo[a] = fminf(lo[a], c[a]);
+						hi[a] = fmaxf(hi[a], c[a]);
+					}
+				}
+			for(int a = 0; a < 3; ++a) {
+				lo[a] = pad_down_dev(lo[a]);
+				hi[a] = pad_up_dev(hi[a]);
+			}
+		} else {  // the child's padded boxes already bound it
+			const float* ch = reinterpret_cast<const float*>(nodes + size_t(code) * (kNodeFloats / 4));
+			for(int c = 0; c < 4; ++c) {
+				if(__float_as_uint(ch[24 + c]) == kEmptyChild) continue;
+				for(int a = 0; a < 3; ++a) {
+					lo[a] = fminf(lo[a], ch[(2 * a) * 4 + c]);
+					hi[a] = fmaxf(hi[a], ch[(2 * a + 1) * 4 + c]);
+				}
+			}
+		}
+		for(int a = 0; a < 3; ++a) {
+			rec[(2 * a) * 4 + k] = lo[a];
+			rec[(2 * a + 1) * 4 + k] = hi[a];
+		}
+	}
+}
+
 // ---- probe-state init: one block per probe, 512 rays (probesInit.rgen:31-64) ----
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_init_probes(SceneDev sc, GridConst g, const float4* __restrict__ dirs512,
@@ -621,6 +707,21 @@ hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit*
 		if(cull) hipLaunchKernelGGL((k_trace_rays<kLdsStack, false, true>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
 		else hipLaunchKernelGGL((k_trace_rays<kLdsStack, false, false>), grid, dim3(kTraceBlock), 0, s, sc, rays, hits, count, mask);
 	}
+	return hipGetLastError();
+}
+
+hipError_t launch_retransform(float4* tris, const float* obj, uint32_t tri_count, const float* xf, const uint32_t* meta,
+							  uint32_t first, uint32_t count, hipStream_t s) {
+	if(tri_count == 0 || count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_retransform, dim3(blocks_for(tri_count, 256)), dim3(256), 0, s, tris, obj, tri_count, xf, meta,
+					   first, count);
+	return hipGetLastError();
+}
+
+hipError_t launch_refit_level(float4* nodes, const float4* tris, const uint32_t* level_nodes, uint32_t count,
+							  hipStream_t s) {
+	if(count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(count, 128)), dim3(128), 0, s, nodes, tris, level_nodes, count);
 	return hipGetLastError();
 }
 

@@ -92,6 +92,13 @@ class IrradianceProbes:
         check(self._lib.ddgi_set_scene(self.handle, _p(v), len(v), _p(i), len(i), _p(o), len(o), _p(inst), len(inst),
                                        _p(m), len(m)), self.handle)
 
+    def update_instances(self, first: int, transforms: np.ndarray, stream=None) -> None:
+        """Move instances first .. first + len(transforms) - 1 to new row-major 3x4 transforms
+        ((count, 3, 4) or (count, 12) float32) and refit the BVH (src/Renderer.cpp:671-742)."""
+        xf = np.ascontiguousarray(transforms, np.float32).reshape(-1, 12)
+        check(self._lib.ddgi_update_instances(self.handle, int(first), len(xf), _p(xf), _stream_ptr(stream)),
+              self.handle)
+
     def bvh_info(self) -> dict:
         n, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(self._lib.ddgi_get_bvh_info(self.handle, C.byref(n), C.byref(t), C.byref(b)), self.handle)
