@@ -14,6 +14,7 @@ replica) through its thread pool on a bounded sample of the same frame.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -43,6 +44,8 @@ def parse():
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-frame", action="store_true")
+    ap.add_argument("--no-dynamic", action="store_true")
+    ap.add_argument("--no-sky", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 3 elementwise copies of 256 MiB (torch.add(x, 0)) so that profiler byte "
                          "counters can be scaled against a kernel of known traffic")
@@ -151,6 +154,72 @@ def frame_bench(probes, scene, width=1920, height=1080):
     px = width * height
     return {"pixels": px, "hit_fraction": round(hit, 3), "gbuffer_ms": round(g_ms, 4), "gather_ms": round(f_ms, 4),
             "Mpixels/s": round(px / ((g_ms + f_ms) * 1e-3) / 1e6, 1)}
+
+
+def dynamic_bench(probes, scene, R, lib):
+    """Dynamic scene on the large workload: every instance but the floor turns by a few degrees
+    about its own origin and shifts by 0.5 m; one ddgi_update_instances call (device re-transform
+    + bottom-up refit) against a host rebuild (ddgi_set_scene), and the probe update's time on the
+    refitted tree.  The instances are moved back afterwards."""
+    import time
+    import torch
+    n = len(scene.instances)
+    xf0 = scene.instances["transform"][1:].reshape(-1, 3, 4).astype(np.float64)
+    a = np.radians(7.0)
+    rot = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    xf1 = np.empty_like(xf0)
+    xf1[:, :, :3] = rot @ xf0[:, :, :3]
+    xf1[:, :, 3] = xf0[:, :, 3] + np.array([0.5, 0.0, 0.5])
+    xf1 = xf1.astype(np.float32).reshape(-1, 12)
+    xf0 = xf0.astype(np.float32).reshape(-1, 12)
+    probes.update(R, lib.UPDATE_ALL_PROBES | lib.UPDATE_NO_TIMING)
+    probes.synchronize()
+    before = _event_ms(lambda: probes.update(R, lib.UPDATE_ALL_PROBES | lib.UPDATE_NO_TIMING), reps=5)
+    gpu, wall = [], []
+    for k in range(6):
+        target = xf1 if k % 2 == 0 else xf0
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        probes.update_instances(1, target)
+        e1.record()
+        e1.synchronize()
+        wall.append((time.perf_counter() - t0) * 1e3)
+        gpu.append(e0.elapsed_time(e1))
+    probes.update_instances(1, xf1)
+    probes.synchronize()
+    after = _event_ms(lambda: probes.update(R, lib.UPDATE_ALL_PROBES | lib.UPDATE_NO_TIMING), reps=5)
+    moved = dataclasses.replace(scene, instances=scene.instances.copy())
+    moved.instances["transform"][1:] = xf1
+    t0 = time.perf_counter()
+    probes.set_scene(moved)
+    probes.synchronize()
+    rebuild_s = time.perf_counter() - t0
+    rebuilt = _event_ms(lambda: probes.update(R, lib.UPDATE_ALL_PROBES | lib.UPDATE_NO_TIMING), reps=5)
+    probes.set_scene(scene)
+    probes.synchronize()
+    return {"instances_moved": n - 1, "triangles_moved": int(scene.instances["triangle_count"][1:].sum()),
+            "update_instances_ms": round(float(np.median(gpu)), 4), "update_instances_wall_ms": round(float(np.median(wall)), 4),
+            "host_rebuild_s": round(rebuild_s, 3), "probe_update_ms": {"before_move": round(before, 4),
+            "refitted": round(after, 4), "rebuilt": round(rebuilt, 4)}}
+
+
+def sky_bench(probes, R, lib):
+    """The probe update of the large workload with the atmospheric sky (sky.glsl's 64-sample
+    scattering integral per missed ray) instead of the constant sky."""
+    flags = lib.UPDATE_ALL_PROBES | lib.UPDATE_NO_TIMING
+    constant = _event_ms(lambda: probes.update(R, flags), reps=5)
+    probes.set_sky_model(lib.SKY_ATMOSPHERE)
+    try:
+        probes.update(R, flags)
+        atmosphere = _event_ms(lambda: probes.update(R, flags), reps=5)
+        rays, _ = probes.last_update_rays()
+        miss = float((rays[..., 3] == -1).mean())
+    finally:
+        probes.set_sky_model(lib.SKY_CONSTANT)
+    return {"update_ms_constant_sky": round(constant, 4), "update_ms_atmosphere": round(atmosphere, 4),
+            "miss_fraction": round(miss, 3)}
 
 
 def diff_bench(lib):
@@ -342,6 +411,8 @@ def main():
     blend = None if args.no_blend_compare else blend_compare(probes, Rs[-1], _lib)
     diff = None if args.no_diff else diff_bench(_lib)
     batched = None if args.no_batch else batch_bench(_lib)
+    dynamic = None if args.no_dynamic else dynamic_bench(probes, scene, Rs[-1], _lib)
+    atmosphere = None if args.no_sky else sky_bench(probes, Rs[-1], _lib)
 
     if args.calibrate_copy:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -367,6 +438,7 @@ def main():
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
         "full_frame": frame, "blend_precision": blend, "differentiable": diff, "batched_volumes": batched,
+        "dynamic_scene": dynamic, "atmospheric_sky": atmosphere,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -157,7 +157,8 @@ typedef struct ddgi_hit {
 typedef struct ddgi_gbuffer_texel {
 	float position[3];  /* world position of the nearest front-facing hit */
 	float depth;        /* its distance from the ray origin; 0 on a miss */
-	float normal[3];    /* normalize(normal matrix * interpolated vertex normal) */
+	float normal[3];    /* normalize(normal matrix * interpolated vertex normal); on a miss the ray
+	                       direction (the gather's sky lookup) */
 	float metalness;
 	float albedo[3];    /* interpolated vertex colour * base colour */
 	float roughness;
@@ -213,11 +214,20 @@ const char* ddgi_last_error(void* handle);
  * ddgi_set_params: IrradianceProbes::updateUniforms (src/IrradianceProbes.cpp:106-108); the
  *   resolution must equal the one given at creation (the reference re-inits to resize).
  * ddgi_set_light: the light UBO of binding 10 (src/IrradianceProbes.cpp:327-336).
- * ddgi_set_sky: constant radiance returned by misses (replaces src/shaders/miss.rmiss:17-23). */
+ *   color[3] scales the sun colour in the atmospheric sky (src/shaders/miss.rmiss:22).
+ * ddgi_set_sky: constant radiance returned by misses under DDGI_SKY_CONSTANT.
+ * ddgi_set_sky_model: what misses return (src/shaders/miss.rmiss:17-23): DDGI_SKY_CONSTANT (the
+ *   default) or DDGI_SKY_ATMOSPHERE, the reference's sky() (src/shaders/sky.glsl:60-127) seen from
+ *   the ray origin with the light's direction and colour * color[3], sun disc included; the
+ *   gather's camera misses use colour * 1 (src/shaders/FinalGather.frag:52-54).  Batched updates
+ *   (ddgi_batch_update) support the constant sky only. */
+#define DDGI_SKY_CONSTANT 0u
+#define DDGI_SKY_ATMOSPHERE 1u
 ddgi_status ddgi_set_params(void* handle, const ddgi_grid_params* params);
 ddgi_status ddgi_get_params(void* handle, ddgi_grid_params* out_params);
 ddgi_status ddgi_set_light(void* handle, const ddgi_light* light);
 ddgi_status ddgi_set_sky(void* handle, const float rgb[3]);
+ddgi_status ddgi_set_sky_model(void* handle, uint32_t model);
 
 /* ---- scene (Renderer::allocateMeshes + createAccelerationStructures + createTLAS,
  * src/Renderer.cpp:57-95,272-449,525-642).  All arrays are host memory and are copied.

@@ -7,7 +7,8 @@
 //   traceProbes ................. src/shaders/traceProbes.rgen:27-49
 //   closest hit (NO_REFLECTION) . src/shaders/closesthit.glsl:133-289
 //   pbrMetallicRoughness ........ src/shaders/pbrMetallicRoughness.glsl:8-84
-//   miss / shadow miss .......... src/shaders/miss.rmiss:17-23 (constant sky), src/shaders/shadow.rmiss:6-9
+//   miss / shadow miss .......... src/shaders/miss.rmiss:17-23 (constant sky, or sky.glsl:60-127),
+//                                 src/shaders/shadow.rmiss:6-9
 //   sampleProbes ................ src/shaders/irradiance.glsl:145-237
 //   blend + state machine ....... src/shaders/probesUpdate.glsl:49-119
 //   border copy ................. src/shaders/probesCopyBorders.comp:21-246 (as the mirror formula;
@@ -514,6 +515,114 @@ V3<T> pbr_metallic_roughness(V3<T> n, V3<T> v, V3<T> light_color, V3<T> light_di
 }
 
 // =============================================================================================
+// Atmospheric sky, src/shaders/sky.glsl:60-127 (O'Neil, "Accurate Atmospheric Scattering", GPU
+// Gems 2 ch. 16), evaluated in double.  The reference's choices are kept as they are:
+//   - traceSphereInside (:33-42) adds the closest-approach offset when the closest approach lies
+//     ahead of the origin and subtracts it otherwise (the reverse of the geometric exit distance);
+//   - a sample whose attenuation is infinite or NaN is skipped and the sample point is not
+//     advanced (:106, `continue` before `samplePoint += sampleRay`);
+//   - a colour with an infinite channel gives black (:118-126).
+// "Infinite" means what it means in the shader's FP32: exp overflows above ln(FLT_MAX), and a
+// colour channel above FLT_MAX counts as infinite.  Sample k lies at position + (k + 1/2) step,
+// the point the shader reaches by repeated addition.
+namespace {
+constexpr double kSkyInner = 100000.0, kSkyOuter = 102500.0;
+constexpr double kSkyScale = 1.0 / (kSkyOuter - kSkyInner), kSkyScaleDepth = 0.25;
+constexpr double kSkyPi = double(3.1415926538f);  // common.glsl:4 (a float constant)
+constexpr double kSkyKr = double(0.0025f), kSkyKm = double(0.0010f), kSkyG = double(-0.990f);
+
+double sky_scale(double c) {  // sky.glsl:44-48
+	const double x = 1.0 - c;
+	return kSkyScaleDepth * std::exp(-0.00287 + x * (0.459 + x * (3.83 + x * (-6.80 + x * 5.25))));
+}
+double sky_enter(V3<double> o, V3<double> d, double r) {  // traceSphereOutside, centre 0 (:14-30)
+	const double a = dot(d, d), b = dot(d, o), c = dot(o, o) - r * r;
+	const double g = b * b - a * c;
+	if(g > 0.0) {
+		const double dis = (-std::sqrt(g) - b) / a;
+		if(dis > 0.0) return dis;
+	}
+	return -1.0;
+}
+double sky_leave(V3<double> o, V3<double> d, double r) {  // traceSphereInside, centre 0 (:33-42)
+	const double docdir = dot(-o, d);
+	const V3<double> pc = o + docdir * d;
+	const double lpc = length(pc);
+	const double dist = std::sqrt(r * r - lpc * lpc);
+	return docdir > 0.0 ? dist - length(pc - o) : dist + length(pc - o);
+}
+bool f32_inf(double v) { return std::isinf(v) || std::fabs(v) > double(std::numeric_limits<float>::max()); }
+}  // namespace
+
+V3<double> sky_atmosphere(V3<double> origin, V3<double> dir, V3<double> sun_dir, V3<double> sun_color, double brightness,
+						  bool show_sun) {
+	sun_color = sun_color * brightness;
+	V3<double> pos(origin.x, origin.y + (kSkyInner + 100.0), origin.z);
+	double height = length(pos);
+	const V3<double> L = normalize(sun_dir);
+	if(std::fabs(height - kSkyInner) < 1e-3) {
+		pos = pos + 1e-2 * normalize(pos);
+		height = length(pos);
+	}
+	if(height < kSkyOuter) {
+		if(height > kSkyInner) {
+			const double pd = sky_enter(pos, dir, kSkyInner);
+			if(pd >= 0.0) {
+				const double c = std::max(0.1, dot(L, normalize(pos + pd * dir))) * 0.05;
+				return {c, c, c};
+			}
+		} else {
+			return {};
+		}
+		const double ray_depth = sky_leave(pos, dir, kSkyOuter);
+		if(std::isinf(ray_depth) || std::isnan(ray_depth)) return {};
+		const double start_offset = std::exp(kSkyScale / kSkyScaleDepth * (kSkyInner - height)) *
+									sky_scale(dot(dir, pos) / height);
+		const double seg = ray_depth / 64.0;
+		const V3<double> step = dir * seg;
+		const V3<double> inv_wl(1.0 / std::pow(double(0.650f), 4.0), 1.0 / std::pow(double(0.570f), 4.0),
+								1.0 / std::pow(double(0.475f), 4.0));
+		const double k_r4 = kSkyKr * 4.0 * kSkyPi, k_m4 = kSkyKm * 4.0 * kSkyPi;
+		const double ln_max = std::log(double(std::numeric_limits<float>::max()));
+		V3<double> color;
+		int advanced = 0;
+		for(int i = 0; i < 64; ++i) {
+			const V3<double> sp = pos + (double(advanced) + 0.5) * step;
+			const double h = length(sp);
+			const double depth = std::exp(kSkyScale / kSkyScaleDepth * (kSkyInner - h));
+			const double scatter = start_offset + depth * (sky_scale(dot(L, sp) / h) - sky_scale(dot(dir, sp) / h));
+			bool bad = false;
+			V3<double> att;
+			for(int c = 0; c < 3; ++c) {
+				const double e = -scatter * (inv_wl[c] * k_r4 + k_m4);
+				if(std::isnan(e) || e > ln_max) bad = true;
+				att[c] = std::exp(e);
+			}
+			if(bad) continue;
+			color = color + att * (depth * seg * kSkyScale);
+			++advanced;
+		}
+		const V3<double> secondary = color * kSkyKm * sun_color;
+		color = color * inv_wl * kSkyKr * sun_color;
+		if(show_sun) {
+			const double mc = dot(L, -dir);
+			const double g2 = kSkyG * kSkyG;
+			const double phase = 1.5 * ((1.0 - g2) / (2.0 + g2)) * (1.0 + mc * mc) /
+								 std::pow(std::max(1e-3, 1.0 + g2 - 2.0 * kSkyG * mc), 1.5);
+			color = color + phase * secondary;
+		}
+		if(!f32_inf(color.x) && !f32_inf(color.y) && !f32_inf(color.z)) return color;
+	} else {
+		const double d = sky_enter(pos, dir, kSkyOuter);
+		if(d > 0.0) {
+			const double c = dot(L, normalize(pos + d * dir)) * 0.5;
+			return {c * 0.5294117647, c * 0.80784313725, c * 0.92156862745};
+		}
+	}
+	return {};
+}
+
+// =============================================================================================
 // Closest-hit shading of one probe ray (src/shaders/closesthit.glsl:133-289, NO_REFLECTION) plus
 // the miss shader with a constant sky.  Writes (rgb, depth).
 // Per-ray radiance factors: for a non-metallic hit material the closest-hit radiance is affine in
@@ -532,8 +641,12 @@ struct ShadeAux {
 };
 
 template <class T>
-static void shade_ray(const Scene& s, const Grid& g, const Light& L, const V3<double>& sky, const uint32_t* states,
+static void shade_ray(const Scene& s, const Grid& g, const Light& L, const SkyCfg& skycfg, const uint32_t* states,
 					  const Atlas& irr, const Atlas& dep, const Ray32& r, const Hit32& h, T out[4], ShadeAux* aux) {
+	V3<double> sky = skycfg.constant;
+	if(!h.hit() && skycfg.model == 1)
+		sky = sky_atmosphere(V3<double>(r.o[0], r.o[1], r.o[2]), V3<double>(r.d[0], r.d[1], r.d[2]), L.dir, L.color,
+							 L.sky_brightness, true);
 	if(aux) {
 		std::memset(aux, 0, sizeof(*aux));
 		aux->material = 0xFFFFFFFFu;
@@ -816,6 +929,7 @@ Light make_light(const ddgi_light* l) {
 	Light L;
 	L.dir = V3<double>(l->direction[0], l->direction[1], l->direction[2]);
 	L.color = V3<double>(l->color[0], l->color[1], l->color[2]);
+	L.sky_brightness = l->color[3];
 	return L;
 }
 
@@ -1014,11 +1128,14 @@ void orc_probe_rays(const ddgi_grid_params* p, const float* R, const uint32_t* i
 // count*n ShadeAux; max_change/oor (optional) per slot.  phase_ms[3] = trace, blend, border.
 int orc_update(void* h, const ddgi_grid_params* p, const ddgi_light* light, const double* sky, const float* R,
 			   const uint32_t* indices, uint32_t count, uint32_t* states, double* irr, double* dep, double* rays_out,
-			   uint32_t* hits_out, void* aux_out, double* max_change, uint32_t* oor, int fp32, double* phase_ms) {
+			   uint32_t* hits_out, void* aux_out, double* max_change, uint32_t* oor, int fp32, double* phase_ms,
+			   int sky_model) {
 	const Scene& s = static_cast<OracleScene*>(h)->s;
 	Grid g = make_grid(p);
 	const Light L = make_light(light);
-	const V3<double> skyc(sky[0], sky[1], sky[2]);
+	SkyCfg skyc;
+	skyc.constant = V3<double>(sky[0], sky[1], sky[2]);
+	skyc.model = sky_model;
 	const uint32_t n = p->rays_per_probe;
 	const uint64_t nr = uint64_t(count) * n;
 	std::vector<double> rays_local;
@@ -1176,6 +1293,26 @@ void orc_sample(const ddgi_grid_params* p, const uint32_t* states, const double*
 }
 
 // ---- pbrMetallicRoughness at one point (closed-form tests) ----
+// scale() of src/shaders/sky.glsl:44-48 (O'Neil's fit of the optical depth).
+double orc_sky_scale(double c) { return sky_scale(c); }
+
+// sky() (src/shaders/sky.glsl:60-127) for `count` rays: origins and dirs float3 (FP32 values, as
+// a shader sees them), sun direction / colour float3; writes count * 3 doubles.
+void orc_sky(uint64_t count, const float* origins, const float* dirs, const float* sun_dir, const float* sun_color,
+			 double brightness, int show_sun, double* out) {
+	const V3<double> L(sun_dir[0], sun_dir[1], sun_dir[2]), C(sun_color[0], sun_color[1], sun_color[2]);
+	ThreadPool::parallel_for(count, 1024, [&](uint64_t b, uint64_t e) {
+		for(uint64_t i = b; i < e; ++i) {
+			const V3<double> c =
+				sky_atmosphere(V3<double>(origins[i * 3], origins[i * 3 + 1], origins[i * 3 + 2]),
+							   V3<double>(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]), L, C, brightness, show_sun != 0);
+			out[i * 3] = c.x;
+			out[i * 3 + 1] = c.y;
+			out[i * 3 + 2] = c.z;
+		}
+	});
+}
+
 void orc_pbr(const double* n, const double* v, const double* lc, const double* ld, const double* albedo, double metal,
 			 double rough, double* out) {
 	V3<double> r = pbr_metallic_roughness<double>(V3<double>(n[0], n[1], n[2]), V3<double>(v[0], v[1], v[2]),

@@ -217,6 +217,18 @@ struct Atlas {
 struct Light {
 	V3<double> dir;    // toward the light (src/Light.hpp:7)
 	V3<double> color;  // radiance (rgb)
+	double sky_brightness = 10.0;  // color.a: the atmospheric sky's sun scale (src/shaders/miss.rmiss:22)
 };
+
+// What misses see: the constant radiance, or (model 1) the atmospheric sky of
+// src/shaders/sky.glsl:60-127 with the light's direction and colour * sky_brightness.
+struct SkyCfg {
+	V3<double> constant;
+	int model = 0;
+};
+
+// sky() of src/shaders/sky.glsl:60-127 in double precision (see ddgi_oracle.cpp).
+V3<double> sky_atmosphere(V3<double> origin, V3<double> dir, V3<double> sun_dir, V3<double> sun_color, double brightness,
+						  bool show_sun);
 
 }  // namespace orc

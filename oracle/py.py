@@ -50,7 +50,9 @@ def lib():
             "orc_scene_world_tris": (None, [P, P]),
             "orc_trace": (None, [P, P, u64, u32, C.c_int, C.c_int, P, P]),
             "orc_probe_rays": (None, [P, P, P, u32, P]),
-            "orc_update": (C.c_int, [P, P, P, P, P, P, u32, P, P, P, P, P, P, P, P, C.c_int, P]),
+            "orc_update": (C.c_int, [P, P, P, P, P, P, u32, P, P, P, P, P, P, P, P, C.c_int, P, C.c_int]),
+            "orc_sky": (None, [u64, P, P, P, P, C.c_double, C.c_int, P]),
+            "orc_sky_scale": (C.c_double, [C.c_double]),
             "orc_shade_aux_size": (u64, []),
             "orc_blend": (None, [P, P, P, P, u32, P, P, P, P, P, C.c_int]),
             "orc_sample": (None, [P, P, P, P, P, P, P, u64, P, C.c_int]),
@@ -191,9 +193,10 @@ class OracleScene:
         return out
 
     def update(self, grid: Grid, light: np.ndarray, sky, R, indices, states, irr, dep, fp32=False,
-               want_hits=False, want_aux=False):
+               want_hits=False, want_aux=False, sky_model=0):
         """One probe update frame.  ``states``, ``irr``, ``dep`` are updated in place.  Returns a dict
-        with the ray buffer (count, n, 4), optional hit records, max_change, out_of_range, phase ms."""
+        with the ray buffer (count, n, 4), optional hit records, max_change, out_of_range, phase ms.
+        ``sky_model`` 1: misses see the atmospheric sky (sky_atmosphere) instead of ``sky``."""
         indices = np.ascontiguousarray(indices, np.uint32)
         count = len(indices)
         n = int(grid.params["rays_per_probe"])
@@ -208,7 +211,7 @@ class OracleScene:
         sky = np.ascontiguousarray(sky, np.float64)
         lib().orc_update(self.h, _p(grid.params), _p(light), _p(sky), _p(np.ascontiguousarray(R, np.float32)),
                          _p(indices), count, _p(states), _p(irr), _p(dep), _p(rays), _p(hits), _p(aux),
-                         _p(mc), _p(oor), int(fp32), _p(ms))
+                         _p(mc), _p(oor), int(fp32), _p(ms), int(sky_model))
         res = {"rays": rays, "max_change": mc, "out_of_range": oor,
                "trace_ms": ms[0], "blend_ms": ms[1], "border_ms": ms[2]}
         if want_hits:
@@ -307,10 +310,12 @@ def gbuffer(scene, rays: np.ndarray, hits: np.ndarray) -> np.ndarray:
     front-facing hit (``hits`` from OracleScene.trace(..., cull_back=True); back faces are culled,
     src/RenderPasses/GBufferPipeline.cpp:212), depth = |position - o|, normal = normalize(transpose(
     inverse(M)) * sum_i b_i n_i), albedo = sum_i b_i colour_i * baseColorFactor, metalness,
-    roughness and emissive factors of the material.  A miss leaves depth 0."""
+    roughness and emissive factors of the material.  A miss leaves depth 0 and carries the ray
+    direction in the normal columns (for the gather's sky)."""
     rays = np.asarray(rays, np.float32).reshape(-1, 8)
     out = np.zeros((len(rays), GBUFFER_COLUMNS))
     out[:, 15] = -1.0
+    out[:, 4:7] = rays[:, 4:7]
     hit = hits[:, 1] != 0xFFFFFFFF
     idx = np.nonzero(hit)[0]
     if len(idx) == 0:
@@ -345,6 +350,29 @@ def gbuffer(scene, rays: np.ndarray, hits: np.ndarray) -> np.ndarray:
     return out
 
 
+def sky_atmosphere(origins, dirs, light: np.ndarray, brightness=None, show_sun=True) -> np.ndarray:
+    """sky() of src/shaders/sky.glsl:60-127 in FP64 for rays (origins, dirs: (N, 3), taken as
+    float32 values) under ``light`` (LIGHT_DTYPE record); the sun colour is scaled by
+    ``brightness`` (default: the light colour's 4th component, as miss.rmiss:22 passes it)."""
+    o = np.ascontiguousarray(np.asarray(origins, np.float32).reshape(-1, 3))
+    d = np.ascontiguousarray(np.asarray(dirs, np.float32).reshape(-1, 3))
+    if len(o) == 1 and len(d) > 1:
+        o = np.ascontiguousarray(np.repeat(o, len(d), 0))
+    assert o.shape == d.shape
+    sun = np.ascontiguousarray(light["direction"][:3], np.float32)
+    col = np.ascontiguousarray(light["color"][:3], np.float32)
+    b = float(light["color"][3]) if brightness is None else float(brightness)
+    out = np.zeros((len(d), 3))
+    lib().orc_sky(len(d), _p(o), _p(d), _p(sun), _p(col), b, int(bool(show_sun)), _p(out))
+    return out
+
+
+def sky_scale(c: float) -> float:
+    """scale() of sky.glsl:44-48: O'Neil's polynomial fit of the optical depth along a ray leaving
+    the ground at cos(zenith angle) c, in units of the atmosphere thickness."""
+    return float(lib().orc_sky_scale(float(c)))
+
+
 def shadow_direction32(light_dir) -> np.ndarray:
     """normalize(light direction) in FP32, operation by operation as the library computes it."""
     L = np.asarray(light_dir, np.float32)[:3]
@@ -354,9 +382,11 @@ def shadow_direction32(light_dir) -> np.ndarray:
 
 
 def gather(grid: Grid, states, irr, dep, oscene: OracleScene, light: np.ndarray, sky, gb: np.ndarray, origin,
-           direct=None, reflection=None) -> np.ndarray:
+           direct=None, reflection=None, sky_model=0) -> np.ndarray:
     """Final gather (FinalGather.frag:38-77) in FP64 over G-buffer texels ``gb`` ((N, 16) float32
-    records as the library writes them).  Miss (depth <= 0): the constant sky.  Hit:
+    records as the library writes them).  Miss (depth <= 0): the constant sky, or with
+    ``sky_model`` 1 the atmospheric sky from ``origin`` along the ray direction the texel carries in
+    its normal, sun colour scale 1 (FinalGather.frag:52-54).  Hit:
     visibility * pbrMetallicRoughness(n, view, L colour, L, albedo, metal, rough) + mix(f0, albedo,
     metal) * reflection + sampleProbes(position, n, view) * albedo (1 - f0)(1 - metal) + emissive,
     f0 = 0.004, view = normalize(origin - position).  ``direct`` None: visibility from a hard shadow
@@ -369,6 +399,8 @@ def gather(grid: Grid, states, irr, dep, oscene: OracleScene, light: np.ndarray,
     sky = np.asarray(sky, np.float64)
     miss = gb[:, 3] <= 0
     out[miss] = sky
+    if sky_model == 1 and miss.any():
+        out[miss] = sky_atmosphere(np.asarray(origin, np.float32)[None, :], gb[miss, 4:7], light, brightness=1.0)
     idx = np.nonzero(~miss)[0]
     if len(idx) == 0:
         return out

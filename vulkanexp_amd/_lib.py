@@ -28,6 +28,8 @@ ATLAS_IRRADIANCE, ATLAS_DEPTH = 0, 1
 LAYOUT_TILED, LAYOUT_IMAGE = 0, 1
 TRACE_ANY_HIT = 0x1
 TRACE_CULL_BACK_FACES = 0x2
+SKY_CONSTANT = 0
+SKY_ATMOSPHERE = 1
 GBUFFER_FIELDS = ("position", "depth", "normal", "metalness", "albedo", "roughness", "emissive", "material")
 
 
@@ -81,6 +83,7 @@ _SIGNATURES = {
     "ddgi_get_params": (C.c_int, [_P, _P]),
     "ddgi_set_light": (C.c_int, [_P, _P]),
     "ddgi_set_sky": (C.c_int, [_P, _P]),
+    "ddgi_set_sky_model": (C.c_int, [_P, C.c_uint32]),
     "ddgi_set_scene": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ]),
     "ddgi_get_bvh_info": (C.c_int, [_P, _P, _P, _P]),
     "ddgi_update_instances": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),

@@ -243,7 +243,7 @@ void destroy_handle(Handle* h) {
 		(void)hipFree(h->irr[i]);
 		(void)hipFree(h->dep[i]);
 	}
-	void* bufs[] = {h->states, h->indices, h->sel,    h->rays, h->hits,  h->shade_list, h->shade_count, h->fib,
+	void* bufs[] = {h->states, h->indices, h->sel,    h->rays, h->hits,  h->shade_list, h->shade_count, h->miss_list, h->fib,
 					h->perm,   h->fib512,  h->dirs,   h->wt,   h->inv_sum, h->w_hi,      h->w_lo,        h->visit};
 	for(void* b : bufs) (void)hipFree(b);
 	for(auto& e : h->ev)
@@ -422,6 +422,8 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	for(int k = 0; k < 3; ++k) {
 		h->light.color[k] = 10.0f;
 		h->light.sky[k] = 0.5f;
+		h->light.sky_brightness = 10.0f;  // LightBuffer colour (10, 10, 10, 10)
+		h->light.sky_model = DDGI_SKY_CONSTANT;
 	}
 	if((st = ensure_fib(h)) != DDGI_OK) return cleanup_fail(st);
 	*out_handle = h;
@@ -462,12 +464,27 @@ ddgi_status ddgi_get_params(void* handle, ddgi_grid_params* out) {
 ddgi_status ddgi_set_light(void* handle, const ddgi_light* light) {
 	if(!handle || !light) return fail(H(handle), DDGI_ERROR_INVALID_ARGUMENT, "null argument");
 	Handle* h = H(handle);
-	if(!finite_all(light->direction, 3) || !finite_all(light->color, 3))
+	if(!finite_all(light->direction, 3) || !finite_all(light->color, 4))
 		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "non-finite light");
 	for(int k = 0; k < 3; ++k) {
 		h->light.dir[k] = light->direction[k];
 		h->light.color[k] = light->color[k];
 	}
+	h->light.sky_brightness = light->color[3];
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_set_sky_model(void* handle, uint32_t model) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(model != DDGI_SKY_CONSTANT && model != DDGI_SKY_ATMOSPHERE)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "unknown sky model");
+	if(model == DDGI_SKY_ATMOSPHERE && !h->miss_list) {
+		ddgi_status st = set_device(h);
+		if(st != DDGI_OK) return st;
+		HIP_TRY(h, dalloc(&h->miss_list, size_t(h->P) * h->max_rays * sizeof(uint32_t)));
+	}
+	h->light.sky_model = model;
 	return DDGI_OK;
 }
 
@@ -710,6 +727,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	a.rays = h->rays;
 	a.shade_list = h->shade_list;
 	a.shade_count = h->shade_count;
+	a.miss_list = h->miss_list;
 	a.max_slots = max_slots;
 	a.visit_counters = nullptr;
 	a.factors = factors;
@@ -720,6 +738,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	HIP_TRY(h, launch_trace_primary(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
 	HIP_TRY(h, launch_shade(a, s));
+	if(h->light.sky_model == DDGI_SKY_ATMOSPHERE) HIP_TRY(h, launch_sky(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
 	if((st = enqueue_blend_and_publish(h, R, h->rays, h->perm, max_slots, all, flags, s)) != DDGI_OK) return st;
 	if(timing) {

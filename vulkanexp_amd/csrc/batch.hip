@@ -190,6 +190,8 @@ void fill_volume(const Handle& h, const Rot& R, BatchVolume& v) {
 	a.scene = h.sd;
 	a.grid = g;
 	a.light = h.light;
+	a.light.sky_model = DDGI_SKY_CONSTANT;  // batches run no sky pass (check_update_args refuses the atmosphere)
+	a.miss_list = nullptr;
 	a.indices = h.indices;
 	a.count = &h.sel->count;
 	a.dirs = h.dirs;
@@ -299,6 +301,9 @@ ddgi_status check_update_args(Batch* b, const float* orientations, uint32_t flag
 	if((flags & DDGI_UPDATE_BLEND_BF16) && (flags & DDGI_UPDATE_BLEND_FP16))
 		return bfail(DDGI_ERROR_INVALID_ARGUMENT, "BF16 and FP16 blends are exclusive");
 	if(!orientations) return bfail(DDGI_ERROR_INVALID_ARGUMENT, "null orientations");
+	for(const Handle* h : b->vols)
+		if(h->light.sky_model != DDGI_SKY_CONSTANT)
+			return bfail(DDGI_ERROR_UNSUPPORTED, "batched updates support the constant sky only");
 	for(size_t i = 0; i < b->vols.size() * 9; ++i)
 		if(!std::isfinite(orientations[i])) return bfail(DDGI_ERROR_INVALID_ARGUMENT, "non-finite orientation");
 	return DDGI_OK;

@@ -368,6 +368,116 @@ __device__ __forceinline__ Vec3 normalize3(Vec3 a) {
 	return {a.x / l, a.y / l, a.z / l};
 }
 
+// ---- atmospheric sky (src/shaders/sky.glsl:60-127, after O'Neil, GPU Gems 2 ch. 16) ----
+// Planet of radius 100 km whose surface lies at y = -100, atmosphere 2.5 km thick, scale height a
+// quarter of it; Rayleigh (Kr 0.0025, 1/lambda^4 for lambda = 650/570/475 nm) and Mie (Km 0.001,
+// g = -0.99) single scattering integrated with 64 midpoint samples.  The reference's own choices
+// are kept: the exit distance of a ray leaving the atmosphere adds the closest-approach offset
+// where it should subtract it (and the reverse), a sample whose attenuation overflows is skipped
+// without advancing the sample point, and a colour with an infinite channel gives black.
+// Sample points are formed as position + (k + 1/2) * step instead of by repeated addition (the
+// same point, without the FP32 drift of 64 additions at 1e5 m).  Exponentials use the hardware
+// exp2 (__expf: relative error about 6e-8 |x| for the arguments here, |x| < 90).
+namespace sky {
+constexpr float kInner = 100000.0f;
+constexpr float kOuter = 102500.0f;
+constexpr float kScale = 1.0f / (kOuter - kInner);
+constexpr float kScaleDepth = 0.25f;
+constexpr float kPi = 3.1415926538f;  // common.glsl:4
+constexpr float kKr = 0.0025f, kKm = 0.0010f, kG = -0.990f;
+constexpr int kSamples = 64;
+
+__device__ __forceinline__ float scale_fn(float c) {
+	const float x = 1.0f - c;
+	return kScaleDepth * __expf(-0.00287f + x * (0.459f + x * (3.83f + x * (-6.80f + x * 5.25f))));
+}
+// First crossing ahead of the sphere (radius r, centred at the origin) by a ray from outside, -1 if none.
+__device__ __forceinline__ float enter_sphere(Vec3 o, Vec3 d, float r) {
+	const float a = dot3(d, d), b = dot3(d, o), c = dot3(o, o) - r * r;
+	const float g = b * b - a * c;
+	if(g > 0.0f) {
+		const float dis = (-sqrtf(g) - b) / a;
+		if(dis > 0.0f) return dis;
+	}
+	return -1.0f;
+}
+// Exit distance from the sphere of a ray inside it, as sky.glsl:33-42 computes it.
+__device__ __forceinline__ float leave_sphere(Vec3 o, Vec3 d, float r) {
+	const float along = -dot3(o, d);
+	const Vec3 pc = o + d * along;
+	const float lpc = sqrtf(dot3(pc, pc));
+	const float dist = sqrtf(r * r - lpc * lpc);
+	const Vec3 off = pc - o;
+	const float loff = sqrtf(dot3(off, off));
+	return along > 0.0f ? dist - loff : dist + loff;
+}
+}  // namespace sky
+
+__device__ __forceinline__ Vec3 sky_atmosphere(Vec3 origin, Vec3 dir, Vec3 sun_dir, Vec3 sun_color, float brightness,
+											   bool show_sun) {
+	using namespace sky;
+	sun_color = sun_color * brightness;
+	Vec3 pos = v3(origin.x, origin.y + (kInner + 100.0f), origin.z);
+	float height = sqrtf(dot3(pos, pos));
+	const Vec3 L = normalize3(sun_dir);
+	if(fabsf(height - kInner) < 1e-3f) {
+		pos = pos + normalize3(pos) * 1e-2f;
+		height = sqrtf(dot3(pos, pos));
+	}
+	const Vec3 zero = v3(0.0f, 0.0f, 0.0f);
+	if(height < kOuter) {
+		if(height > kInner) {
+			const float pd = enter_sphere(pos, dir, kInner);
+			if(pd >= 0.0f) {
+				const float c = fmaxf(0.1f, dot3(L, normalize3(pos + dir * pd)));
+				return v3(c * 0.05f, c * 0.05f, c * 0.05f);
+			}
+		} else {
+			return zero;
+		}
+		const float ray_depth = leave_sphere(pos, dir, kOuter);
+		if(isinf(ray_depth) || isnan(ray_depth)) return zero;
+		const float start_depth = __expf(kScale / kScaleDepth * (kInner - height));
+		const float start_offset = start_depth * scale_fn(dot3(dir, pos) / height);
+		const float seg = ray_depth / float(kSamples);
+		const float scaled = seg * kScale;
+		const Vec3 step = dir * seg;
+		// 1 / lambda^4 per channel, times 4 pi Kr, plus 4 pi Km
+		const Vec3 inv_wl = v3(1.0f / (0.650f * 0.650f * 0.650f * 0.650f), 1.0f / (0.570f * 0.570f * 0.570f * 0.570f),
+							   1.0f / (0.475f * 0.475f * 0.475f * 0.475f));
+		const Vec3 ext = v3(inv_wl.x * (kKr * 4.0f * kPi) + kKm * 4.0f * kPi, inv_wl.y * (kKr * 4.0f * kPi) + kKm * 4.0f * kPi,
+							inv_wl.z * (kKr * 4.0f * kPi) + kKm * 4.0f * kPi);
+		Vec3 color = zero;
+		int k = 0;  // samples advanced so far
+		for(int i = 0; i < kSamples; ++i) {
+			const Vec3 sp = pos + step * (float(k) + 0.5f);
+			const float h = sqrtf(dot3(sp, sp));
+			const float depth = __expf(kScale / kScaleDepth * (kInner - h));
+			const float scatter = start_offset + depth * (scale_fn(dot3(L, sp) / h) - scale_fn(dot3(dir, sp) / h));
+			const Vec3 att = v3(__expf(-scatter * ext.x), __expf(-scatter * ext.y), __expf(-scatter * ext.z));
+			if(!isfinite(att.x) || !isfinite(att.y) || !isfinite(att.z)) continue;
+			color = color + att * (depth * scaled);
+			++k;
+		}
+		const Vec3 secondary = color * sun_color * kKm;
+		color = color * inv_wl * sun_color * kKr;
+		if(show_sun) {
+			const float mc = -dot3(L, dir);
+			const float m = fmaxf(1e-3f, 1.0f + kG * kG - 2.0f * kG * mc);
+			const float phase = 1.5f * ((1.0f - kG * kG) / (2.0f + kG * kG)) * (1.0f + mc * mc) / (m * sqrtf(m));
+			color = color + secondary * phase;
+		}
+		if(!isinf(color.x) && !isinf(color.y) && !isinf(color.z)) return color;
+	} else {
+		const float d = enter_sphere(pos, dir, kOuter);
+		if(d > 0.0f) {
+			const float c = dot3(L, normalize3(pos + dir * d)) * 0.5f;
+			return v3(c * 0.5294117647f, c * 0.80784313725f, c * 0.92156862745f);
+		}
+	}
+	return zero;
+}
+
 // Weight (visibility x backface x trilinear) and fallback weight of cage probe i of a
 // sampleProbes call (irradiance.glsl:164-206); false when the probe is off-grid or disabled.
 __device__ __forceinline__ bool cage_probe(const GridConst& g, const uint32_t* __restrict__ states,

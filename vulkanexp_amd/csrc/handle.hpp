@@ -37,7 +37,8 @@ struct Handle {
 	float4* rays = nullptr;
 	float4* hits = nullptr;
 	uint32_t* shade_list = nullptr;   // P * max_rays entries
-	uint32_t* shade_count = nullptr;
+	uint32_t* shade_count = nullptr;  // [0] shade list, [1] miss list
+	uint32_t* miss_list = nullptr;    // P * max_rays entries, allocated with the atmospheric sky
 	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
 	uint16_t* perm = nullptr;   // kMaxRays
 	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512

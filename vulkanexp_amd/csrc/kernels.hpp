@@ -39,6 +39,8 @@ struct LightConst {
 	float dir[3];    // toward the light
 	float color[3];  // radiance
 	float sky[3];    // constant miss radiance
+	float sky_brightness;  // the light colour's 4th component: sun scale of the atmospheric sky
+	uint32_t sky_model;    // DDGI_SKY_CONSTANT or DDGI_SKY_ATMOSPHERE
 };
 
 // Per-instance shading record: normal matrix transpose(inverse(M)) row-major, custom index
@@ -80,7 +82,8 @@ struct TraceFrameArgs {
 	float4* hits;
 	float4* rays;              // (rgb, depth)
 	uint32_t* shade_list;      // ray positions (slot * n + k) of the front-face hits, filled by the primary pass
-	uint32_t* shade_count;     // its length (cleared before the primary pass)
+	uint32_t* shade_count;     // [0]: its length, [1]: the miss list's (both cleared before the primary pass)
+	uint32_t* miss_list;       // atmospheric sky only: ray positions of the misses
 	uint32_t max_slots;        // grid sized for this many slots
 	// optional (statistics runs only): [0]/[1] primary node / triangle fetches, [2]/[3] shadow
 	// node / triangle fetches, [4] shadow rays
@@ -139,6 +142,8 @@ hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint
 							 uint32_t* zero = nullptr);
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s);
+// Atmospheric-sky radiance of the frame's misses (the miss list).
+hipError_t launch_sky(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
 							 bool any_hit, bool cull, hipStream_t s);
 // Instance moves: re-transform the triangles of instances [first, first + count) (xf: 12 floats,

@@ -36,12 +36,12 @@ __device__ __forceinline__ float4 frame_dir(const Rot& R, float4 f) {
 	return make_float4(d[0], d[1], d[2], 0.0f);
 }
 
-// `zero` (optional): a counter the next kernels start from (the shade-list length), cleared here so
-// that no separate memset is needed.
+// `zero` (optional): two counters the next kernels start from (the shade- and miss-list lengths),
+// cleared here so that no separate memset is needed.
 __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count,
 							 uint32_t* zero) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if(zero && i == 0) *zero = 0u;
+	if(zero && i == 0) zero[0] = zero[1] = 0u;
 	if(i >= count) return;
 	dirs[i] = frame_dir(R, fib[i]);
 }
@@ -69,7 +69,14 @@ __device__ __forceinline__ void finish_primary_ray(const TraceFrameArgs& a, uint
 	const bool front = code != 0xFFFFFFFFu && !(code & 0x80000000u);
 	const uint32_t pos = wave_append(a.shade_count, front);  // front-face hits are queued for shading
 	if(front) a.shade_list[pos] = ri;
-	else write_unshaded(a, ri, code == 0xFFFFFFFFu, h.t);
+	const bool miss = code == 0xFFFFFFFFu;
+	if(a.light.sky_model == DDGI_SKY_ATMOSPHERE) {  // misses are queued for the sky kernel
+		const uint32_t mpos = wave_append(a.shade_count + 1, miss);
+		if(miss) a.miss_list[mpos] = ri;
+		else if(!front) write_unshaded(a, ri, false, h.t);
+	} else if(!front) {
+		write_unshaded(a, ri, miss, h.t);
+	}
 }
 
 // Traversal thread `gtid` = slot * n + j traces ray perm[j] of the probe in `slot`; `stack` is the
@@ -211,6 +218,31 @@ __global__ __launch_bounds__(kTraceBlock, 5) void k_shade(TraceFrameArgs a) {
 		shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kTraceBlock);
 }
 
+// ---- misses under the atmospheric sky (miss.rmiss:17-23 with sky.glsl:60-127) ----
+// Entry k of the miss list: the sky seen from the probe along the ray, with the light's colour
+// scaled by its 4th component and the sun's Mie lobe; depth -1.
+__global__ __launch_bounds__(256) void k_sky(TraceFrameArgs a) {
+	const uint32_t count = a.shade_count[1];
+	const uint32_t n = a.grid.n;
+	const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
+	const Vec3 C = v3(a.light.color[0], a.light.color[1], a.light.color[2]);
+	for(uint32_t k = blockIdx.x * 256 + threadIdx.x; k < count; k += gridDim.x * 256) {
+		const uint32_t ri = a.miss_list[k];
+		const uint32_t slot = ri / n, j = ri - slot * n;
+		float o[3];
+		probe_origin(a.grid, a.indices[slot], o);
+		const float4 d = a.dirs[a.perm[j]];
+		const Vec3 c = sky_atmosphere(v3(o[0], o[1], o[2]), v3(d.x, d.y, d.z), L, C, a.light.sky_brightness, true);
+		const float4 rec = make_float4(c.x, c.y, c.z, -1.0f);
+		a.rays[ri] = rec;
+		if(a.factors) {
+			float4* f = a.factors + size_t(ri) * 4;
+			f[0] = make_float4(rec.x, rec.y, rec.z, __uint_as_float(0xFFFFFFFFu));
+			f[1] = f[2] = f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
+	}
+}
+
 // ---- batched update of many volumes ----
 // Start of a batched update: block v writes volume v's frame directions; block 0 also resets the
 // work-queue heads and completion stamps and records the start time.
@@ -218,7 +250,7 @@ __global__ __launch_bounds__(512) void k_batch_begin(BatchLaunch b) {
 	const BatchVolume& v = b.vols[blockIdx.x];
 	const uint32_t n = v.trace.grid.n;
 	for(uint32_t i = threadIdx.x; i < n; i += 512) v.dirs[i] = frame_dir(v.R, v.fib[i]);
-	if(threadIdx.x == 0) *v.trace.shade_count = 0u;
+	if(threadIdx.x == 0) v.trace.shade_count[0] = v.trace.shade_count[1] = 0u;
 	if(blockIdx.x == 0) {
 		if(threadIdx.x < 4) b.ctl->next[threadIdx.x] = 0u;
 		for(uint32_t k = threadIdx.x; k < kMaxBatchVolumes; k += 512) b.ctl->done[k] = 0ull;
@@ -302,8 +334,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_gbuffer(SceneDev sc, const ddgi
 	const HitResult h = traverse<false, STACK, false, true>(sc.nodes, sc.tris, r, q.tmin, q.tmax, 0xFFu,
 															 stack + threadIdx.x, kTraceBlock, nn, nt);
 	ddgi_gbuffer_texel g;
-	if(h.slot == 0xFFFFFFFFu) {
-		for(int k = 0; k < 3; ++k) g.position[k] = g.normal[k] = g.albedo[k] = g.emissive[k] = 0.0f;
+	if(h.slot == 0xFFFFFFFFu) {  // depth 0; the normal slot carries the ray direction (for the sky)
+		for(int k = 0; k < 3; ++k) {
+			g.position[k] = g.albedo[k] = g.emissive[k] = 0.0f;
+			g.normal[k] = q.direction[k];
+		}
 		g.depth = g.metalness = g.roughness = 0.0f;
 		g.material = 0xFFFFFFFFu;
 		out[id] = g;
@@ -351,7 +386,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_gbuffer(SceneDev sc, const ddgi
 }
 
 // ---- final gather (FinalGather.frag:38-77) over a G-buffer ----
-// Miss: the constant sky.  Hit: sun visibility * pbrMetallicRoughness + mix(f0, albedo, metal) *
+// Miss: the sky (the constant, or the atmospheric model along the pixel ray that the G-buffer texel
+// carries in its normal).  Hit: sun visibility * pbrMetallicRoughness + mix(f0, albedo, metal) *
 // reflection + sampleProbes(position, normal, view) * albedo (1 - f0)(1 - metal) + emissive, with
 // f0 = 0.004.  Sun visibility: `direct[i]` when given, else a hard shadow toward the sun as
 // directLight.rgen:66-90 without its jitter: when dot(sdir, n) > 0, an any-hit ray from the
@@ -364,10 +400,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(GatherArgs a) {
 	if(id >= a.count) return;
 	const ddgi_gbuffer_texel g = a.gbuffer[id];
 	float* o = a.out + id * 3;
-	if(g.depth <= 0.0f) {
-		o[0] = a.light.sky[0];
-		o[1] = a.light.sky[1];
-		o[2] = a.light.sky[2];
+	if(g.depth <= 0.0f) {  // FinalGather.frag:52-54: the sky (sun scale 1) along the pixel's ray
+		Vec3 c = v3(a.light.sky[0], a.light.sky[1], a.light.sky[2]);
+		if(a.light.sky_model == DDGI_SKY_ATMOSPHERE)
+			c = sky_atmosphere(v3(a.origin[0], a.origin[1], a.origin[2]), v3(g.normal[0], g.normal[1], g.normal[2]),
+							   v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]),
+							   v3(a.light.color[0], a.light.color[1], a.light.color[2]), 1.0f, true);
+		o[0] = c.x;
+		o[1] = c.y;
+		o[2] = c.z;
 		return;
 	}
 	const Vec3 position = v3(g.position[0], g.position[1], g.position[2]);
@@ -677,6 +718,13 @@ hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 	const dim3 grid(blocks_for(threads, kTraceBlock));
 	if(a.visit_counters) hipLaunchKernelGGL((k_trace_primary<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, a);
 	else hipLaunchKernelGGL((k_trace_primary<kLdsStack, false>), grid, dim3(kTraceBlock), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_sky(const TraceFrameArgs& a, hipStream_t s) {
+	// grid-stride: the miss count is known only on the device; enough blocks for every CU
+	const uint32_t cap = a.max_slots * a.grid.n;
+	hipLaunchKernelGGL(k_sky, dim3(std::max(1u, std::min(blocks_for(cap, 256), 2048u))), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 

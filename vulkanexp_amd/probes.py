@@ -85,6 +85,10 @@ class IrradianceProbes:
         rgb = np.ascontiguousarray(rgb, np.float32)
         check(self._lib.ddgi_set_sky(self.handle, _p(rgb)), self.handle)
 
+    def set_sky_model(self, model: int) -> None:
+        """_lib.SKY_CONSTANT (misses return set_sky's radiance) or _lib.SKY_ATMOSPHERE (sky.glsl)."""
+        check(self._lib.ddgi_set_sky_model(self.handle, int(model)), self.handle)
+
     def set_scene(self, scene: Scene) -> None:
         arrs = [np.ascontiguousarray(a) for a in (scene.vertices, scene.indices, scene.offsets, scene.instances,
                                                    scene.materials)]
