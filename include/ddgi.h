@@ -66,7 +66,7 @@ typedef struct ddgi_vertex {
 } ddgi_vertex;
 
 /* 48-byte material record (src/vulkan/Material.hpp:16-25, src/shaders/unpackMaterial.glsl:7-19).
- * Textures are not sampled by this library; texture ids must be 0xFFFFFFFF. */
+ * Textures are not sampled by this library: texture ids are carried but ignored. */
 typedef struct ddgi_material {
 	float metallic;
 	float roughness;
@@ -249,6 +249,47 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, const float* transforms, void* stream);
 /* Node / triangle counts and bytes of the uploaded BVH. */
 ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes);
+
+/* ---- scene ingestion (no counterpart in IrradianceProbes: the reference's Scene::load / save,
+ * src/Scene.cpp:30-49 (.scene, .gltf, .glb), :115-533 (glTF 2.0), :647-684 (.mat), :710-934
+ * (.scene), flattened as Renderer::updateMeshOffsetTable + createTLAS build their tables,
+ * src/Renderer.cpp:97-126, :512-551).  Host-only; a scene object owns the tables it returns. ----
+ * The flattening: one offset entry per mesh (= glTF primitive) with indices, meshes packed back to
+ *   back; one instance per node that renders a mesh, ordered by (material, mesh) and then node
+ *   order, with the node's global transform, mask DDGI_MASK_STATIC and no facing flip; a default
+ *   material (metallic 1, roughness 1, white) is appended when a mesh refers to a missing one.
+ *   Skinned primitives are loaded but not instanced.  Textures are kept as records only.
+ * ddgi_scene_load: .scene, .gltf (external or base64 buffers) or .glb.
+ * ddgi_scene_load_material: adds the materials (and texture records) of a .mat file.
+ * ddgi_scene_create: a scene from flattened tables (static instances only), e.g. to save one.
+ * ddgi_scene_save: writes the .scene container (header "SCEN", a JSON chunk, then per mesh a
+ *   vertex chunk and an index chunk, README.md:75-91). */
+typedef struct ddgi_scene_tables {
+	const ddgi_vertex* vertices;
+	uint64_t vertex_count;
+	const uint32_t* indices;
+	uint64_t index_count;
+	const ddgi_offset_entry* offsets;
+	uint64_t offset_count;
+	const ddgi_instance* instances;
+	uint64_t instance_count;
+	const ddgi_material* materials;
+	uint64_t material_count;
+	float bounds_min[3]; /* Scene::computeBounds (src/Scene.cpp:1077-1094) */
+	float bounds_max[3];
+	uint64_t mesh_count;
+	uint64_t node_count;
+	uint64_t texture_count;
+} ddgi_scene_tables;
+ddgi_status ddgi_scene_load(const char* path, void** out_scene);
+ddgi_status ddgi_scene_load_material(void* scene, const char* path);
+ddgi_status ddgi_scene_create(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices,
+							  size_t index_count, const ddgi_offset_entry* offsets, size_t offset_count,
+							  const ddgi_instance* instances, size_t instance_count, const ddgi_material* materials,
+							  size_t material_count, void** out_scene);
+ddgi_status ddgi_scene_get_tables(void* scene, ddgi_scene_tables* out);
+ddgi_status ddgi_scene_save(void* scene, const char* path);
+ddgi_status ddgi_scene_destroy(void* scene);
 
 /* ---- per-frame work ----
  * ddgi_init_probes: IrradianceProbes::initProbes (src/IrradianceProbes.cpp:357-394) with the

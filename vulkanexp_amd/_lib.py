@@ -71,6 +71,14 @@ class BatchTuneRecord(C.Structure):
     _fields_ = [("config", BatchConfig), ("ms", C.c_float), ("phase", C.c_uint32)]
 
 
+class SceneTables(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint64), ("indices", C.c_void_p),
+                ("index_count", C.c_uint64), ("offsets", C.c_void_p), ("offset_count", C.c_uint64),
+                ("instances", C.c_void_p), ("instance_count", C.c_uint64), ("materials", C.c_void_p),
+                ("material_count", C.c_uint64), ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3),
+                ("mesh_count", C.c_uint64), ("node_count", C.c_uint64), ("texture_count", C.c_uint64)]
+
+
 _P = C.c_void_p
 _U32, _U64, _SZ = C.c_uint32, C.c_uint64, C.c_size_t
 _SIGNATURES = {
@@ -87,6 +95,12 @@ _SIGNATURES = {
     "ddgi_set_scene": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ]),
     "ddgi_get_bvh_info": (C.c_int, [_P, _P, _P, _P]),
     "ddgi_update_instances": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "ddgi_scene_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "ddgi_scene_load_material": (C.c_int, [_P, C.c_char_p]),
+    "ddgi_scene_create": (C.c_int, [_P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, C.POINTER(C.c_void_p)]),
+    "ddgi_scene_get_tables": (C.c_int, [_P, C.POINTER(SceneTables)]),
+    "ddgi_scene_save": (C.c_int, [_P, C.c_char_p]),
+    "ddgi_scene_destroy": (C.c_int, [_P]),
     "ddgi_init_probes": (C.c_int, [_P, _P, _P]),
     "ddgi_update": (C.c_int, [_P, _P, _U32, _P]),
     "ddgi_blend_rays": (C.c_int, [_P, _P, _P, _U32, _P, _U32, _P]),

@@ -35,6 +35,8 @@ ddgi_status fail(Handle* h, ddgi_status s, const std::string& msg) {
 	return s;
 }
 
+void record_error(const std::string& msg) { g_last_error = msg; }
+
 bool finite_all(const float* v, int n) {
 	for(int i = 0; i < n; ++i)
 		if(!std::isfinite(v[i])) return false;

@@ -342,3 +342,57 @@ def batched_config(seed=5, count=24):
         lo, hi = inset_extent(scenes[si])
         vols.append((si, grid_params(lo, hi, res, n)))
     return scenes, vols
+
+
+# ---- scene files (.scene / .gltf / .glb / .mat) through libddgi's loaders ------------------------
+def _scene_from_handle(lib, h, name: str) -> Scene:
+    import ctypes as C
+    from . import _lib
+    t = _lib.SceneTables()
+    _lib.check(lib.ddgi_scene_get_tables(h, C.byref(t)))
+
+    def arr(ptr, count, dtype):
+        if count == 0:
+            return np.zeros(0, dtype)
+        buf = (C.c_char * (int(count) * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(buf, dtype).copy()
+    return Scene(arr(t.vertices, t.vertex_count, VERTEX_DTYPE), arr(t.indices, t.index_count, np.uint32),
+                 arr(t.offsets, t.offset_count, OFFSET_DTYPE), arr(t.instances, t.instance_count, INSTANCE_DTYPE),
+                 arr(t.materials, t.material_count, MATERIAL_DTYPE), np.array(t.bounds_min[:], np.float32),
+                 np.array(t.bounds_max[:], np.float32), name)
+
+
+def load(path: str, materials=()) -> Scene:
+    """Load a .scene, .gltf or .glb file (Scene::load, src/Scene.cpp:30-49), add the materials of the
+    given .mat files, and flatten it into the tables ddgi_set_scene takes."""
+    import ctypes as C
+    import os
+    from . import _lib
+    lib = _lib.load()
+    h = C.c_void_p()
+    _lib.check(lib.ddgi_scene_load(os.fsencode(path), C.byref(h)))
+    try:
+        for m in materials:
+            _lib.check(lib.ddgi_scene_load_material(h, os.fsencode(m)))
+        return _scene_from_handle(lib, h, os.path.basename(path))
+    finally:
+        lib.ddgi_scene_destroy(h)
+
+
+def save(scene: Scene, path: str) -> None:
+    """Write ``scene`` (static instances) as a .scene container (Scene::save, src/Scene.cpp:710-816)."""
+    import ctypes as C
+    import os
+    from . import _lib
+    lib = _lib.load()
+    arrs = [np.ascontiguousarray(a) for a in (scene.vertices, scene.indices, scene.offsets, scene.instances,
+                                              scene.materials)]
+    args = []
+    for a in arrs:
+        args += [a.ctypes.data_as(C.c_void_p), len(a)]
+    h = C.c_void_p()
+    _lib.check(lib.ddgi_scene_create(*args, C.byref(h)))
+    try:
+        _lib.check(lib.ddgi_scene_save(h, os.fsencode(path)))
+    finally:
+        lib.ddgi_scene_destroy(h)
