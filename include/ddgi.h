@@ -326,6 +326,28 @@ ddgi_status ddgi_gather(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_
 						const float* direct, const float* reflection, float* out_rgb, void* stream);
 ddgi_status ddgi_sample(void* handle, const float* position, const float* normal, const float* to_camera,
 						uint64_t count, float* rgb_out, void* stream);
+/* Ray-traced shadow pass (src/shaders/directLight.rgen:42-99, DirectLightPipeline.cpp:5-247) over
+ * device G-buffer texels into device float4 records (the reference's rgba32f directLightOut):
+ * (-1,-1,-1,-1) on a miss texel; otherwise the sun direction normalize(light.direction), jittered
+ * inside a 0.02 rad cone by the texel's noise sample (device float2 per texel in [0,1)^2, the
+ * blue-noise texel the shader reads; NULL = no jitter), gives (0,0,0,0) when it faces away from the
+ * normal, else an any-hit ray (mask 0xFF, tmin 0.01, tmax 10000) gives (visibility, previous.g,
+ * previous.b, 1) with visibility 1 on escape (faded below the horizon) or 0.  previous: device
+ * float4 per texel (previousDirectLight), NULL = zeros.  The record's first channel is what
+ * ddgi_gather's `direct` takes (after the reference's filter passes, which are not part of this
+ * library). */
+ddgi_status ddgi_direct_light(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float* noise,
+							  const float* previous, float* out_rgba, void* stream);
+/* Reflection pass (src/shaders/reflection.rgen:101-175) over device G-buffer texels into device
+ * float4 records (rgb, roughness): texels with depth > 0 and (roughness < 0.4 or metalness > 0.01)
+ * trace one ray (closest hit, mask 0xFF, tmin 0.1, tmax 10000) from the texel along
+ * reflect(-normalize(view_origin - position), normal), rotated within the roughness lobe by the
+ * texel's noise sample (device float2 per texel, NULL = the mirror direction); the hit is shaded as
+ * closesthit.glsl at recursion depth 1 (specular from sampleProbes along the reflected ray), a back
+ * face is black, a miss sees the sky model; rgb = colorCompression(radiance).  Other texels: 0.
+ * The rgb part is what ddgi_gather's `reflection` takes. */
+ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
+							const float* noise, float* out_rgba, void* stream);
 
 /* ---- differentiable probes (no counterpart in the reference, which is not differentiable) ----
  * Gradients of a shaded-G-buffer loss with respect to per-material albedo and the sun radiance,

@@ -1293,6 +1293,42 @@ void orc_sample(const ddgi_grid_params* p, const uint32_t* states, const double*
 }
 
 // ---- pbrMetallicRoughness at one point (closed-form tests) ----
+// ---- camera-effect passes: ray directions in FP32, operation by operation as the kernels form
+// them (so that the traced rays are the same bits), shading in double ----
+// sin / cos of an FP32 angle: Cody-Waite reduction by the nearest multiple of pi/2, Taylor series
+// of sin (to r^11) and cos (to r^12) on |r| <= pi/4, quadrant sign/swap.
+static void sincos_series32(float x, float& s, float& c) {
+	const float q = std::nearbyint(x * 0.636619772f);
+	const float r = (x - q * 1.57079637f) - q * -4.37113883e-08f;
+	const float r2 = r * r;
+	const float sr = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f +
+																									r2 * (-1.0f / 39916800.0f)))));
+	const float cr = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f +
+															r2 * (-1.0f / 3628800.0f + r2 * (1.0f / 479001600.0f))))));
+	switch(int(q) & 3) {
+		case 0: s = sr; c = cr; break;
+		case 1: s = cr; c = -sr; break;
+		case 2: s = -sr; c = -cr; break;
+		default: s = -cr; c = sr; break;
+	}
+}
+static V3<float> cross32(V3<float> a, V3<float> b) {
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+static float dot32(V3<float> a, V3<float> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static V3<float> normalize32(V3<float> a) {
+	const float l = std::sqrt(dot32(a, a));
+	return {a.x / l, a.y / l, a.z / l};
+}
+// rotateAxis (src/shaders/common.glsl:6-8)
+static V3<float> rotate_axis32(V3<float> p, V3<float> axis, float angle) {
+	float sn, cs;
+	sincos_series32(angle, sn, cs);
+	const V3<float> along = axis * dot32(axis, p);
+	const V3<float> m = along * (1.0f - cs) + p * cs;
+	return m + cross32(axis, p) * sn;
+}
+
 // scale() of src/shaders/sky.glsl:44-48 (O'Neil's fit of the optical depth).
 double orc_sky_scale(double c) { return sky_scale(c); }
 
@@ -1309,6 +1345,126 @@ void orc_sky(uint64_t count, const float* origins, const float* dirs, const floa
 			out[i * 3] = c.x;
 			out[i * 3 + 1] = c.y;
 			out[i * 3 + 2] = c.z;
+		}
+	});
+}
+
+double orc_sin_series(float x) {
+	float sn, cs;
+	sincos_series32(x, sn, cs);
+	return sn;
+}
+double orc_cos_series(float x) {
+	float sn, cs;
+	sincos_series32(x, sn, cs);
+	return cs;
+}
+
+// directLight.rgen:42-99 for `count` G-buffer texels (16 floats each, ddgi_gbuffer_texel); sdir =
+// normalize(light direction) in FP32; noise count * 2 or null; previous count * 4 or null; writes
+// count * 4 floats.  All arithmetic FP32 as the shader forms it: the result is exact.
+void orc_direct_light(void* h, const float* sdir, const float* gbuffer, uint64_t count, const float* noise,
+					  const float* previous, float* out) {
+	const Scene& s = static_cast<OracleScene*>(h)->s;
+	const float kPi = 3.1415926538f;
+	ThreadPool::parallel_for(count, 256, [&](uint64_t b, uint64_t e) {
+		for(uint64_t i = b; i < e; ++i) {
+			const float* g = gbuffer + i * 16;
+			float* o = out + i * 4;
+			if(!(g[3] > 0.0f)) {
+				o[0] = o[1] = o[2] = o[3] = -1.0f;
+				continue;
+			}
+			const V3<float> normal(g[4], g[5], g[6]);
+			V3<float> dir(sdir[0], sdir[1], sdir[2]);
+			if(noise) {
+				const float nx = noise[i * 2], ny = noise[i * 2 + 1];
+				const V3<float> temp = rotate_axis32(dir, normalize32(cross32(normal, dir)), 2.0f * (nx - 0.5f) * 0.02f);
+				dir = rotate_axis32(temp, dir, (2.0f * kPi) * ny);
+			}
+			o[0] = o[1] = o[2] = o[3] = 0.0f;
+			if(dot32(dir, normal) > 0.0f) {
+				Ray32 r;
+				for(int k = 0; k < 3; ++k) {
+					r.o[k] = g[k];
+					r.d[k] = dir[k];
+				}
+				r.tmin = 0.01f;
+				r.tmax = 10000.0f;
+				ray32_prepare(r);
+				const Hit32 hh = trace32(s, r, 0xFFu, true, false);
+				float vis = 0.0f;
+				if(!hh.hit()) {
+					vis = 1.0f;
+					if(dir.y < 0.0f) vis *= 1.0f - std::min(std::max(-dir.y, 0.0f), 0.1f) / 0.1f;
+				}
+				o[0] = vis;
+				o[1] = previous ? previous[i * 4 + 1] : 0.0f;
+				o[2] = previous ? previous[i * 4 + 2] : 0.0f;
+				o[3] = 1.0f;
+			}
+		}
+	});
+}
+
+// reflection.rgen:101-175 for `count` G-buffer texels: the reflected ray in FP32 as the shader
+// forms it, traced with the FP32 replica, shaded by closesthit.glsl (recursion depth 1) in double;
+// writes count * 4 doubles (colorCompression(rgb), roughness) and, when hits_out is given, the
+// ray's hit record (6 uint32, as orc_trace) for traced texels (0xFFFFFFFF instance otherwise).
+void orc_reflection(void* h, const ddgi_grid_params* p, const ddgi_light* light, const double* sky, int sky_model,
+					const uint32_t* states, const double* irr, const double* dep, const float* gbuffer, uint64_t count,
+					const float* origin, const float* noise, double* out, uint32_t* hits_out) {
+	const Scene& s = static_cast<OracleScene*>(h)->s;
+	const Grid g = make_grid(p);
+	const Light L = make_light(light);
+	SkyCfg skyc;
+	skyc.constant = V3<double>(sky[0], sky[1], sky[2]);
+	skyc.model = sky_model;
+	const Atlas A_irr{g.image_width(p->color_res), g.image_height(p->color_res), 3, irr};
+	const Atlas A_dep{g.image_width(p->depth_res), g.image_height(p->depth_res), 2, dep};
+	const float kPi = 3.1415926538f;
+	ThreadPool::parallel_for(count, 64, [&](uint64_t b, uint64_t e) {
+		for(uint64_t i = b; i < e; ++i) {
+			const float* t = gbuffer + i * 16;
+			double* o = out + i * 4;
+			o[0] = o[1] = o[2] = o[3] = 0.0;
+			if(hits_out) {
+				uint32_t* hr = hits_out + i * 6;
+				const float neg = -1.0f;
+				std::memcpy(&hr[0], &neg, 4);
+				hr[1] = hr[2] = 0xFFFFFFFFu;
+				hr[3] = hr[4] = hr[5] = 0u;
+			}
+			const float rough = t[11], metal = t[7];
+			if(!(t[3] > 0.0f && (rough < 0.4f || metal > 0.01f))) continue;
+			const V3<float> position(t[0], t[1], t[2]), normal(t[4], t[5], t[6]);
+			const V3<float> to_origin = normalize32(V3<float>(origin[0], origin[1], origin[2]) - position);
+			const V3<float> incident(-to_origin.x, -to_origin.y, -to_origin.z);
+			const V3<float> rdir = normalize32(incident - normal * (2.0f * dot32(normal, incident)));
+			V3<float> dir = rdir;
+			if(noise) {
+				const float nx = noise[i * 2], ny = noise[i * 2 + 1];
+				const float theta = rough * (nx - 0.5f) * 2.0f * kPi;
+				const float phi = (ny - 0.5f) * 2.0f * kPi;
+				const V3<float> tangent = dot32(rdir, normal) < 0.9f ? normalize32(cross32(rdir, normal))
+																	 : normalize32(cross32(rdir, V3<float>(1.0f, 0.0f, 0.0f)));
+				dir = rotate_axis32(rdir, tangent, theta);
+				dir = rotate_axis32(dir, rdir, phi);
+			}
+			Ray32 r;
+			for(int k = 0; k < 3; ++k) {
+				r.o[k] = position[k];
+				r.d[k] = dir[k];
+			}
+			r.tmin = 0.1f;
+			r.tmax = 10000.0f;
+			ray32_prepare(r);
+			const Hit32 hh = trace32(s, r, 0xFFu, false, false);
+			if(hits_out) fill_hit_record(s, hh, &hits_out[i * 6]);
+			double c[4];
+			shade_ray<double>(s, g, L, skyc, states, A_irr, A_dep, r, hh, c, nullptr);
+			for(int k = 0; k < 3; ++k) o[k] = c[k] * (1.0 + c[k] / 1.0) / (1.0 + c[k]);  // reinhard_whitepoint(v, 1)
+			o[3] = rough;
 		}
 	});
 }

@@ -53,6 +53,10 @@ def lib():
             "orc_update": (C.c_int, [P, P, P, P, P, P, u32, P, P, P, P, P, P, P, P, C.c_int, P, C.c_int]),
             "orc_sky": (None, [u64, P, P, P, P, C.c_double, C.c_int, P]),
             "orc_sky_scale": (C.c_double, [C.c_double]),
+            "orc_sin_series": (C.c_double, [C.c_float]),
+            "orc_cos_series": (C.c_double, [C.c_float]),
+            "orc_direct_light": (None, [P, P, P, u64, P, P, P]),
+            "orc_reflection": (None, [P, P, P, P, C.c_int, P, P, P, P, u64, P, P, P, P]),
             "orc_shade_aux_size": (u64, []),
             "orc_blend": (None, [P, P, P, P, u32, P, P, P, P, P, C.c_int]),
             "orc_sample": (None, [P, P, P, P, P, P, P, u64, P, C.c_int]),
@@ -371,6 +375,43 @@ def sky_scale(c: float) -> float:
     """scale() of sky.glsl:44-48: O'Neil's polynomial fit of the optical depth along a ray leaving
     the ground at cos(zenith angle) c, in units of the atmosphere thickness."""
     return float(lib().orc_sky_scale(float(c)))
+
+
+def sincos_series(x: float):
+    """(sin, cos) of an FP32 angle as the camera-effect kernels compute them (Taylor series after a
+    Cody-Waite reduction, all FP32)."""
+    return float(lib().orc_sin_series(float(x))), float(lib().orc_cos_series(float(x)))
+
+
+def direct_light(oscene: "OracleScene", light: np.ndarray, gb: np.ndarray, noise=None, previous=None) -> np.ndarray:
+    """directLight.rgen:42-99 over G-buffer texels ``gb`` ((N, 16) float32): (N, 4) float32, exact
+    (the FP32 ray directions and the FP32 intersection replica are the kernels' operations)."""
+    gb = np.ascontiguousarray(np.asarray(gb, np.float32).reshape(-1, GBUFFER_COLUMNS))
+    sd = np.ascontiguousarray(shadow_direction32(light["direction"]))
+    nz = None if noise is None else np.ascontiguousarray(np.asarray(noise, np.float32).reshape(-1, 2))
+    pv = None if previous is None else np.ascontiguousarray(np.asarray(previous, np.float32).reshape(-1, 4))
+    out = np.zeros((len(gb), 4), np.float32)
+    lib().orc_direct_light(oscene.h, _p(sd), _p(gb), len(gb), _p(nz), _p(pv), _p(out))
+    return out
+
+
+def reflection(grid: Grid, states, irr, dep, oscene: "OracleScene", light: np.ndarray, sky, gb: np.ndarray, origin,
+               noise=None, sky_model=0, want_hits=False):
+    """reflection.rgen:101-175 over G-buffer texels ``gb``: (N, 4) float64 (colorCompression(rgb),
+    roughness); with ``want_hits`` also the reflected rays' hit records (N, 6) uint32.  Ray
+    directions and intersections FP32 as the kernel forms them, shading in FP64."""
+    gb = np.ascontiguousarray(np.asarray(gb, np.float32).reshape(-1, GBUFFER_COLUMNS))
+    nz = None if noise is None else np.ascontiguousarray(np.asarray(noise, np.float32).reshape(-1, 2))
+    o = np.ascontiguousarray(origin, np.float32)
+    sky = np.ascontiguousarray(sky, np.float64)
+    states = np.ascontiguousarray(states, np.uint32)
+    irr = np.ascontiguousarray(irr, np.float64)
+    dep = np.ascontiguousarray(dep, np.float64)
+    out = np.zeros((len(gb), 4))
+    hits = np.zeros((len(gb), 6), np.uint32) if want_hits else None
+    lib().orc_reflection(oscene.h, _p(grid.params), _p(light), _p(sky), int(sky_model), _p(states), _p(irr), _p(dep),
+                         _p(gb), len(gb), _p(o), _p(nz), _p(out), _p(hits))
+    return (out, hits) if want_hits else out
 
 
 def shadow_direction32(light_dir) -> np.ndarray:

@@ -1052,6 +1052,57 @@ ddgi_status ddgi_gather(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_
 	return DDGI_OK;
 }
 
+ddgi_status ddgi_direct_light(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float* noise,
+							  const float* previous, float* out_rgba, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_direct_light");
+	if(count && (!gbuffer || !out_rgba)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null G-buffer or output");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	DirectLightArgs a{};
+	a.scene = h->sd;
+	a.gbuffer = gbuffer;
+	a.noise = reinterpret_cast<const float2*>(noise);
+	a.previous = reinterpret_cast<const float4*>(previous);
+	a.out = reinterpret_cast<float4*>(out_rgba);
+	a.count = count;
+	const float* L = h->light.dir;  // normalize(light direction) in FP32 (directLight.rgen:71)
+	const float len = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+	for(int k = 0; k < 3; ++k) a.sdir[k] = L[k] / len;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, launch_direct_light(a, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
+							const float* noise, float* out_rgba, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_reflection");
+	if(count && (!gbuffer || !out_rgba)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null G-buffer or output");
+	if(!view_origin || !finite_all(view_origin, 3)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid view origin");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	ReflectionArgs a{};
+	a.scene = h->sd;
+	a.grid = make_grid_const(*h);
+	a.light = h->light;
+	a.states = h->states;
+	a.irr = h->irr[h->cur];
+	a.dep = h->dep[h->cur];
+	a.gbuffer = gbuffer;
+	a.noise = reinterpret_cast<const float2*>(noise);
+	a.out = reinterpret_cast<float4*>(out_rgba);
+	a.count = count;
+	for(int k = 0; k < 3; ++k) a.origin[k] = view_origin[k];
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	HIP_TRY(h, launch_reflection(a, s));
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
 ddgi_status ddgi_sample(void* handle, const float* position, const float* normal, const float* to_camera, uint64_t count,
 						float* rgb_out, void* stream) {
 	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");

@@ -368,6 +368,38 @@ __device__ __forceinline__ Vec3 normalize3(Vec3 a) {
 	return {a.x / l, a.y / l, a.z / l};
 }
 
+__device__ __forceinline__ Vec3 cross3(Vec3 a, Vec3 b) {
+	return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// sin and cos of an FP32 angle by a fixed sequence of FP32 operations, so that a host replica gives
+// the same bits: Cody-Waite reduction by the nearest multiple of pi/2 (pi/2 split into a float
+// and its remainder), then the Taylor series of sin (to r^11) and cos (to r^12) on |r| <= pi/4,
+// whose truncation error is below 1e-10, and the quadrant's sign/swap.
+__device__ __forceinline__ void sincos_series(float x, float& s, float& c) {
+	const float q = rintf(x * 0.636619772f);
+	const float r = (x - q * 1.57079637f) - q * -4.37113883e-08f;
+	const float r2 = r * r;
+	const float sr = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f +
+																									r2 * (-1.0f / 39916800.0f)))));
+	const float cr = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f +
+															r2 * (-1.0f / 3628800.0f + r2 * (1.0f / 479001600.0f))))));
+	switch(int(q) & 3) {
+		case 0: s = sr; c = cr; break;
+		case 1: s = cr; c = -sr; break;
+		case 2: s = -sr; c = -cr; break;
+		default: s = -cr; c = sr; break;
+	}
+}
+
+// rotateAxis (common.glsl:6-8): mix(dot(axis, p) axis, p, cos a) + cross(axis, p) sin a.
+__device__ __forceinline__ Vec3 rotate_axis(Vec3 p, Vec3 axis, float angle) {
+	float s, c;
+	sincos_series(angle, s, c);
+	const Vec3 along = axis * dot3(axis, p);
+	return (along * (1.0f - c) + p * c) + cross3(axis, p) * s;
+}
+
 // ---- atmospheric sky (src/shaders/sky.glsl:60-127, after O'Neil, GPU Gems 2 ch. 16) ----
 // Planet of radius 100 km whose surface lies at y = -100, atmosphere 2.5 km thick, scale height a
 // quarter of it; Rayleigh (Kr 0.0025, 1/lambda^4 for lambda = 650/570/475 nm) and Mie (Km 0.001,

@@ -136,6 +136,32 @@ struct GatherArgs {
 	float sdir[3];            // normalize(light direction), computed on the host
 };
 
+// Ray-traced shadow pass (directLight.rgen) over G-buffer texels.
+struct DirectLightArgs {
+	SceneDev scene;
+	const ddgi_gbuffer_texel* gbuffer;
+	const float2* noise;     // optional: the blue-noise sample per texel
+	const float4* previous;  // optional: the previous frame's output (its g, b pass through)
+	float4* out;
+	uint64_t count;
+	float sdir[3];           // normalize(light direction), computed on the host
+};
+
+// Reflection pass (reflection.rgen) over G-buffer texels.
+struct ReflectionArgs {
+	SceneDev scene;
+	GridConst grid;
+	LightConst light;
+	const uint32_t* states;
+	const float4* irr;
+	const float2* dep;
+	const ddgi_gbuffer_texel* gbuffer;
+	const float2* noise;  // optional: the blue-noise sample per texel
+	float4* out;
+	uint64_t count;
+	float origin[3];      // camera position
+};
+
 // ---- launchers (return hipGetLastError of the launch) ----
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
@@ -154,6 +180,8 @@ hipError_t launch_refit_level(float4* nodes, const float4* tris, const uint32_t*
 							  hipStream_t s);
 hipError_t launch_gbuffer(const SceneDev& sc, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, hipStream_t s);
+hipError_t launch_direct_light(const DirectLightArgs& a, hipStream_t s);
+hipError_t launch_reflection(const ReflectionArgs& a, hipStream_t s);
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
 							  uint32_t* backface_counts, hipStream_t s);
 hipError_t launch_select(const uint32_t* states, uint32_t P, uint32_t cap, SelectState* st, uint32_t* indices,

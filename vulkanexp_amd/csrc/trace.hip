@@ -109,7 +109,97 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a)
 									stack + threadIdx.x, kTraceBlock);
 }
 
-// ---- shading of the front-face hits + shadow ray (closesthit.glsl:143-289) ----
+// ---- closest-hit shading + shadow ray (closesthit.glsl:143-289) ----
+// Shading of the front-face hit (t, BVH slot, barycentrics b1 b2) of the ray o + t d at recursion
+// depth >= 1, where the specular term reads the probes along the reflected direction: emissive +
+// specular * sampleProbes(reflect) + diffuse * sampleProbes(normal) + the sun through an any-hit
+// shadow ray (mask 0xFF, tmin 0.1, tmax 10000), faded when the sun is below the horizon.  The
+// parts are returned too (the differentiable update decomposes them).
+struct SurfaceShade {
+	Vec3 color;
+	Vec3 em_spec;   // emissive + specular * reflection
+	Vec3 indirect;  // sampleProbes along the normal
+	Vec3 normal, to_cam;
+	float metal, rough, fade;
+	uint32_t mat;
+	bool lit;
+};
+
+template <int STACK, bool COUNT>
+__device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const GridConst& grid, const LightConst& light,
+													  const uint32_t* states, const float4* irr, const float2* dep,
+													  const float o[3], Vec3 d, float t, uint32_t tslot, float b1,
+													  float b2, uint32_t* stack, int stride,
+													  unsigned long long* visit_counters) {
+	SurfaceShade r;
+	const float b0 = 1.0f - b1 - b2;
+	const Vec3 position = v3(d.x * t + o[0], d.y * t + o[1], d.z * t + o[2]);
+	// fetch geometry and material records (closesthit.glsl:147-159)
+	const float4* tr = sc.tris + size_t(tslot) * 3;
+	const uint32_t gid = __float_as_uint(tr[0].w), inst = __float_as_uint(tr[1].w);
+	const InstanceDev I = sc.inst[inst];
+	const uint32_t prim = gid - I.prim_base;
+	const uint32_t* oe = sc.offsets + size_t(I.custom_index) * 3;
+	const uint32_t mat = oe[0], voff = oe[1], ib = oe[2] + 3u * prim;
+	Vec3 nrm[3];
+	for(int k = 0; k < 3; ++k) {
+		const float4* vr = sc.vertices + size_t(voff + sc.indices[ib + k]) * 4;
+		const float4 q1 = vr[1], q2 = vr[2];
+		nrm[k] = v3(q1.z, q1.w, q2.x);
+	}
+	const Vec3 tsn = normalize3(nrm[0] * b0 + nrm[1] * b1 + nrm[2] * b2);
+	const Vec3 normal = normalize3(v3(I.nm[0] * tsn.x + I.nm[1] * tsn.y + I.nm[2] * tsn.z,
+									  I.nm[3] * tsn.x + I.nm[4] * tsn.y + I.nm[5] * tsn.z,
+									  I.nm[6] * tsn.x + I.nm[7] * tsn.y + I.nm[8] * tsn.z));
+	const float* m = sc.materials + size_t(mat) * 12;
+	const float metal = m[0], rough = m[1];
+	const Vec3 albedo = v3(m[2], m[3], m[4]);
+	Vec3 color = v3(m[5], m[6], m[7]);
+	const float f0 = 0.04f;
+	const Vec3 diffuse = albedo * ((1.0f - f0) * (1.0f - metal));
+	const Vec3 spec = v3(f0 * (1.0f - metal) + albedo.x * metal, f0 * (1.0f - metal) + albedo.y * metal,
+						 f0 * (1.0f - metal) + albedo.z * metal);
+	const float ndd = dot3(normal, d);
+	const Vec3 refl = d - normal * (2.0f * ndd);
+	const Vec3 to_cam = v3(-d.x, -d.y, -d.z);
+	const Vec3 reflection = sample_probes(grid, states, irr, dep, position, refl, to_cam);
+	color = color + spec * reflection;
+	r.em_spec = color;
+	const Vec3 indirect = sample_probes(grid, states, irr, dep, position, normal, to_cam);
+	color = color + indirect * diffuse;
+	// shadow ray toward the light: any hit, mask 0xFF, tmin 0.1, tmax 10000
+	RayPrep sr;
+	ray_prepare(position.x, position.y, position.z, light.dir[0], light.dir[1], light.dir[2], sr);
+	uint32_t nn = 0, nt = 0;
+	const HitResult sh = traverse<true, STACK, COUNT>(sc.nodes, sc.tris, sr, 0.1f, 10000.0f, 0xFFu, stack, stride, nn, nt);
+	if(COUNT) {
+		atomicAdd(&visit_counters[2], (unsigned long long)nn);
+		atomicAdd(&visit_counters[3], (unsigned long long)nt);
+		atomicAdd(&visit_counters[4], 1ull);
+	}
+	const bool lit = sh.slot == 0xFFFFFFFFu;
+	float fade = 1.0f;
+	if(lit) {
+		const Vec3 L = v3(light.dir[0], light.dir[1], light.dir[2]);
+		color = color + pbr_metallic_roughness(normal, normalize3(to_cam), v3(light.color[0], light.color[1], light.color[2]),
+											   L, albedo, metal, rough);
+		if(L.y < 0.0f) {
+			fade = 1.0f - fminf(fmaxf(-L.y, 0.0f), 0.1f) / 0.1f;
+			color = color * fade;
+		}
+	}
+	r.color = color;
+	r.indirect = indirect;
+	r.normal = normal;
+	r.to_cam = to_cam;
+	r.metal = metal;
+	r.rough = rough;
+	r.fade = fade;
+	r.mat = mat;
+	r.lit = lit;
+	return r;
+}
+
 // Entry k of the frame's shade list (the front-face hits queued by the primary traversal), so that
 // every lane of a wave shades and casts a shadow ray.  Each ray's result depends only on its own
 // record: the order of the list does not change any output.
@@ -123,85 +213,30 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 	const float4 hit = a.hits[ri];
 	const uint32_t code = __float_as_uint(hit.y);
 	const float t = hit.x;
-	const uint32_t tslot = code & 0x7FFFFFFFu;
 	float o[3];
 	probe_origin(a.grid, a.indices[slot], o);
 	const float4 d4 = a.dirs[i];
-	const Vec3 d = v3(d4.x, d4.y, d4.z);
-	const float b1 = hit.z, b2 = hit.w, b0 = 1.0f - b1 - b2;
-	const Vec3 position = v3(d.x * t + o[0], d.y * t + o[1], d.z * t + o[2]);
-	// fetch geometry and material records (closesthit.glsl:147-159)
-	const float4* tr = a.scene.tris + size_t(tslot) * 3;
-	const uint32_t gid = __float_as_uint(tr[0].w), inst = __float_as_uint(tr[1].w);
-	const InstanceDev I = a.scene.inst[inst];
-	const uint32_t prim = gid - I.prim_base;
-	const uint32_t* oe = a.scene.offsets + size_t(I.custom_index) * 3;
-	const uint32_t mat = oe[0], voff = oe[1], ib = oe[2] + 3u * prim;
-	Vec3 nrm[3];
-	for(int k = 0; k < 3; ++k) {
-		const float4* vr = a.scene.vertices + size_t(voff + a.scene.indices[ib + k]) * 4;
-		const float4 q1 = vr[1], q2 = vr[2];
-		nrm[k] = v3(q1.z, q1.w, q2.x);
-	}
-	const Vec3 tsn = normalize3(nrm[0] * b0 + nrm[1] * b1 + nrm[2] * b2);
-	const Vec3 normal = normalize3(v3(I.nm[0] * tsn.x + I.nm[1] * tsn.y + I.nm[2] * tsn.z,
-									  I.nm[3] * tsn.x + I.nm[4] * tsn.y + I.nm[5] * tsn.z,
-									  I.nm[6] * tsn.x + I.nm[7] * tsn.y + I.nm[8] * tsn.z));
-	const float* m = a.scene.materials + size_t(mat) * 12;
-	const float metal = m[0], rough = m[1];
-	const Vec3 albedo = v3(m[2], m[3], m[4]);
-	Vec3 color = v3(m[5], m[6], m[7]);
-	const float f0 = 0.04f;
-	const Vec3 diffuse = albedo * ((1.0f - f0) * (1.0f - metal));
-	const Vec3 spec = v3(f0 * (1.0f - metal) + albedo.x * metal, f0 * (1.0f - metal) + albedo.y * metal,
-						 f0 * (1.0f - metal) + albedo.z * metal);
-	const float ndd = dot3(normal, d);
-	const Vec3 refl = d - normal * (2.0f * ndd);
-	const Vec3 to_cam = v3(-d.x, -d.y, -d.z);
-	const Vec3 reflection = sample_probes(a.grid, a.states, a.irr, a.dep, position, refl, to_cam);
-	color = color + spec * reflection;
-	const Vec3 indirect = sample_probes(a.grid, a.states, a.irr, a.dep, position, normal, to_cam);
-	color = color + indirect * diffuse;
-	// shadow ray toward the light: any hit, mask 0xFF, tmin 0.1, tmax 10000
-	RayPrep sr;
-	ray_prepare(position.x, position.y, position.z, a.light.dir[0], a.light.dir[1], a.light.dir[2], sr);
-	uint32_t nn = 0, nt = 0;
-	const HitResult sh = traverse<true, STACK, COUNT>(a.scene.nodes, a.scene.tris, sr, 0.1f, 10000.0f, 0xFFu, stack,
-													  stride, nn, nt);
-	if(COUNT) {
-		atomicAdd(&a.visit_counters[2], (unsigned long long)nn);
-		atomicAdd(&a.visit_counters[3], (unsigned long long)nt);
-		atomicAdd(&a.visit_counters[4], 1ull);
-	}
-	const bool lit = sh.slot == 0xFFFFFFFFu;
-	float fade = 1.0f;
-	if(lit) {
-		const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
-		color = color + pbr_metallic_roughness(normal, normalize3(to_cam), v3(a.light.color[0], a.light.color[1], a.light.color[2]),
-											   L, albedo, metal, rough);
-		if(L.y < 0.0f) {
-			fade = 1.0f - fminf(fmaxf(-L.y, 0.0f), 0.1f) / 0.1f;
-			color = color * fade;
-		}
-	}
-	a.rays[ri] = make_float4(color.x, color.y, color.z, t);
+	const SurfaceShade r = shade_surface<STACK, COUNT>(a.scene, a.grid, a.light, a.states, a.irr, a.dep, o,
+														v3(d4.x, d4.y, d4.z), t, code & 0x7FFFFFFFu, hit.z, hit.w, stack,
+														stride, a.visit_counters);
+	a.rays[ri] = make_float4(r.color.x, r.color.y, r.color.z, t);
 	if(a.factors) {
 		// affine decomposition in (albedo, light radiance) for non-metallic materials
-		const Vec3 em = v3(m[5], m[6], m[7]);
-		const Vec3 c0 = (em + spec * reflection) * fade;
-		const Vec3 A = indirect * ((1.0f - f0) * (1.0f - metal) * fade);
+		const float f0 = 0.04f;
+		const Vec3 c0 = r.em_spec * r.fade;
+		const Vec3 A = r.indirect * ((1.0f - f0) * (1.0f - r.metal) * r.fade);
 		Vec3 B = v3(0.0f, 0.0f, 0.0f), C = v3(0.0f, 0.0f, 0.0f);
-		if(lit) {
+		if(r.lit) {
 			const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
 			const Vec3 one = v3(1.0f, 1.0f, 1.0f), zero = v3(0.0f, 0.0f, 0.0f);
-			const Vec3 with = pbr_metallic_roughness(normal, normalize3(to_cam), one, L, one, metal, rough);
-			const Vec3 without = pbr_metallic_roughness(normal, normalize3(to_cam), one, L, zero, metal, rough);
-			B = (with - without) * fade;
-			C = without * fade;
+			const Vec3 with = pbr_metallic_roughness(r.normal, normalize3(r.to_cam), one, L, one, r.metal, r.rough);
+			const Vec3 without = pbr_metallic_roughness(r.normal, normalize3(r.to_cam), one, L, zero, r.metal, r.rough);
+			B = (with - without) * r.fade;
+			C = without * r.fade;
 		}
 		float4* f = a.factors + ri * 4;
-		f[0] = make_float4(c0.x, c0.y, c0.z, __uint_as_float(mat));
-		f[1] = make_float4(A.x, A.y, A.z, __uint_as_float(lit ? 1u : 0u));
+		f[0] = make_float4(c0.x, c0.y, c0.z, __uint_as_float(r.mat));
+		f[1] = make_float4(A.x, A.y, A.z, __uint_as_float(r.lit ? 1u : 0u));
 		f[2] = make_float4(B.x, B.y, B.z, 0.0f);
 		f[3] = make_float4(C.x, C.y, C.z, 0.0f);
 	}
@@ -450,6 +485,100 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(GatherArgs a) {
 	o[0] = color.x;
 	o[1] = color.y;
 	o[2] = color.z;
+}
+
+// ---- ray-traced shadow pass (directLight.rgen:42-99) ----
+// Miss texel: -1.  Otherwise the sun direction, jittered inside a 0.02 rad cone by the texel's
+// noise sample (none: the exact direction); facing away from the normal: 0; else an any-hit ray
+// (mask 0xFF, tmin 0.01, tmax 10000) gives (visibility, previous g, previous b, 1), visibility 1 on
+// escape, faded while the sun is below the horizon.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_direct_light(DirectLightArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint64_t id = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+	if(id >= a.count) return;
+	const ddgi_gbuffer_texel g = a.gbuffer[id];
+	float4 out = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
+	if(g.depth > 0.0f) {
+		const Vec3 normal = v3(g.normal[0], g.normal[1], g.normal[2]);
+		Vec3 dir = v3(a.sdir[0], a.sdir[1], a.sdir[2]);
+		if(a.noise) {
+			const float2 nz = a.noise[id];
+			const Vec3 temp = rotate_axis(dir, normalize3(cross3(normal, dir)), 2.0f * (nz.x - 0.5f) * 0.02f);
+			dir = rotate_axis(temp, dir, (2.0f * sky::kPi) * nz.y);
+		}
+		out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		if(dot3(dir, normal) > 0.0f) {
+			RayPrep sr;
+			ray_prepare(g.position[0], g.position[1], g.position[2], dir.x, dir.y, dir.z, sr);
+			uint32_t nn, nt;
+			const HitResult sh = traverse<true, STACK>(a.scene.nodes, a.scene.tris, sr, 0.01f, 10000.0f, 0xFFu,
+														stack + threadIdx.x, kTraceBlock, nn, nt);
+			float vis = 0.0f;
+			if(sh.slot == 0xFFFFFFFFu) {
+				vis = 1.0f;
+				if(dir.y < 0.0f) vis *= 1.0f - fminf(fmaxf(-dir.y, 0.0f), 0.1f) / 0.1f;
+			}
+			const float4 prev = a.previous ? a.previous[id] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			out = make_float4(vis, prev.y, prev.z, 1.0f);
+		}
+	}
+	a.out[id] = out;
+}
+
+// ---- reflection pass (reflection.rgen:101-175) ----
+// Texels with depth > 0 and (roughness < 0.4 or metalness > 0.01) trace one ray along the mirror
+// direction of the view vector, rotated by the texel's noise sample within the roughness lobe (no
+// sample: the mirror direction), closest hit, mask 0xFF, tmin 0.1, tmax 10000.  The hit is shaded
+// by closesthit.glsl at recursion depth 1 (its specular term reads the probes), a back face is
+// black, a miss sees the sky; out = (colorCompression(radiance), roughness), other texels 0.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_reflection(ReflectionArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint64_t id = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
+	if(id >= a.count) return;
+	const ddgi_gbuffer_texel g = a.gbuffer[id];
+	float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	if(g.depth > 0.0f && (g.roughness < 0.4f || g.metalness > 0.01f)) {
+		const float o[3] = {g.position[0], g.position[1], g.position[2]};
+		const Vec3 position = v3(o[0], o[1], o[2]);
+		const Vec3 normal = v3(g.normal[0], g.normal[1], g.normal[2]);
+		const Vec3 to_origin = normalize3(v3(a.origin[0], a.origin[1], a.origin[2]) - position);
+		const Vec3 incident = v3(-to_origin.x, -to_origin.y, -to_origin.z);
+		const Vec3 rdir = normalize3(incident - normal * (2.0f * dot3(normal, incident)));
+		Vec3 dir = rdir;
+		if(a.noise) {
+			const float2 nz = a.noise[id];
+			const float theta = g.roughness * (nz.x - 0.5f) * 2.0f * sky::kPi;
+			const float phi = (nz.y - 0.5f) * 2.0f * sky::kPi;
+			const Vec3 tangent = dot3(rdir, normal) < 0.9f ? normalize3(cross3(rdir, normal))
+														   : normalize3(cross3(rdir, v3(1.0f, 0.0f, 0.0f)));
+			dir = rotate_axis(rdir, tangent, theta);
+			dir = rotate_axis(dir, rdir, phi);
+		}
+		RayPrep r;
+		ray_prepare(o[0], o[1], o[2], dir.x, dir.y, dir.z, r);
+		uint32_t nn, nt;
+		const HitResult h = traverse<false, STACK>(a.scene.nodes, a.scene.tris, r, 0.1f, 10000.0f, 0xFFu,
+													stack + threadIdx.x, kTraceBlock, nn, nt);
+		Vec3 c = v3(0.0f, 0.0f, 0.0f);
+		const float4 hit = pack_hit(h);
+		const uint32_t code = __float_as_uint(hit.y);
+		if(code == 0xFFFFFFFFu) {  // miss.rmiss
+			c = v3(a.light.sky[0], a.light.sky[1], a.light.sky[2]);
+			if(a.light.sky_model == DDGI_SKY_ATMOSPHERE)
+				c = sky_atmosphere(position, dir, v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]),
+								   v3(a.light.color[0], a.light.color[1], a.light.color[2]), a.light.sky_brightness, true);
+		} else if(!(code & 0x80000000u)) {
+			c = shade_surface<STACK, false>(a.scene, a.grid, a.light, a.states, a.irr, a.dep, o, dir, hit.x, code, hit.z,
+											 hit.w, stack + threadIdx.x, kTraceBlock, nullptr)
+					.color;
+		}
+		// colorCompression = reinhard_whitepoint(v, 1): v (1 + v / 1) / (1 + v)
+		out = make_float4(c.x * (1.0f + c.x / 1.0f) / (1.0f + c.x), c.y * (1.0f + c.y / 1.0f) / (1.0f + c.y),
+						  c.z * (1.0f + c.z / 1.0f) / (1.0f + c.z), g.roughness);
+	}
+	a.out[id] = out;
 }
 
 // ---- instance moves (the TLAS instance update + refit of src/Renderer.cpp:671-742) ----
@@ -783,6 +912,18 @@ hipError_t launch_gbuffer(const SceneDev& sc, const ddgi_ray* rays, uint64_t cou
 hipError_t launch_gather(const GatherArgs& a, hipStream_t s) {
 	if(a.count == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_gather<kLdsStack>, dim3(blocks_for(a.count, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_direct_light(const DirectLightArgs& a, hipStream_t s) {
+	if(a.count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_direct_light<kLdsStack>, dim3(blocks_for(a.count, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_reflection(const ReflectionArgs& a, hipStream_t s) {
+	if(a.count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_reflection<kLdsStack>, dim3(blocks_for(a.count, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
 	return hipGetLastError();
 }
 

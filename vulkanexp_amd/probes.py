@@ -150,6 +150,36 @@ class IrradianceProbes:
                                     _stream_ptr(stream)), self.handle)
         return out
 
+    def direct_light(self, gbuffer, noise=None, previous=None, stream=None):
+        """Ray-traced shadow pass (directLight.rgen) of a G-buffer -> (N, 4) float32 CUDA tensor
+        (visibility, previous g, previous b, 1), -1 on misses, 0 facing away.  noise: (N, 2) CUDA
+        float32 blue-noise samples (None: no jitter); previous: (N, 4) CUDA float32 or None."""
+        import torch
+        gbuffer = gbuffer.contiguous()
+        n = int(gbuffer.shape[0])
+        out = torch.empty((n, 4), dtype=torch.float32, device=gbuffer.device)
+        nz = noise.contiguous() if noise is not None else None
+        pv = previous.contiguous() if previous is not None else None
+        check(self._lib.ddgi_direct_light(self.handle, _tensor_ptr(gbuffer), n,
+                                          _tensor_ptr(nz) if nz is not None else None,
+                                          _tensor_ptr(pv) if pv is not None else None, _tensor_ptr(out),
+                                          _stream_ptr(stream)), self.handle)
+        return out
+
+    def reflection(self, gbuffer, view_origin, noise=None, stream=None):
+        """Reflection pass (reflection.rgen) of a G-buffer -> (N, 4) float32 CUDA tensor
+        (compressed rgb, roughness).  noise: (N, 2) CUDA float32 samples or None (mirror rays)."""
+        import torch
+        gbuffer = gbuffer.contiguous()
+        n = int(gbuffer.shape[0])
+        out = torch.empty((n, 4), dtype=torch.float32, device=gbuffer.device)
+        origin = np.ascontiguousarray(view_origin, np.float32).reshape(3)
+        nz = noise.contiguous() if noise is not None else None
+        check(self._lib.ddgi_reflection(self.handle, _tensor_ptr(gbuffer), n, _p(origin),
+                                        _tensor_ptr(nz) if nz is not None else None, _tensor_ptr(out),
+                                        _stream_ptr(stream)), self.handle)
+        return out
+
     def trace_rays(self, rays, mask: int = 0xFF, any_hit: bool = False, stream=None, cull_back: bool = False):
         """rays: CUDA float32 tensor (N, 8) = (origin, tmin, direction, tmax).  Returns a CUDA int32
         tensor (N, 6): t bits, u bits, v bits, instance, primitive, flags."""
