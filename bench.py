@@ -152,8 +152,18 @@ def frame_bench(probes, scene, width=1920, height=1080):
     f_ms = _event_ms(lambda: probes.gather(gb, eye))
     hit = float((gb[:, 3] > 0).float().mean())
     px = width * height
+    # ray-traced passes: jittered shadow rays, and reflections with every hit texel made glossy
+    # (roughness 0.25 < 0.4) so that each one traces and shades a reflected ray
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    noise = torch.rand((px, 2), device="cuda", generator=gen)
+    d_ms = _event_ms(lambda: probes.direct_light(gb, noise=noise))
+    glossy = gb.clone()
+    glossy[:, 11] = 0.25
+    probes.reflection(glossy, eye, noise=noise)
+    r_ms = _event_ms(lambda: probes.reflection(glossy, eye, noise=noise))
     return {"pixels": px, "hit_fraction": round(hit, 3), "gbuffer_ms": round(g_ms, 4), "gather_ms": round(f_ms, 4),
-            "Mpixels/s": round(px / ((g_ms + f_ms) * 1e-3) / 1e6, 1)}
+            "Mpixels/s": round(px / ((g_ms + f_ms) * 1e-3) / 1e6, 1), "shadow_pass_ms": round(d_ms, 4),
+            "reflection_pass_ms": round(r_ms, 4), "reflection_rays": int(round(hit * px))}
 
 
 def dynamic_bench(probes, scene, R, lib):
