@@ -222,7 +222,13 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 		code[i] = sw_ ? cj_ : ci_;                     \
 		code[j] = sw_ ? ci_ : cj_;                     \
 	}
+	// Leaves are postponed (Aila and Laine's speculative traversal): a lane that reaches a leaf keeps
+	// it in `leaf` and goes on descending from its stack until every active lane of the wavefront
+	// holds a leaf (or has run out of nodes); the wavefront then tests leaves together.  Closest hit
+	// is the minimum of (t, global id) over all triangles, so the order changes no result.
+	constexpr uint32_t kNone = 0xFFFFFFFFu;  // no node (empty stack) / no postponed leaf
 	uint32_t node = 0;  // root (always an inner node)
+	uint32_t leaf = kNone;
 	uint32_t n_nodes = 0, n_tris = 0;
 	const float kInf = __int_as_float(0x7f800000);
 	for(;;) {
@@ -244,22 +250,27 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 			key[3] = (slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, best.t, tn) && code[3] != kEmptyChild) ? tn : kInf;
 			const int nh = int(key[0] != kInf) + int(key[1] != kInf) + int(key[2] != kInf) + int(key[3] != kInf);
 			if(nh == 0) {
-				DDGI_STACK_POP(node, goto done)
-				continue;
+				DDGI_STACK_POP(node, node = kNone)
+			} else {
+				DDGI_CSWAP(0, 1)
+				DDGI_CSWAP(2, 3)
+				DDGI_CSWAP(0, 2)
+				DDGI_CSWAP(1, 3)
+				DDGI_CSWAP(1, 2)
+				node = code[0];
+				if(nh > 3) DDGI_STACK_PUSH(code[3])
+				if(nh > 2) DDGI_STACK_PUSH(code[2])
+				if(nh > 1) DDGI_STACK_PUSH(code[1])
 			}
-			DDGI_CSWAP(0, 1)
-			DDGI_CSWAP(2, 3)
-			DDGI_CSWAP(0, 2)
-			DDGI_CSWAP(1, 3)
-			DDGI_CSWAP(1, 2)
-			node = code[0];
-			if(nh > 3) DDGI_STACK_PUSH(code[3])
-			if(nh > 2) DDGI_STACK_PUSH(code[2])
-			if(nh > 1) DDGI_STACK_PUSH(code[1])
+			if(leaf == kNone && node != kNone && (node & kLeafFlag)) {  // postpone the leaf
+				leaf = node;
+				DDGI_STACK_POP(node, node = kNone)
+			}
+			if(__ballot(leaf == kNone) == 0ull) break;  // every active lane holds a leaf
 		}
-		// leaf: up to 8 triangles, two per step
-		{
-			const uint32_t first = (node & ~kLeafFlag) >> 3, cnt = (node & 7u) + 1u;
+		// leaves: the postponed one, then any leaves next on the stack
+		while(leaf != kNone) {
+			const uint32_t first = (leaf & ~kLeafFlag) >> 3, cnt = (leaf & 7u) + 1u;
 			for(uint32_t k = 0; k < cnt; k += 2) {
 				const float4* tr = tris + size_t(first + k) * 3;
 				const float4 ta = tr[0], tb = tr[1], tc = tr[2];
@@ -275,8 +286,13 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 				if(test_triangle<ANY, CULL>(r, tmin, mask, first + k, ta, tb, tc, best) && ANY) goto done;
 				if(two && test_triangle<ANY, CULL>(r, tmin, mask, first + k + 1, ua, ub, uc, best) && ANY) goto done;
 			}
+			leaf = kNone;
+			if(node != kNone && (node & kLeafFlag)) {
+				leaf = node;
+				DDGI_STACK_POP(node, node = kNone)
+			}
 		}
-		DDGI_STACK_POP(node, break)
+		if(node == kNone) break;
 	}
 done:
 #undef DDGI_STACK_POP
