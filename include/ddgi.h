@@ -251,7 +251,7 @@ ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, 
 ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes);
 
 /* ---- scene ingestion (no counterpart in IrradianceProbes: the reference's Scene::load / save,
- * src/Scene.cpp:30-49 (.scene, .gltf, .glb), :115-533 (glTF 2.0), :647-684 (.mat), :710-934
+ * src/Scene.cpp:30-49 (.scene, .gltf, .glb, .obj), :115-533 (glTF 2.0), :534-646 (OBJ), :647-684 (.mat), :710-934
  * (.scene), flattened as Renderer::updateMeshOffsetTable + createTLAS build their tables,
  * src/Renderer.cpp:97-126, :512-551).  Host-only; a scene object owns the tables it returns. ----
  * The flattening: one offset entry per mesh (= glTF primitive) with indices, meshes packed back to
@@ -259,7 +259,9 @@ ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* tria
  *   order, with the node's global transform, mask DDGI_MASK_STATIC and no facing flip; a default
  *   material (metallic 1, roughness 1, white) is appended when a mesh refers to a missing one.
  *   Skinned primitives are loaded but not instanced.  Textures are kept as records only.
- * ddgi_scene_load: .scene, .gltf (external or base64 buffers) or .glb.
+ * ddgi_scene_load: .scene, .gltf (external or base64 buffers), .glb or .obj (Scene::loadOBJ,
+ *   src/Scene.cpp:534-646: triangles, groups as sub-meshes, vertex normals recomputed as the
+ *   reference does).
  * ddgi_scene_load_material: adds the materials (and texture records) of a .mat file.
  * ddgi_scene_create: a scene from flattened tables (static instances only), e.g. to save one.
  * ddgi_scene_save: writes the .scene container (header "SCEN", a JSON chunk, then per mesh a

@@ -1,6 +1,6 @@
 """Scene ingestion on the host (libddgi's loaders, no GPU): glTF 2.0 (.gltf with an external or a
-base64 buffer, .glb), the .scene container (load and save) and .mat material files, flattened into
-the ddgi_set_scene tables.
+base64 buffer, .glb), the .scene container (load and save), Wavefront OBJ and .mat material files,
+flattened into the ddgi_set_scene tables.
 
 Pinned against data the loaders do not produce themselves: the reference's own glTF fixture
 (data/debug-models/sphere.gltf + sphere.bin, copied to tests/golden/gltf) decoded independently
@@ -205,12 +205,77 @@ def test_material_file_adds_materials_and_texture_records():
         (0, 1, 2, 0xFFFFFFFF)
 
 
+_CUBE_OBJ = """# unit cube faces and a triangle; a group's vertices follow its `g`, as the reference expects
+o cube
+g bottom_and_top
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v 0 0 1
+v 1 0 1
+v 1 1 1
+v 0 1 1
+vt 0.25 0.75
+vn 0 0 1
+f 1 3 2
+f 1 4 3
+f 5/1 6 7
+f 5 7 8//1
+g sides
+v 2 0 0
+v 3 0 0
+v 3 1 0
+f 9 10 11
+usemtl ignored
+"""
+
+
+def test_obj_groups_indices_and_recomputed_normals(tmp_path):
+    """OBJ as the reference loads it: a `g` with vertices already read closes the sub-mesh (its
+    normals recomputed as the average of the adjacent triangles' normalize(cross(p0 - p1, p2 - p0)),
+    inward for counter-clockwise faces) and opens the next; face indices are absolute in the file;
+    `vt`/`vn` indices set that vertex's attributes (the normal is recomputed afterwards anyway)."""
+    path = tmp_path / "cube.obj"
+    path.write_text(_CUBE_OBJ)
+    s = scenes.load(str(path))
+    # two meshes: the 8-vertex cube faces (4 triangles) and the 3-vertex triangle
+    assert s.offsets.tolist() == [(0, 0, 0), (0, 8, 12)]
+    assert len(s.vertices) == 11 and s.indices.tolist() == [0, 2, 1, 0, 3, 2, 4, 5, 6, 4, 6, 7, 0, 1, 2]
+    assert len(s.instances) == 2 and s.instances["triangle_count"].tolist() == [4, 1]
+    assert np.all(s.instances["transform"] == scenes.IDENTITY_3x4)
+    pos = s.vertices["pos"].astype(np.float64)
+    # recomputed normals: the bottom face (z = 0, CCW seen from -z... i.e. faces 1 3 2 / 1 4 3) and
+    # the top face (z = 1): each triangle's normalize(cross(p0 - p1, p2 - p0))
+    want = np.zeros((8, 3))
+    cnt = np.zeros(8)
+    idx = s.indices[:12].reshape(-1, 3)
+    for a, b, c in idx:
+        n = np.cross(pos[a] - pos[b], pos[c] - pos[a])
+        n /= np.linalg.norm(n)
+        for v in (a, b, c):
+            want[v] += n
+            cnt[v] += 1
+    used = cnt > 0
+    assert np.allclose(s.vertices["normal"][:8][used], want[used] / cnt[used, None], atol=1e-7)
+    # the bottom face's triangles wind 1 3 2 (clockwise seen from +z): cross(p0 - p1, p2 - p0) = +z
+    assert np.allclose(s.vertices["normal"][0], [0, 0, 1]) and np.allclose(s.vertices["normal"][4], [0, 0, -1])
+    assert np.all(s.vertices["tangent"][:8][used, 3] == 1)
+    # vt index 1 on vertex 5 of the file; colours default to white
+    assert s.vertices["tex_coord"][4].tolist() == [0.25, 0.75] and np.all(s.vertices["color"] == 1)
+    assert len(s.materials) == 1  # the default material: OBJ materials are not read
+    assert np.allclose(s.bounds_min, [0, 0, 0]) and np.allclose(s.bounds_max, [3, 1, 1])
+
+
 def test_loader_errors(tmp_path):
     with pytest.raises(_lib.DdgiError):
         scenes.load(str(tmp_path / "missing.gltf"))
-    (tmp_path / "x.obj").write_text("v 0 0 0\n")
+    (tmp_path / "x.ply").write_text("ply\n")
     with pytest.raises(_lib.DdgiError):
-        scenes.load(str(tmp_path / "x.obj"))
+        scenes.load(str(tmp_path / "x.ply"))
+    (tmp_path / "bad.obj").write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\n")  # vertex 3 does not exist
+    with pytest.raises(_lib.DdgiError):
+        scenes.load(str(tmp_path / "bad.obj"))
     (tmp_path / "bad.gltf").write_text('{"meshes": [')
     with pytest.raises(_lib.DdgiError):
         scenes.load(str(tmp_path / "bad.gltf"))

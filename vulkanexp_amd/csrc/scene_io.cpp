@@ -862,6 +862,142 @@ void load_gltf(Scene& sc, const fs::path& file) {
 }
 
 // ============================================================================================
+// Wavefront OBJ (src/Scene.cpp:534-646), with the reference's behaviour kept: `o` starts a new mesh
+// node but keeps filling the current mesh until a `g` starts a new sub-mesh; face indices are
+// absolute in the file and made relative to the current sub-mesh; only triangles; `vt`/`vn`
+// indices in a face set that vertex's attributes; every finished sub-mesh gets its vertex normals
+// recomputed (Mesh::computeVertexNormals, src/vulkan/Mesh.cpp:69-94: the average of the adjacent
+// triangles' normalize(cross(p0 - p1, p2 - p0)), which for counter-clockwise faces points inward,
+// and tangents normalize(p0 - p1)), overwriting the file's normals.  Unlike the reference, an index
+// outside the data read so far is an error instead of an out-of-bounds write.
+void compute_vertex_normals(SMesh& m) {
+	for(auto& v : m.vertices) {
+		for(int k = 0; k < 3; ++k) v.normal[k] = 0.0f;
+		for(int k = 0; k < 4; ++k) v.tangent[k] = 0.0f;
+	}
+	for(size_t i = 0; i + 2 < m.indices.size(); i += 3) {
+		const float* p0 = m.vertices[m.indices[i]].pos;
+		const float* p1 = m.vertices[m.indices[i + 1]].pos;
+		const float* p2 = m.vertices[m.indices[i + 2]].pos;
+		const float a[3] = {p0[0] - p1[0], p0[1] - p1[1], p0[2] - p1[2]};
+		const float b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+		float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+		const float nl = 1.0f / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+		const float al = 1.0f / std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+		for(int j = 0; j < 3; ++j) {
+			ddgi_vertex& v = m.vertices[m.indices[i + j]];
+			for(int k = 0; k < 3; ++k) {
+				v.normal[k] += n[k] * nl;
+				v.tangent[k] += a[k] * al;
+			}
+			v.tangent[3] += 1.0f;
+		}
+	}
+	for(auto& v : m.vertices) {  // a vertex no triangle uses divides by zero, as in the reference
+		for(int k = 0; k < 3; ++k) {
+			v.normal[k] = v.normal[k] / v.tangent[3];
+			v.tangent[k] = v.tangent[k] / v.tangent[3];
+		}
+		v.tangent[3] = 1.0f;
+	}
+}
+
+void load_obj(Scene& sc, const fs::path& file) {
+	std::ifstream f(file);
+	if(!f) throw Failure(DDGI_ERROR_INVALID_ARGUMENT, "cannot open '" + file.string() + "'");
+	const int root = sc.add_node("Unamed Node");
+	sc.add_child(sc.root, root);
+	int mesh_node = -1, cur = -1;  // current mesh node, current mesh
+	uint32_t vertex_offset = 0;
+	std::vector<std::array<float, 2>> uvs;
+	std::vector<std::array<float, 3>> normals;
+	auto next_mesh = [&] {
+		mesh_node = sc.add_node("Unamed Node");
+		sc.add_child(root, mesh_node);
+	};
+	auto next_submesh = [&] {
+		if(cur >= 0) vertex_offset += uint32_t(sc.meshes[cur].vertices.size());
+		sc.meshes.emplace_back();
+		sc.meshes.back().name = "AnonMesh";
+		cur = int(sc.meshes.size()) - 1;
+		const int sub = sc.add_node("Unamed Node");
+		sc.add_child(mesh_node, sub);
+		sc.nodes[sub].mesh = cur;
+		sc.nodes[sub].material = 0xFFFFFFFFu;  // a renderer without a material of its own
+	};
+	next_mesh();
+	next_submesh();
+	ddgi_vertex v{};
+	v.color[0] = v.color[1] = v.color[2] = 1.0f;
+	std::string line;
+	size_t lineno = 0;
+	auto where = [&] { return file.filename().string() + ":" + std::to_string(lineno); };
+	while(std::getline(f, line)) {
+		++lineno;
+		if(!line.empty() && line.back() == '\r') line.pop_back();
+		if(line.empty()) continue;
+		const size_t first = line.find_first_not_of(" \t");
+		if(first == std::string::npos || line[first] == '#') continue;
+		SMesh& m = sc.meshes[cur];
+		if(line[0] == 'v') {
+			if(line.size() > 1 && line[1] == 't') {
+				char* p = line.data() + std::min<size_t>(3, line.size());
+				std::array<float, 2> uv{};
+				for(int k = 0; k < 2; ++k) uv[k] = std::strtof(p, &p);
+				uvs.push_back(uv);
+			} else if(line.size() > 1 && line[1] == 'n') {
+				char* p = line.data() + std::min<size_t>(3, line.size());
+				std::array<float, 3> n{};
+				for(int k = 0; k < 3; ++k) n[k] = std::strtof(p, &p);
+				const float l = 1.0f / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+				for(int k = 0; k < 3; ++k) n[k] *= l;
+				normals.push_back(n);
+			} else {
+				char* p = line.data() + std::min<size_t>(2, line.size());
+				for(int k = 0; k < 3; ++k) v.pos[k] = std::strtof(p, &p);
+				m.vertices.push_back(v);
+			}
+		} else if(line[0] == 'f') {
+			char* p = line.data() + std::min<size_t>(2, line.size());
+			for(int k = 0; k < 3; ++k) {  // triangles only
+				const long vi = std::strtol(p, &p, 10);
+				if(vi <= 0) bad(where() + ": OBJ face index must be a positive absolute index");
+				const long rel = vi - 1 - long(vertex_offset);
+				if(rel < 0 || size_t(rel) >= m.vertices.size()) bad(where() + ": OBJ face refers to a vertex outside its group");
+				m.indices.push_back(uint32_t(rel));
+				if(*p == '/') {
+					++p;
+					const long ti = std::strtol(p, &p, 10);
+					if(ti > 0) {
+						if(size_t(ti) > uvs.size()) bad(where() + ": OBJ face refers to a missing texture coordinate");
+						m.vertices[rel].tex_coord[0] = uvs[ti - 1][0];
+						m.vertices[rel].tex_coord[1] = uvs[ti - 1][1];
+					}
+					if(*p == '/') {
+						++p;
+						const long ni = std::strtol(p, &p, 10);
+						if(ni > 0) {
+							if(size_t(ni) > normals.size()) bad(where() + ": OBJ face refers to a missing normal");
+							for(int q = 0; q < 3; ++q) m.vertices[rel].normal[q] = normals[ni - 1][q];
+						}
+					}
+				}
+			}
+		} else if(line[0] == 'o') {
+			if(!m.vertices.empty()) next_mesh();
+		} else if(line[0] == 'g') {
+			if(!m.vertices.empty()) {
+				compute_vertex_normals(m);
+				m.compute_bounds();
+				next_submesh();
+			}
+		}  // other statements (usemtl, mtllib, s, ...) are ignored, as the reference does with a warning
+	}
+	compute_vertex_normals(sc.meshes[cur]);
+	sc.meshes[cur].compute_bounds();
+}
+
+// ============================================================================================
 // .scene (src/Scene.cpp:710-934)
 void load_scene_file(Scene& sc, const fs::path& file) {
 	auto c = read_container(read_file(file), kMagicScene);
@@ -1047,6 +1183,9 @@ ddgi_status ddgi_scene_load(const char* path, void** out_scene) {
 		} else if(ext == ".gltf" || ext == ".glb") {
 			sc->root = sc->add_node("Root");
 			load_gltf(*sc, file);
+		} else if(ext == ".obj") {
+			sc->root = sc->add_node("Root");
+			load_obj(*sc, file);
 		} else {
 			throw Failure(DDGI_ERROR_UNSUPPORTED, "unsupported scene file extension '" + ext + "'");
 		}

@@ -363,7 +363,7 @@ def _scene_from_handle(lib, h, name: str) -> Scene:
 
 
 def load(path: str, materials=()) -> Scene:
-    """Load a .scene, .gltf or .glb file (Scene::load, src/Scene.cpp:30-49), add the materials of the
+    """Load a .scene, .gltf, .glb or .obj file (Scene::load, src/Scene.cpp:30-49), add the materials of the
     given .mat files, and flatten it into the tables ddgi_set_scene takes."""
     import ctypes as C
     import os
