@@ -391,11 +391,11 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->perm, kMaxRays * sizeof(uint16_t));
 	if(e == hipSuccess) e = dalloc(&h->fib512, 512 * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->dirs, 512 * sizeof(float4));
-	if(e == hipSuccess) e = dalloc(&h->wt, size_t(kMaxRays) * kBlendTexels * sizeof(float));
+	if(e == hipSuccess) e = dalloc(&h->wt, size_t(kWeightRows) * kMaxRays * sizeof(float));
 	if(e == hipSuccess) e = dalloc(&h->inv_sum, kBlendTexels * sizeof(float));
 	// 16-bit MFMA weights: 256 padded rows x 512 rays
-	if(e == hipSuccess) e = dalloc(&h->w_hi, size_t(256) * kMaxRays * 2);
-	if(e == hipSuccess) e = dalloc(&h->w_lo, size_t(256) * kMaxRays * 2);
+	if(e == hipSuccess) e = dalloc(&h->w_hi, size_t(kWeightRows) * kMaxRays * 2);
+	if(e == hipSuccess) e = dalloc(&h->w_lo, size_t(kWeightRows) * kMaxRays * 2);
 	if(e == hipSuccess) e = dalloc(&h->visit, 8 * sizeof(unsigned long long));
 	if(e != hipSuccess) {
 		h->err = std::string("allocation failed: ") + hipGetErrorString(e);
@@ -842,7 +842,7 @@ ddgi_status ddgi_diff_prepare(void* handle, const float orientation[9], const fl
 	if(e == hipSuccess) e = diff_alloc(h, &dep, P * 256 * sizeof(float2));
 	if(e == hipSuccess) e = diff_alloc(h, &states, P * 4);
 	if(e == hipSuccess) e = diff_alloc(h, &indices, P * 4);
-	if(e == hipSuccess) e = diff_alloc(h, &D.wt, size_t(kMaxRays) * kBlendTexels * 4);
+	if(e == hipSuccess) e = diff_alloc(h, &D.wt, size_t(kWeightRows) * kMaxRays * 4);
 	if(e == hipSuccess) e = diff_alloc(h, &D.inv_sum, kBlendTexels * 4);
 	if(e == hipSuccess) e = diff_alloc(h, &D.dirs, kMaxRays * sizeof(float4));
 	if(e == hipSuccess) e = diff_alloc(h, &D.count, 4);

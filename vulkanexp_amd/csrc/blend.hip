@@ -2,11 +2,13 @@
 // (src/shaders/probesCopyBorders.comp:21-246) and the probe state machine, on gfx950.
 //
 // The reference recomputes max(0, dot(texelDir, rayDir)) (and its pow) per probe, although the
-// weights depend only on the frame's rotation.  Here one small kernel computes, once per frame,
-//   Wt[i][t] = max(0, texelDir_t . rayDir_i)               t < 36   (irradiance texels)
-//   Wt[i][36 + t] = max(0, texelDir_t . rayDir_i)^sharpness  t < 196  (depth texels)
+// weights depend only on the frame's rotation.  Here one small kernel computes, once per frame, the
+// weight matrix in 256 padded rows of n_pad rays (irradiance texels at rows 0..35, depth texels at
+// rows 48..243, the other rows and the rays beyond n zero)
+//   W[t][i] = max(0, texelDir_t . rayDir_i)               irradiance texel t
+//   W[48 + t][i] = max(0, texelDir_t . rayDir_i)^sharpness  depth texel t
 //   inv_sum[t] = 1 / sum_i W (1 when the sum <= 1e-3, i.e. no normalisation)
-// and the blend of a probe is a [232 x n] x [n x 3] product.  The border copy is fused into the
+// and the blend of a probe is a [232 x n] x [n x 3] product, on the matrix cores.  The border copy is fused into the
 // epilogue: each interior texel also writes the border texels that mirror it, so the separate
 // border pass and its re-read of the atlas disappear.
 #include <hip/hip_bf16.h>
@@ -18,13 +20,10 @@ namespace ddgi {
 
 namespace {
 
-constexpr int kBlendThreads = 256;  // one thread per texel (232 used)
 constexpr int kProbesPerBlock = 16;
-constexpr int kRayChunk = 32;
 
-// One block per padded weight row (256 rows: irradiance texels at rows 0..35, depth texels at rows
-// 48..243, the rest zero): the weight of every ray, their sum, and for the MFMA blends the 16-bit
-// copies in [row][n_pad] layout (BF16: hi/lo split; FP16: one copy).
+// One block per padded weight row: the weight of every ray and their sum, stored in [row][n_pad]
+// layout as FP32, or as 16-bit copies (BF16: hi/lo split; FP16: one copy).
 __device__ __forceinline__ void blend_weight_row(const GridConst& g, const float4* __restrict__ dirs,
 												 const uint16_t* __restrict__ order, float* __restrict__ wt,
 												 float* __restrict__ inv_sum, void* __restrict__ w_hi_v,
@@ -46,9 +45,10 @@ __device__ __forceinline__ void blend_weight_row(const GridConst& g, const float
 			const float4 d = dirs[order ? order[i] : i];
 			w = fmaxf(0.0f, tx * d.x + ty * d.y + tz * d.z);
 			if(dep_row) w = powf(w, g.sharpness);
-			if(mode == kBlendFP32) wt[size_t(i) * kBlendTexels + t] = w;
 		}
-		if(mode == kBlendBF16) {
+		if(mode == kBlendFP32) {
+			wt[size_t(row) * n_pad + i] = w;
+		} else if(mode == kBlendBF16) {
 			// compensated split: w = hi + lo, both bf16; normalise by the sum of what the MFMA
 			// multiplies, so the rounding of the weights cancels in the normalisation
 			const __bf16 hi = __bf16(w);
@@ -128,34 +128,74 @@ __device__ __forceinline__ int border_targets(int x, int y, int dx[3], int dy[3]
 	return k;
 }
 
-// Epilogue shared by the FP32 and MFMA blends: normalise, hysteresis against the exposed atlas,
-// store interior + mirrored border texels into the next atlas, collect maxChange.
-__device__ __forceinline__ void store_texel(const BlendArgs& a, uint32_t t, uint32_t probe, const float acc[3],
-											float inv, float* max_change) {
+// Epilogue shared by the blends, for the block's np probes (atlas indices in s_idx): normalise the
+// sums the K loop left in LDS ([48][ci] irradiance rows, [208][cd] depth rows; probe p's channels in
+// columns 3p + c and 2p + c), apply hysteresis against the exposed atlas, store interior + mirrored
+// border texels into the next atlas and collect maxChange per probe in s_maxc.  A thread takes four
+// items at a time and issues their four exposed-atlas loads before any of their stores, so the
+// load latency is paid once per four items.
+__device__ __forceinline__ void blend_epilogue(const BlendArgs& a, const uint32_t* s_idx, uint32_t np,
+											   const float* s_out_irr, int ci, const float* s_out_dep, int cd,
+											   uint32_t* s_maxc, bool irr_only) {
+	constexpr uint32_t kU = 4;
 	const float h = a.grid.hysteresis;
-	if(t < kIrrTexels) {
-		const int x = int(t % 6u) + 1, y = int(t / 6u) + 1;
-		const float4 prev = a.irr_prev[size_t(probe) * 64 + y * 8 + x];
-		const float r0 = acc[0] * inv, r1 = acc[1] * inv, r2 = acc[2] * inv;
-		*max_change = fmaxf(fmaxf(fabsf(r0 - prev.x), fabsf(r1 - prev.y)), fabsf(r2 - prev.z));
-		const float4 v = make_float4(r0 * (1.0f - h) + prev.x * h, r1 * (1.0f - h) + prev.y * h,
-									 r2 * (1.0f - h) + prev.z * h, 1.0f);
-		float4* tile = a.irr_out + size_t(probe) * 64;
-		tile[y * 8 + x] = v;
-		int dx[3], dy[3];
-		const int nb = border_targets<8>(x, y, dx, dy);
-		for(int k = 0; k < nb; ++k) tile[dy[k] * 8 + dx[k]] = v;
-	} else {
-		const uint32_t d = t - kIrrTexels;
-		const int x = int(d % 14u) + 1, y = int(d / 14u) + 1;
-		const float2 prev = a.dep_prev[size_t(probe) * 256 + y * 16 + x];
-		const float r0 = acc[0] * inv, r1 = acc[1] * inv;
-		const float2 v = make_float2(r0 * (1.0f - h) + prev.x * h, r1 * (1.0f - h) + prev.y * h);
-		float2* tile = a.dep_out + size_t(probe) * 256;
-		tile[y * 16 + x] = v;
-		int dx[3], dy[3];
-		const int nb = border_targets<16>(x, y, dx, dy);
-		for(int k = 0; k < nb; ++k) tile[dy[k] * 16 + dx[k]] = v;
+	const float4* __restrict__ irr_prev = a.irr_prev;
+	const float2* __restrict__ dep_prev = a.dep_prev;
+	float4* __restrict__ irr_out = a.irr_out;
+	float2* __restrict__ dep_out = a.dep_out;
+	const uint32_t ni = np * kIrrTexels;
+	for(uint32_t e0 = threadIdx.x; e0 < ni; e0 += 256u * kU) {
+		float4 prev[kU];
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u) {
+			const uint32_t e = e0 + 256u * u, p = e / kIrrTexels, tex = e % kIrrTexels;
+			if(e < ni) prev[u] = irr_prev[size_t(s_idx[p]) * 64 + (tex / 6u + 1u) * 8 + tex % 6u + 1u];
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u) {
+			const uint32_t e = e0 + 256u * u, p = e / kIrrTexels, tex = e % kIrrTexels;
+			if(e >= ni) break;
+			const int x = int(tex % 6u) + 1, y = int(tex / 6u) + 1;
+			const float inv = a.inv_sum[tex];
+			const float* sum = s_out_irr + tex * ci + p * 3;
+			const float r0 = sum[0] * inv, r1 = sum[1] * inv, r2 = sum[2] * inv;
+			const float4 pv = prev[u];
+			const float mc = fmaxf(fmaxf(fabsf(r0 - pv.x), fabsf(r1 - pv.y)), fabsf(r2 - pv.z));
+			const float4 v = make_float4(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h,
+										 r2 * (1.0f - h) + pv.z * h, 1.0f);
+			float4* tile = irr_out + size_t(s_idx[p]) * 64;
+			tile[y * 8 + x] = v;
+			int dx[3], dy[3];
+			const int nb = border_targets<8>(x, y, dx, dy);
+			for(int k = 0; k < nb; ++k) tile[dy[k] * 8 + dx[k]] = v;
+			atomicMax(&s_maxc[p], __float_as_uint(mc));
+		}
+	}
+	if(irr_only) return;
+	const uint32_t nd = np * kDepTexels;
+	for(uint32_t e0 = threadIdx.x; e0 < nd; e0 += 256u * kU) {
+		float2 prev[kU];
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u) {
+			const uint32_t e = e0 + 256u * u, p = e / kDepTexels, tex = e % kDepTexels;
+			if(e < nd) prev[u] = dep_prev[size_t(s_idx[p]) * 256 + (tex / 14u + 1u) * 16 + tex % 14u + 1u];
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u) {
+			const uint32_t e = e0 + 256u * u, p = e / kDepTexels, tex = e % kDepTexels;
+			if(e >= nd) break;
+			const int x = int(tex % 14u) + 1, y = int(tex / 14u) + 1;
+			const float inv = a.inv_sum[kIrrTexels + tex];
+			const float* sum = s_out_dep + tex * cd + p * 2;
+			const float r0 = sum[0] * inv, r1 = sum[1] * inv;
+			const float2 pv = prev[u];
+			const float2 v = make_float2(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h);
+			float2* tile = dep_out + size_t(s_idx[p]) * 256;
+			tile[y * 16 + x] = v;
+			int dx[3], dy[3];
+			const int nb = border_targets<16>(x, y, dx, dy);
+			for(int k = 0; k < nb; ++k) tile[dy[k] * 16 + dx[k]] = v;
+		}
 	}
 }
 
@@ -173,105 +213,150 @@ __device__ __forceinline__ void update_state(uint32_t* states, uint32_t probe, f
 	states[probe] = st;
 }
 
-// FP32 VALU blend: a block owns NP probe slots from slot0; thread t owns texel t for all NP probes
-// and accumulates 3 channels (rgb for irradiance, (d, d^2, 0) for depth) over ray chunks staged in
-// LDS.  Each probe's sums run over the rays in order, so the result does not depend on NP.
+// FP32 blend on the matrix cores (v_mfma_f32_16x16x4_f32, FP32 operands and accumulation).  A
+// block owns NP probe slots from slot0.  Output tiles of 16 x 16: M = the 16 padded texel row tiles
+// of the weight matrix (3 irradiance, 13 depth; wave w takes tiles w, w+4, w+8, w+12), N = the
+// block's data columns (irradiance: 3 per probe, depth: (d, d^2) per probe, each padded to a
+// multiple of 16), K = rays, staged in LDS in chunks of 64 as [column][ray].  A group of 16 rays
+// k0 .. k0+15 takes 4 MFMAs: lane l reads the float4 of rays k0 + 4 (l / 16) .. +3 of its weight row
+// and of its data column, and step s multiplies component s, so instruction K index l / 16 is ray
+// k0 + 4 (l / 16) + s for both operands.  Every output element sees the same sequence of MFMAs for
+// any NP, so the result does not depend on the tile size.
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaChunk = 64;              // rays staged per LDS chunk
+constexpr int kWRows = 256;                 // padded weight rows: irr 0..47, depth 48..255
+constexpr int kDepRow0 = 48;
+
 template <int NP>
-__device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slot0) {
-	__shared__ float4 s_irr[NP][kRayChunk];
-	__shared__ float2 s_dep[NP][kRayChunk];
+__device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slot0, uint32_t n_pad) {
+	constexpr int kNI = (3 * NP + 15) / 16, kND = (2 * NP + 15) / 16;  // N tiles
+	constexpr int kCI = 16 * kNI, kCD = 16 * kND;                       // padded columns
+	constexpr int kStride = kMfmaChunk + 4;  // floats per staged column: 16-byte skew between columns
+	constexpr int kStageFloats = (kCI + kCD) * kStride;
+	constexpr int kOutFloats = 48 * kCI + 208 * kCD;
+	__shared__ __attribute__((aligned(16))) float smem[kStageFloats > kOutFloats ? kStageFloats : kOutFloats];
 	__shared__ uint32_t s_oor[NP];
 	__shared__ uint32_t s_maxc[NP];
+	__shared__ uint32_t s_idx[NP];
+	float* sb = smem;                    // [column][kStride]: irradiance 3p + c, then depth kCI + 2p + c
+	float* s_out_irr = smem;             // after the K loop: [48][kCI]
+	float* s_out_dep = smem + 48 * kCI;  //                   [208][kCD]
+
 	// count and n steer loops that contain barriers: make them provably uniform (count is read
 	// through a pointer the compiler cannot prove to be global, which it would treat as divergent)
 	const uint32_t count = __builtin_amdgcn_readfirstlane(*a.count);
 	if(slot0 >= count) return;
 	const uint32_t np = min(uint32_t(NP), count - slot0);
 	const uint32_t n = __builtin_amdgcn_readfirstlane(a.grid.n);
-	const uint32_t t = threadIdx.x;
+	n_pad = __builtin_amdgcn_readfirstlane(n_pad);
 	const float cell_len = a.grid.cell_len;
+	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+	const int lrow = lane & 15, lgrp = lane >> 4;
 	if(t < NP) {
 		s_oor[t] = 0;
 		s_maxc[t] = 0;
+		s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
 	}
-	float acc[NP][3];
+	floatx4_t acc[4][kNI];
 #pragma unroll
-	for(int p = 0; p < NP; ++p) acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
-	const bool is_depth = t >= kIrrTexels;
-	const bool active = t < kBlendTexels;
-	for(uint32_t i0 = 0; i0 < n; i0 += kRayChunk) {
-		const uint32_t iend = min(uint32_t(kRayChunk), n - i0);
-		// this chunk's weights of texel t, fetched before the barrier so that the loads overlap the
-		// staging instead of stalling each step of the accumulation
-		float w[kRayChunk];
+	for(int q = 0; q < 4; ++q)
 #pragma unroll
-		for(int i = 0; i < int(kRayChunk); ++i)
-			w[i] = (active && uint32_t(i) < iend) ? a.wt[size_t(i0 + i) * kBlendTexels + t] : 0.0f;
+		for(int c = 0; c < kNI; ++c) acc[q][c] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+
+	for(uint32_t k0 = 0; k0 < n_pad; k0 += kMfmaChunk) {
 		__syncthreads();
-		// stage NP probes x 32 rays: (r, g, b) for irradiance, (d, d^2) for depth
-		for(uint32_t e = t; e < NP * kRayChunk; e += kBlendThreads) {
-			const uint32_t p = e / kRayChunk, i = e % kRayChunk;
-			float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-			float2 dd = make_float2(0.0f, 0.0f);
-			if(p < np && i0 + i < n) {
-				r = a.rays[size_t(slot0 + p) * n + i0 + i];
-				float dc = fminf(cell_len, r.w);
-				if(dc < 0.0f) dc = cell_len;
-				dd = make_float2(dc, dc * dc);
-				if(r.w < 0.0f || r.w > cell_len) atomicAdd(&s_oor[p], 1u);
-				r.w = 0.0f;
+		// staging: 16 consecutive threads per probe, 4 consecutive rays each
+		for(int e = t; e < NP * 16; e += 256) {
+			const int sp = e >> 4, sk = (e & 15) * 4;
+			float v[5][4];
+			uint32_t oor = 0;
+#pragma unroll
+			for(int q = 0; q < 4; ++q) {
+				const uint32_t k = k0 + sk + q;
+				float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				float dc = 0.0f;
+				if(uint32_t(sp) < np && k < n) {
+					r = a.rays[size_t(slot0 + sp) * n + k];
+					if(r.w < 0.0f || r.w > cell_len) ++oor;
+					dc = fminf(cell_len, r.w);
+					if(dc < 0.0f) dc = cell_len;
+				}
+				v[0][q] = r.x;
+				v[1][q] = r.y;
+				v[2][q] = r.z;
+				v[3][q] = dc;
+				v[4][q] = dc * dc;
 			}
-			s_irr[p][i] = r;
-			s_dep[p][i] = dd;
+			const int col[5] = {sp * 3, sp * 3 + 1, sp * 3 + 2, kCI + sp * 2, kCI + sp * 2 + 1};
+#pragma unroll
+			for(int c = 0; c < 5; ++c)
+				*reinterpret_cast<float4*>(sb + col[c] * kStride + sk) = make_float4(v[c][0], v[c][1], v[c][2], v[c][3]);
+			for(int off = 8; off > 0; off >>= 1) oor += __shfl_xor(oor, off, 16);
+			if((e & 15) == 0 && oor != 0u) atomicAdd(&s_oor[sp], oor);
 		}
 		__syncthreads();
-		// rays in order for every (texel, probe, channel): the sums do not depend on NP.  Waves 1..3
-		// hold depth texels only and take the two-channel path as a whole.
-		if(active && is_depth) {
 #pragma unroll
-			for(int i = 0; i < int(kRayChunk); ++i) {
-				if(uint32_t(i) < iend) {
+		for(int kb = 0; kb < kMfmaChunk; kb += 16) {
+			const int kk = kb + 4 * lgrp;
+			// the data fragments serve all of the wave's row tiles
+			float4 bi[kNI], bd[kND];
+			if(wave < 3) {
 #pragma unroll
-					for(int p = 0; p < NP; ++p) {
-						const float2 x = s_dep[p][i];
-						acc[p][0] += w[i] * x.x;
-						acc[p][1] += w[i] * x.y;
+				for(int c = 0; c < kNI; ++c) bi[c] = *reinterpret_cast<const float4*>(sb + (c * 16 + lrow) * kStride + kk);
+			}
+#pragma unroll
+			for(int c = 0; c < kND; ++c) bd[c] = *reinterpret_cast<const float4*>(sb + (kCI + c * 16 + lrow) * kStride + kk);
+#pragma unroll
+			for(int q = 0; q < 4; ++q) {
+				const int mt = wave + 4 * q;  // 0..2 irradiance, 3..15 depth
+				if(mt < 3) {
+					const float4 w = *reinterpret_cast<const float4*>(a.wt + size_t(mt * 16 + lrow) * n_pad + k0 + kk);
+#pragma unroll
+					for(int c = 0; c < kNI; ++c) {
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, bi[c].x, acc[q][c], 0, 0, 0);
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, bi[c].y, acc[q][c], 0, 0, 0);
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.z, bi[c].z, acc[q][c], 0, 0, 0);
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.w, bi[c].w, acc[q][c], 0, 0, 0);
+					}
+				} else {
+					const float4 w = *reinterpret_cast<const float4*>(
+						a.wt + size_t(kDepRow0 + (mt - 3) * 16 + lrow) * n_pad + k0 + kk);
+#pragma unroll
+					for(int c = 0; c < kND; ++c) {
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, bd[c].x, acc[q][c], 0, 0, 0);
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, bd[c].y, acc[q][c], 0, 0, 0);
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.z, bd[c].z, acc[q][c], 0, 0, 0);
+						acc[q][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.w, bd[c].w, acc[q][c], 0, 0, 0);
 					}
 				}
 			}
-		} else if(active) {
-#pragma unroll
-			for(int i = 0; i < int(kRayChunk); ++i) {
-				if(uint32_t(i) < iend) {
-#pragma unroll
-					for(int p = 0; p < NP; ++p) {
-						const float4 x = s_irr[p][i];
-						acc[p][0] += w[i] * x.x;
-						acc[p][1] += w[i] * x.y;
-						acc[p][2] += w[i] * x.z;
-					}
-				}
-			}
 		}
 	}
-	const bool irr_only = (a.flags & kBlendIrradianceOnly) != 0u;
-	if(active && !(is_depth && irr_only)) {
-		const float inv = a.inv_sum[t];
+	__syncthreads();  // staging done: reuse LDS for the accumulators (lane l, register r = D[4 (l/16) + r][l % 16])
 #pragma unroll
-		for(int p = 0; p < NP; ++p) {
-			if(uint32_t(p) < np) {
-				float mc = 0.0f;
-				store_texel(a, t, a.indices[slot0 + p], acc[p], inv, &mc);
-				if(!is_depth) atomicMax(&s_maxc[p], __float_as_uint(mc));
+	for(int q = 0; q < 4; ++q) {
+		const int mt = wave + 4 * q;
+#pragma unroll
+		for(int c = 0; c < kNI; ++c) {
+#pragma unroll
+			for(int r = 0; r < 4; ++r) {
+				const int row = 4 * lgrp + r;
+				if(mt < 3) s_out_irr[(mt * 16 + row) * kCI + c * 16 + lrow] = acc[q][c][r];
+				else if(c < kND) s_out_dep[((mt - 3) * 16 + row) * kCD + c * 16 + lrow] = acc[q][c][r];
 			}
 		}
 	}
 	__syncthreads();
-	if(t < np && !irr_only) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
+	const bool irr_only = (a.flags & kBlendIrradianceOnly) != 0u;
+	blend_epilogue(a, s_idx, np, s_out_irr, kCI, s_out_dep, kCD, s_maxc, irr_only);
+	__syncthreads();
+	if(uint32_t(t) < np && !irr_only) update_state(a.states, s_idx[t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
-__global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
-	blend_fp32_tile<kProbesPerBlock>(a, blockIdx.x * kProbesPerBlock);
+// 4 waves/SIMD: the 4 blocks of a CU that a 16-probe tiling of a large grid needs stay resident
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_blend_fp32(BlendArgs a, uint32_t n_pad) {
+	blend_fp32_tile<kProbesPerBlock>(a, blockIdx.x * kProbesPerBlock, n_pad);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -290,13 +375,9 @@ __global__ __launch_bounds__(kBlendThreads) void k_blend_fp32(BlendArgs a) {
 // Accumulator layout (verified for gfx950): lane l, register r holds D[4*(l/16)+r][l%16].
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
-typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kMfmaChunk = 64;              // rays staged per LDS chunk
 constexpr int kBStride = kMfmaChunk + 8;    // 16-bit elements per LDS column (padded)
 constexpr int kIrrCols = kProbesPerBlock * 3, kDepCols = kProbesPerBlock * 2;
-constexpr int kWRows = 256;                 // padded weight rows: irr 0..47, depth 48..255
-constexpr int kDepRow0 = 48;
 
 template <uint32_t MODE>
 struct MfmaTypes;
@@ -327,6 +408,7 @@ __device__ __forceinline__ void blend_mfma_tile(const BlendArgs& a, uint32_t slo
 	__shared__ __attribute__((aligned(16))) unsigned char smem[kStageBytes > kOutBytes ? kStageBytes : kOutBytes];
 	__shared__ uint32_t s_oor[kProbesPerBlock];
 	__shared__ uint32_t s_maxc[kProbesPerBlock];
+	__shared__ uint32_t s_idx[kProbesPerBlock];
 	T* sb = reinterpret_cast<T*>(smem);  // [part][col][kBStride], irr cols first then depth cols
 	float* s_out_irr = reinterpret_cast<float*>(smem);          // [48][48]
 	float* s_out_dep = s_out_irr + 48 * kIrrCols;               // [208][32]
@@ -420,30 +502,18 @@ __device__ __forceinline__ void blend_mfma_tile(const BlendArgs& a, uint32_t slo
 	// out-of-range counts: the 16 stagers of a probe are 16 consecutive lanes of one wave
 	for(int off = 8; off > 0; off >>= 1) my_oor += __shfl_xor(my_oor, off, 16);
 	if((t & 15) == 0) s_oor[sp] = my_oor;
-	if(t < kProbesPerBlock) s_maxc[t] = 0;
-	__syncthreads();
-	// epilogue: (probe, texel) pairs
-	for(uint32_t e = t; e < np * kBlendTexels; e += 256) {
-		const uint32_t p = e / kBlendTexels, tex = e % kBlendTexels;
-		float v[3];
-		if(tex < kIrrTexels) {
-			for(int c = 0; c < 3; ++c) v[c] = s_out_irr[tex * kIrrCols + p * 3 + c];
-		} else {
-			const uint32_t d = tex - kIrrTexels;
-			v[0] = s_out_dep[d * kDepCols + p * 2];
-			v[1] = s_out_dep[d * kDepCols + p * 2 + 1];
-			v[2] = 0.0f;
-		}
-		float mc = 0.0f;
-		store_texel(a, tex, a.indices[slot0 + p], v, a.inv_sum[tex], &mc);
-		if(tex < kIrrTexels) atomicMax(&s_maxc[p], __float_as_uint(mc));
+	if(t < kProbesPerBlock) {
+		s_maxc[t] = 0;
+		s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
 	}
 	__syncthreads();
-	if(uint32_t(t) < np) update_state(a.states, a.indices[slot0 + t], __uint_as_float(s_maxc[t]), s_oor[t], n);
+	blend_epilogue(a, s_idx, np, s_out_irr, kIrrCols, s_out_dep, kDepCols, s_maxc, false);
+	__syncthreads();
+	if(uint32_t(t) < np) update_state(a.states, s_idx[t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) void k_blend_mfma(BlendArgs a, uint32_t n_pad) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_blend_mfma(BlendArgs a, uint32_t n_pad) {
 	blend_mfma_tile<MODE>(a, blockIdx.x * kProbesPerBlock, n_pad);
 }
 
@@ -477,7 +547,7 @@ __global__ __launch_bounds__(256) void k_batch_blend(BatchLaunch b) {
 		const uint32_t vol = __builtin_amdgcn_readfirstlane(b.blend_items[it].volume);
 		const uint32_t first = __builtin_amdgcn_readfirstlane(b.blend_items[it].first);
 		const BatchVolume& v = b.vols[vol];
-		if constexpr(MODE == kBlendFP32) blend_fp32_tile<NP>(v.blend, first);
+		if constexpr(MODE == kBlendFP32) blend_fp32_tile<NP>(v.blend, first, v.n_pad);
 		else blend_mfma_tile<MODE>(v.blend, first, v.n_pad);
 		prev = vol;
 		__syncthreads();
@@ -512,7 +582,7 @@ hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s) {
 	const uint32_t n_pad = (a.grid.n + kMfmaChunk - 1) & ~uint32_t(kMfmaChunk - 1);
 	if(mode == kBlendBF16) hipLaunchKernelGGL(k_blend_mfma<kBlendBF16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
 	else if(mode == kBlendFP16) hipLaunchKernelGGL(k_blend_mfma<kBlendFP16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
-	else hipLaunchKernelGGL(k_blend_fp32, dim3(blocks), dim3(kBlendThreads), 0, s, a);
+	else hipLaunchKernelGGL(k_blend_fp32, dim3(blocks), dim3(256), 0, s, a, n_pad);
 	return hipGetLastError();
 }
 

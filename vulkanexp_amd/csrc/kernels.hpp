@@ -15,6 +15,7 @@ constexpr uint32_t kMaxRays = 512;
 constexpr uint32_t kIrrTexels = 36;    // 6x6 interior of the 8x8 tile
 constexpr uint32_t kDepTexels = 196;   // 14x14 interior of the 16x16 tile
 constexpr uint32_t kBlendTexels = kIrrTexels + kDepTexels;
+constexpr uint32_t kWeightRows = 256;  // blend weight rows: 36 irradiance + 12 zero + 196 depth + 12 zero
 constexpr uint32_t kTraceBlock = 128;  // threads per traversal block
 
 // Grid constants derived on the host (FP32, same operations as the oracle's ray generation).
@@ -105,7 +106,7 @@ struct BlendArgs {
 	const uint32_t* indices;
 	const uint32_t* count;
 	const float4* rays;         // slot-major (rgb, depth)
-	const float* wt;            // [n][232] weights, irradiance texels then depth texels
+	const float* wt;            // FP32 weights [kWeightRows][n_pad]: irradiance rows 0..35, depth rows 48..243
 	const float* inv_sum;       // [232]
 	const void* w_lo16;         // MFMA paths: weights split hi/lo in 16-bit types
 	const void* w_hi16;
