@@ -245,8 +245,9 @@ void destroy_handle(Handle* h) {
 		(void)hipFree(h->irr[i]);
 		(void)hipFree(h->dep[i]);
 	}
-	void* bufs[] = {h->states, h->indices, h->sel,    h->rays, h->hits,  h->shade_list, h->shade_count, h->miss_list, h->fib,
-					h->perm,   h->fib512,  h->dirs,   h->wt,   h->inv_sum, h->w_hi,      h->w_lo,        h->visit};
+	void* bufs[] = {h->states, h->indices, h->sel,     h->rays,         h->hits,       h->shade_list, h->shade_count,
+					h->miss_list, h->shade_keys, h->sort_keys, h->sort_list, h->sort_counts, h->fib,   h->perm,       h->fib512,
+					h->dirs,   h->wt,      h->inv_sum,    h->w_hi,        h->w_lo,       h->visit};
 	for(void* b : bufs) (void)hipFree(b);
 	for(auto& e : h->ev)
 		if(e) (void)hipEventDestroy(e);
@@ -387,6 +388,10 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->hits, P * R * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->shade_list, P * R * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->shade_count, 16);
+	if(e == hipSuccess) e = dalloc(&h->shade_keys, P * R * sizeof(uint32_t));
+	if(e == hipSuccess) e = dalloc(&h->sort_keys, P * R * sizeof(uint32_t));
+	if(e == hipSuccess) e = dalloc(&h->sort_list, P * R * sizeof(uint32_t));
+	if(e == hipSuccess) e = dalloc(&h->sort_counts, size_t(kShadeSortCounts) * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->fib, kMaxRays * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->perm, kMaxRays * sizeof(uint16_t));
 	if(e == hipSuccess) e = dalloc(&h->fib512, 512 * sizeof(float4));
@@ -713,6 +718,11 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 		advance_hysteresis(h, cnt);
 	}
 	const GridConst g = make_grid_const(*h);
+	// shade-list ordering keys: cell coordinates below 64 per axis, Morton-interleaved
+	const uint32_t max_cell = uint32_t(std::max({g.res[0], g.res[1], g.res[2]}) - 1);
+	uint32_t key_shift = 0, key_bits = 0;
+	while((max_cell >> key_shift) >= 64u) ++key_shift;
+	while(((max_cell >> key_shift) >> key_bits) != 0u) ++key_bits;
 	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s, h->shade_count));
 	TraceFrameArgs a{};
 	a.scene = h->sd;
@@ -737,8 +747,12 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 		HIP_TRY(h, hipMemsetAsync(h->visit, 0, 8 * sizeof(unsigned long long), s));
 		a.visit_counters = h->visit;
 	}
+	a.shade_keys = h->shade_keys;
+	a.key_shift = key_shift;
 	HIP_TRY(h, launch_trace_primary(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
+	// shade the front-face hits ordered by the grid cell they lie in
+	HIP_TRY(h, launch_shade_order(a, 3 * key_bits, h->sort_counts, h->sort_keys, h->sort_list, s));
 	HIP_TRY(h, launch_shade(a, s));
 	if(h->light.sky_model == DDGI_SKY_ATMOSPHERE) HIP_TRY(h, launch_sky(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));

@@ -39,6 +39,10 @@ struct Handle {
 	uint32_t* shade_list = nullptr;   // P * max_rays entries
 	uint32_t* shade_count = nullptr;  // [0] shade list, [1] miss list
 	uint32_t* miss_list = nullptr;    // P * max_rays entries, allocated with the atmospheric sky
+	uint32_t* shade_keys = nullptr;   // P * max_rays: cell key of each shade-list entry
+	uint32_t* sort_keys = nullptr;    // P * max_rays: keys / entries bucketed by coarse digit
+	uint32_t* sort_list = nullptr;
+	uint32_t* sort_counts = nullptr;  // kShadeSortCounts
 	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
 	uint16_t* perm = nullptr;   // kMaxRays
 	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512

@@ -92,7 +92,12 @@ struct TraceFrameArgs {
 	// optional (differentiable prepare): 4 float4 per ray, rad = c0 + A a + (B a + C) L:
 	// (c0, material bits), (A, lit bits), (B, 0), (C, 0); material 0xFFFFFFFF for misses/back faces
 	float4* factors;
+	// optional (shade-list ordering): the primary pass stores the cell key of each shade-list entry
+	// (cell coordinates shifted right by key_shift, so that they stay below 64 and keys below 2^18)
+	uint32_t* shade_keys;
+	uint32_t key_shift;
 };
+constexpr uint32_t kShadeSortCounts = 1024 * 128 + 1024;  // (list tile, coarse digit) counters + digit totals
 
 struct SelectState {
 	uint32_t last_offset;
@@ -167,6 +172,11 @@ struct ReflectionArgs {
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);
+// Orders the frame's shade list (a.shade_list with keys a.shade_keys of key_bits <= 18 bits, length
+// a.shade_count[0]) by key, in place.  Scratch: `counts` (kShadeSortCounts), `bucket_keys` and
+// `bucket_list` (the list's capacity each).
+hipError_t launch_shade_order(const TraceFrameArgs& a, uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys,
+							  uint32_t* bucket_list, hipStream_t s);
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s);
 // Atmospheric-sky radiance of the frame's misses (the miss list).

@@ -59,16 +59,36 @@ __device__ __forceinline__ void write_unshaded(const TraceFrameArgs& a, size_t r
 	}
 }
 
+// Shade-list ordering key: the Morton code of the cell of the probe grid that holds p (coordinates
+// clamped to the grid and shifted right by `shift`; a NaN coordinate counts as 0).
+__device__ __forceinline__ uint32_t cell_key(const GridConst& g, float px, float py, float pz, uint32_t shift) {
+	const float p[3] = {px, py, pz};
+	uint32_t key = 0;
+#pragma unroll
+	for(int ax = 0; ax < 3; ++ax) {
+		float f = floorf((p[ax] - g.emin[ax]) / g.cell[ax]);
+		f = fminf(fmaxf(f, 0.0f), float(g.res[ax] - 1));
+		const uint32_t c = uint32_t(f) >> shift;
+#pragma unroll
+		for(int b = 0; b < 6; ++b) key |= ((c >> b) & 1u) << (3 * b + ax);
+	}
+	return key;
+}
+
 // ---- primary probe rays: closest hit, mask STATIC|DYNAMIC, tmin 0.01, tmax = |max - min| ----
 // Result of the ray at position ri (= slot * n + k, thread order): its hit record, and either its
 // final ray record (miss, back face) or a shade-list entry (front-face hit).
-__device__ __forceinline__ void finish_primary_ray(const TraceFrameArgs& a, uint32_t ri, const HitResult& h) {
+__device__ __forceinline__ void finish_primary_ray(const TraceFrameArgs& a, uint32_t ri, const HitResult& h,
+												   uint32_t key) {
 	const float4 packed = pack_hit(h);
 	a.hits[ri] = packed;
 	const uint32_t code = __float_as_uint(packed.y);
 	const bool front = code != 0xFFFFFFFFu && !(code & 0x80000000u);
 	const uint32_t pos = wave_append(a.shade_count, front);  // front-face hits are queued for shading
-	if(front) a.shade_list[pos] = ri;
+	if(front) {
+		a.shade_list[pos] = ri;
+		if(a.shade_keys) a.shade_keys[pos] = key;
+	}
 	const bool miss = code == 0xFFFFFFFFu;
 	if(a.light.sky_model == DDGI_SKY_ATMOSPHERE) {  // misses are queued for the sky kernel
 		const uint32_t mpos = wave_append(a.shade_count + 1, miss);
@@ -95,7 +115,9 @@ __device__ __forceinline__ void trace_primary_ray(const TraceFrameArgs& a, uint3
 	uint32_t nn = 0, nt = 0;
 	const HitResult h = traverse<false, STACK, COUNT>(a.scene.nodes, a.scene.tris, r, 0.01f, a.grid.tmax_primary,
 													  DDGI_MASK_STATIC | DDGI_MASK_DYNAMIC, stack, stride, nn, nt);
-	finish_primary_ray(a, slot * n + j, h);  // thread order: coalesced stores
+	const uint32_t key = a.shade_keys ? cell_key(a.grid, o[0] + h.t * d.x, o[1] + h.t * d.y, o[2] + h.t * d.z, a.key_shift)
+									  : 0u;
+	finish_primary_ray(a, slot * n + j, h, key);  // thread order: coalesced stores
 	if(COUNT) {
 		atomicAdd(&a.visit_counters[0], (unsigned long long)nn);
 		atomicAdd(&a.visit_counters[1], (unsigned long long)nt);
@@ -198,6 +220,212 @@ __device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const 
 	r.mat = mat;
 	r.lit = lit;
 	return r;
+}
+
+// ---- shade-list ordering ----
+// The primary pass appends front-face hits in ray order, so the hits one shading wave takes lie
+// wherever one probe's rays ended.  Sorting the list by the Morton code of the probe-grid cell
+// holding the hit point (cell_key) gives consecutive waves hits that share cage probes (the same
+// atlas tiles) and cast shadow rays from nearby origins along the same direction (the same BVH
+// nodes), which the caches then serve.  Each entry's shading depends on its own record only, so
+// the order changes no output, and entries with equal keys may come in any order.
+//
+// Keys of up to 18 bits = a coarse digit (the high 10 bits) and a fine digit (the rest):
+//  1. k_sort_count: kSortTiles workgroups, one per contiguous tile of the list, count the coarse
+//     digits of their tile in LDS and store them as counts[tile][digit];
+//  2. k_sort_columns: per digit, the exclusive prefix sum over the tiles in place, and the digit's
+//     total;
+//  3. k_sort_bucket: every tile moves its entries into their coarse buckets: a run starts at the
+//     sum of the totals of the smaller digits plus the column prefix (positions inside a run from
+//     LDS counters);
+//  4. k_sort_fine: one workgroup per coarse bucket orders it by the fine digit, back into the list.
+// Every position comes from a prefix sum or an LDS counter: no global atomics, whose contention
+// on a few thousand hot counters costs more than the whole sort.
+constexpr uint32_t kSortTiles = 128, kSortDigits = 1024, kSortCoarseBits = 10, kSortUnroll = 8;
+static_assert(kSortTiles * kSortDigits + kSortDigits == kShadeSortCounts && kSortDigits == 1u << kSortCoarseBits &&
+				  kSortTiles % 4u == 0u,
+			  "sort counters");
+
+// Exclusive prefix sum over a workgroup of 1024 threads (16 waves): wave scans by lane shuffles,
+// then the wave totals through LDS.  Returns this thread's prefix; `total` gets the sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+	const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+	uint32_t inc = v;
+#pragma unroll
+	for(int off = 1; off < 64; off <<= 1) {
+		const uint32_t o = uint32_t(__shfl_up(int(inc), off));
+		if(lane >= uint32_t(off)) inc += o;
+	}
+	if(lane == 63u) s_wave[wave] = inc;
+	__syncthreads();
+	uint32_t before = 0;
+	total = 0;
+#pragma unroll
+	for(uint32_t w = 0; w < 16u; ++w) {
+		const uint32_t x = s_wave[w];
+		before += w < wave ? x : 0u;
+		total += x;
+	}
+	__syncthreads();  // s_wave may be reused after this
+	return before + inc - v;
+}
+
+// Tile b of the list: entries [lo, hi).
+__device__ __forceinline__ void sort_tile(uint32_t count, uint32_t b, uint32_t& lo, uint32_t& hi) {
+	const uint32_t per = (count + kSortTiles - 1u) / kSortTiles;
+	lo = min(count, b * per);
+	hi = min(count, lo + per);
+}
+
+__global__ __launch_bounds__(1024) void k_sort_count(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count_ptr,
+													 uint32_t fine_bits, uint32_t* __restrict__ counts) {
+	__shared__ uint32_t s_cnt[kSortDigits];
+	const uint32_t t = threadIdx.x;
+	s_cnt[t] = 0u;
+	__syncthreads();
+	uint32_t lo, hi;
+	sort_tile(__builtin_amdgcn_readfirstlane(*count_ptr), blockIdx.x, lo, hi);
+	for(uint32_t i0 = lo; i0 < hi; i0 += 1024u * kSortUnroll) {
+		uint32_t k[kSortUnroll];
+#pragma unroll
+		for(uint32_t u = 0; u < kSortUnroll; ++u) {
+			const uint32_t i = i0 + 1024u * u + t;
+			k[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < kSortUnroll; ++u)
+			if(k[u] != 0xFFFFFFFFu) atomicAdd(&s_cnt[k[u] >> fine_bits], 1u);
+	}
+	__syncthreads();
+	counts[blockIdx.x * kSortDigits + t] = s_cnt[t];
+}
+
+// Column prefix sums: workgroup g takes digits 64 g .. 64 g + 63 (one per lane); wave q of the
+// workgroup walks tiles 32 q .. 32 q + 31 of them (loads coalesced over the lanes, all in flight).
+__global__ __launch_bounds__(256) void k_sort_columns(uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
+	constexpr uint32_t kQ = kSortTiles / 4u;
+	__shared__ uint32_t s_part[4][64];
+	const uint32_t lane = __lane_id(), q = threadIdx.x >> 6;
+	const uint32_t d = blockIdx.x * 64u + lane;
+	uint32_t x[kQ];
+	uint32_t sum = 0;
+#pragma unroll
+	for(uint32_t b = 0; b < kQ; ++b) x[b] = counts[(q * kQ + b) * kSortDigits + d];
+#pragma unroll
+	for(uint32_t b = 0; b < kQ; ++b) sum += x[b];
+	s_part[q][lane] = sum;
+	__syncthreads();
+	uint32_t run = 0;
+	for(uint32_t w = 0; w < q; ++w) run += s_part[w][lane];
+	if(q == 3u) totals[d] = run + sum;
+#pragma unroll
+	for(uint32_t b = 0; b < kQ; ++b) {
+		counts[(q * kQ + b) * kSortDigits + d] = run;
+		run += x[b];
+	}
+}
+
+// First entry of coarse bucket d: the sum of the totals of the digits below d (workgroup-wide).
+__device__ __forceinline__ uint32_t bucket_start(const uint32_t* __restrict__ totals, uint32_t d, uint32_t* s_wave) {
+	uint32_t part = 0;
+	for(uint32_t i = threadIdx.x; i < d; i += blockDim.x) part += totals[i];
+#pragma unroll
+	for(int off = 32; off > 0; off >>= 1) part += uint32_t(__shfl_xor(int(part), off));
+	if(__lane_id() == 0u) s_wave[threadIdx.x >> 6] = part;
+	__syncthreads();
+	uint32_t start = 0;
+	for(uint32_t w = 0; w < (blockDim.x >> 6); ++w) start += s_wave[w];
+	__syncthreads();
+	return start;
+}
+
+__global__ __launch_bounds__(1024) void k_sort_bucket(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ list,
+													  const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
+													  const uint32_t* __restrict__ counts, const uint32_t* __restrict__ totals,
+													  uint32_t* __restrict__ bucket_keys, uint32_t* __restrict__ bucket_list) {
+	__shared__ uint32_t s_pos[kSortDigits];
+	__shared__ uint32_t s_wave[16];
+	const uint32_t t = threadIdx.x;
+	uint32_t all;
+	const uint32_t digit_start = block_exclusive_scan_1024(totals[t], s_wave, all);
+	s_pos[t] = digit_start + counts[blockIdx.x * kSortDigits + t];
+	__syncthreads();
+	uint32_t lo, hi;
+	sort_tile(__builtin_amdgcn_readfirstlane(*count_ptr), blockIdx.x, lo, hi);
+	for(uint32_t i0 = lo; i0 < hi; i0 += 1024u * kSortUnroll) {
+		uint32_t k[kSortUnroll], e[kSortUnroll];
+#pragma unroll
+		for(uint32_t u = 0; u < kSortUnroll; ++u) {
+			const uint32_t i = i0 + 1024u * u + t;
+			k[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+			e[u] = i < hi ? list[i] : 0u;
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < kSortUnroll; ++u) {
+			if(k[u] == 0xFFFFFFFFu) continue;
+			const uint32_t p = atomicAdd(&s_pos[k[u] >> fine_bits], 1u);
+			bucket_keys[p] = k[u];
+			bucket_list[p] = e[u];
+		}
+	}
+}
+
+// Workgroup d orders coarse bucket d by the fine digit (at most 256 fine values) into `out`.
+__global__ __launch_bounds__(256) void k_sort_fine(const uint32_t* __restrict__ bucket_keys,
+												   const uint32_t* __restrict__ bucket_list,
+												   const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
+												   const uint32_t* __restrict__ totals, uint32_t* __restrict__ out) {
+	constexpr uint32_t kU = 8;
+	__shared__ uint32_t s_cnt[256];
+	__shared__ uint32_t s_wave[4];
+	const uint32_t d = blockIdx.x, t = threadIdx.x;
+	const uint32_t count = __builtin_amdgcn_readfirstlane(*count_ptr);
+	const uint32_t size = totals[d];
+	if(size == 0u) return;  // uniform over the workgroup
+	const uint32_t start = min(count, bucket_start(totals, d, s_wave));
+	const uint32_t end = min(count, start + size);
+	const uint32_t mask = (1u << fine_bits) - 1u;
+	s_cnt[t] = 0u;
+	__syncthreads();
+	for(uint32_t i0 = start; i0 < end; i0 += 256u * kU) {
+		uint32_t k[kU];
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u) {
+			const uint32_t i = i0 + 256u * u + t;
+			k[u] = i < end ? bucket_keys[i] : 0xFFFFFFFFu;
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u)
+			if(k[u] != 0xFFFFFFFFu) atomicAdd(&s_cnt[k[u] & mask], 1u);
+	}
+	__syncthreads();
+	{  // exclusive scan of the 256 fine counts: wave scans, then the 4 wave totals
+		const uint32_t c = s_cnt[t], lane = __lane_id(), wave = t >> 6;
+		uint32_t inc = c;
+#pragma unroll
+		for(int off = 1; off < 64; off <<= 1) {
+			const uint32_t o = uint32_t(__shfl_up(int(inc), off));
+			if(lane >= uint32_t(off)) inc += o;
+		}
+		if(lane == 63u) s_wave[wave] = inc;
+		__syncthreads();
+		uint32_t before = 0;
+		for(uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+		s_cnt[t] = before + inc - c;
+	}
+	__syncthreads();
+	for(uint32_t i0 = start; i0 < end; i0 += 256u * kU) {
+		uint32_t k[kU], e[kU];
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u) {
+			const uint32_t i = i0 + 256u * u + t;
+			k[u] = i < end ? bucket_keys[i] : 0xFFFFFFFFu;
+			e[u] = i < end ? bucket_list[i] : 0u;
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < kU; ++u)
+			if(k[u] != 0xFFFFFFFFu) out[start + atomicAdd(&s_cnt[k[u] & mask], 1u)] = e[u];
+	}
 }
 
 // Entry k of the frame's shade list (the front-face hits queued by the primary traversal), so that
@@ -838,6 +1066,21 @@ hipError_t resident_blocks(const void* kernel, uint32_t block, uint64_t* out) {
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero) {
 	hipLaunchKernelGGL(k_frame_dirs, dim3(blocks_for(count, 256)), dim3(256), 0, s, R, fib, dirs, count, zero);
+	return hipGetLastError();
+}
+
+hipError_t launch_shade_order(const TraceFrameArgs& a, uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys,
+							  uint32_t* bucket_list, hipStream_t s) {
+	if(uint64_t(a.max_slots) * a.grid.n == 0) return hipSuccess;
+	if(key_bits > kSortCoarseBits + 8u) return hipErrorInvalidValue;
+	const uint32_t fine_bits = key_bits > kSortCoarseBits ? key_bits - kSortCoarseBits : 0u;
+	uint32_t* totals = counts + kSortTiles * kSortDigits;
+	hipLaunchKernelGGL(k_sort_count, dim3(kSortTiles), dim3(1024), 0, s, a.shade_keys, a.shade_count, fine_bits, counts);
+	hipLaunchKernelGGL(k_sort_columns, dim3(kSortDigits / 64u), dim3(256), 0, s, counts, totals);
+	hipLaunchKernelGGL(k_sort_bucket, dim3(kSortTiles), dim3(1024), 0, s, a.shade_keys, a.shade_list, a.shade_count,
+					   fine_bits, counts, totals, bucket_keys, bucket_list);
+	hipLaunchKernelGGL(k_sort_fine, dim3(kSortDigits), dim3(256), 0, s, bucket_keys, bucket_list, a.shade_count, fine_bits,
+					   totals, a.shade_list);
 	return hipGetLastError();
 }
 
