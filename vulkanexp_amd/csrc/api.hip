@@ -718,8 +718,8 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 		advance_hysteresis(h, cnt);
 	}
 	const GridConst g = make_grid_const(*h);
-	// shade-list ordering keys: cell coordinates below 64 per axis, Morton-interleaved
-	const uint32_t max_cell = uint32_t(std::max({g.res[0], g.res[1], g.res[2]}) - 1);
+	// shade-list ordering keys: half-cell coordinates below 64 per axis, Morton-interleaved
+	const uint32_t max_cell = uint32_t(2 * std::max({g.res[0], g.res[1], g.res[2]}) - 1);
 	uint32_t key_shift = 0, key_bits = 0;
 	while((max_cell >> key_shift) >= 64u) ++key_shift;
 	while(((max_cell >> key_shift) >> key_bits) != 0u) ++key_bits;

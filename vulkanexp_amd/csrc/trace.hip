@@ -59,15 +59,16 @@ __device__ __forceinline__ void write_unshaded(const TraceFrameArgs& a, size_t r
 	}
 }
 
-// Shade-list ordering key: the Morton code of the cell of the probe grid that holds p (coordinates
-// clamped to the grid and shifted right by `shift`; a NaN coordinate counts as 0).
+// Shade-list ordering key: the Morton code of the half-cell of the probe grid that holds p
+// (half-cell coordinates clamped to the grid and shifted right by `shift`; a NaN coordinate counts
+// as 0).  Half cells order the hits more finely than the cage cells they share.
 __device__ __forceinline__ uint32_t cell_key(const GridConst& g, float px, float py, float pz, uint32_t shift) {
 	const float p[3] = {px, py, pz};
 	uint32_t key = 0;
 #pragma unroll
 	for(int ax = 0; ax < 3; ++ax) {
-		float f = floorf((p[ax] - g.emin[ax]) / g.cell[ax]);
-		f = fminf(fmaxf(f, 0.0f), float(g.res[ax] - 1));
+		float f = floorf((p[ax] - g.emin[ax]) / g.cell[ax] * 2.0f);
+		f = fminf(fmaxf(f, 0.0f), float(2 * g.res[ax] - 1));
 		const uint32_t c = uint32_t(f) >> shift;
 #pragma unroll
 		for(int b = 0; b < 6; ++b) key |= ((c >> b) & 1u) << (3 * b + ax);
@@ -224,7 +225,7 @@ __device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const 
 
 // ---- shade-list ordering ----
 // The primary pass appends front-face hits in ray order, so the hits one shading wave takes lie
-// wherever one probe's rays ended.  Sorting the list by the Morton code of the probe-grid cell
+// wherever one probe's rays ended.  Sorting the list by the Morton code of the probe-grid half-cell
 // holding the hit point (cell_key) gives consecutive waves hits that share cage probes (the same
 // atlas tiles) and cast shadow rays from nearby origins along the same direction (the same BVH
 // nodes), which the caches then serve.  Each entry's shading depends on its own record only, so
