@@ -347,7 +347,9 @@ ddgi_status ddgi_direct_light(void* handle, const ddgi_gbuffer_texel* gbuffer, u
  * texel's noise sample (device float2 per texel, NULL = the mirror direction); the hit is shaded as
  * closesthit.glsl at recursion depth 1 (specular from sampleProbes along the reflected ray), a back
  * face is black, a miss sees the sky model; rgb = colorCompression(radiance).  Other texels: 0.
- * The rgb part is what ddgi_gather's `reflection` takes. */
+ * The rgb part is what ddgi_gather's `reflection` takes.  The handle keeps device scratch of 32
+ * bytes per texel (of at most 4 Mi texels at a time), allocated by the first call that needs more
+ * than it holds. */
 ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
 							const float* noise, float* out_rgba, void* stream);
 

@@ -238,6 +238,14 @@ void free_diff(Handle* h) {
 	h->diff = Handle::Diff{};
 }
 
+void free_reflection_scratch(Handle* h) {
+	void* bufs[] = {h->refl_hits, h->refl_list, h->refl_keys, h->refl_bkeys, h->refl_blist, h->refl_count, h->refl_counts};
+	for(void* b : bufs) (void)hipFree(b);
+	h->refl_hits = nullptr;
+	h->refl_list = h->refl_keys = h->refl_bkeys = h->refl_blist = h->refl_count = h->refl_counts = nullptr;
+	h->refl_cap = 0;
+}
+
 void destroy_handle(Handle* h) {
 	free_diff(h);
 	free_scene(h);
@@ -245,6 +253,7 @@ void destroy_handle(Handle* h) {
 		(void)hipFree(h->irr[i]);
 		(void)hipFree(h->dep[i]);
 	}
+	free_reflection_scratch(h);
 	void* bufs[] = {h->states, h->indices, h->sel,     h->rays,         h->hits,       h->shade_list, h->shade_count,
 					h->miss_list, h->shade_keys, h->sort_keys, h->sort_list, h->sort_counts, h->fib,   h->perm,       h->fib512,
 					h->dirs,   h->wt,      h->inv_sum,    h->w_hi,        h->w_lo,       h->visit};
@@ -718,11 +727,8 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 		advance_hysteresis(h, cnt);
 	}
 	const GridConst g = make_grid_const(*h);
-	// shade-list ordering keys: half-cell coordinates below 64 per axis, Morton-interleaved
-	const uint32_t max_cell = uint32_t(2 * std::max({g.res[0], g.res[1], g.res[2]}) - 1);
-	uint32_t key_shift = 0, key_bits = 0;
-	while((max_cell >> key_shift) >= 64u) ++key_shift;
-	while(((max_cell >> key_shift) >> key_bits) != 0u) ++key_bits;
+	uint32_t key_shift = 0, key_bits = 0;  // shade-list ordering keys
+	shade_key_layout(g, &key_shift, &key_bits);
 	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s, h->shade_count));
 	TraceFrameArgs a{};
 	a.scene = h->sd;
@@ -752,7 +758,8 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	HIP_TRY(h, launch_trace_primary(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
 	// shade the front-face hits ordered by the grid cell they lie in
-	HIP_TRY(h, launch_shade_order(a, 3 * key_bits, h->sort_counts, h->sort_keys, h->sort_list, s));
+	HIP_TRY(h, launch_key_sort(h->shade_keys, h->shade_list, h->shade_count, uint64_t(max_slots) * g.n, 3 * key_bits,
+							   h->sort_counts, h->sort_keys, h->sort_list, s));
 	HIP_TRY(h, launch_shade(a, s));
 	if(h->light.sky_model == DDGI_SKY_ATMOSPHERE) HIP_TRY(h, launch_sky(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
@@ -1112,7 +1119,41 @@ ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uin
 	a.count = count;
 	for(int k = 0; k < 3; ++k) a.origin[k] = view_origin[k];
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	HIP_TRY(h, launch_reflection(a, s));
+	// texels go in chunks of at most 4 Mi; the scratch (32 B per texel of a chunk) grows on demand
+	constexpr uint64_t kChunk = uint64_t(1) << 22;
+	const uint32_t cap = uint32_t(std::min<uint64_t>(count, kChunk));
+	if(cap > h->refl_cap) {
+		free_reflection_scratch(h);
+		hipError_t e = dalloc(&h->refl_hits, size_t(cap) * sizeof(float4));
+		if(e == hipSuccess) e = dalloc(&h->refl_list, size_t(cap) * 4);
+		if(e == hipSuccess) e = dalloc(&h->refl_keys, size_t(cap) * 4);
+		if(e == hipSuccess) e = dalloc(&h->refl_bkeys, size_t(cap) * 4);
+		if(e == hipSuccess) e = dalloc(&h->refl_blist, size_t(cap) * 4);
+		if(e == hipSuccess) e = dalloc(&h->refl_count, 16);
+		if(e == hipSuccess) e = dalloc(&h->refl_counts, size_t(kShadeSortCounts) * 4);
+		if(e != hipSuccess) {
+			free_reflection_scratch(h);
+			(void)hipGetLastError();
+			return fail(h, DDGI_ERROR_OUT_OF_MEMORY, "reflection scratch allocation failed");
+		}
+		h->refl_cap = cap;
+	}
+	uint32_t key_shift = 0, key_bits = 0;
+	shade_key_layout(a.grid, &key_shift, &key_bits);
+	a.hits = h->refl_hits;
+	a.list = h->refl_list;
+	a.keys = h->refl_keys;
+	a.list_count = h->refl_count;
+	a.key_shift = key_shift;
+	for(uint64_t first = 0; first < count; first += kChunk) {
+		a.first = first;
+		a.chunk = uint32_t(std::min<uint64_t>(kChunk, count - first));
+		HIP_TRY(h, launch_clear(h->refl_count, 1, s));
+		HIP_TRY(h, launch_reflection_trace(a, s));
+		HIP_TRY(h, launch_key_sort(h->refl_keys, h->refl_list, h->refl_count, a.chunk, 3 * key_bits, h->refl_counts,
+								   h->refl_bkeys, h->refl_blist, s));
+		HIP_TRY(h, launch_reflection_shade(a, s));
+	}
 	h->last_stream = s;
 	return DDGI_OK;
 }

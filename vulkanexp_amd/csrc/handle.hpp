@@ -43,6 +43,15 @@ struct Handle {
 	uint32_t* sort_keys = nullptr;    // P * max_rays: keys / entries bucketed by coarse digit
 	uint32_t* sort_list = nullptr;
 	uint32_t* sort_counts = nullptr;  // kShadeSortCounts
+	// reflection pass scratch (ddgi_reflection), grown to the largest chunk of texels seen
+	uint32_t refl_cap = 0;
+	float4* refl_hits = nullptr;
+	uint32_t* refl_list = nullptr;
+	uint32_t* refl_keys = nullptr;
+	uint32_t* refl_bkeys = nullptr;
+	uint32_t* refl_blist = nullptr;
+	uint32_t* refl_count = nullptr;
+	uint32_t* refl_counts = nullptr;
 	float4* fib = nullptr;      // kMaxRays entries of fib(i, n) for the current n
 	uint16_t* perm = nullptr;   // kMaxRays
 	float4* fib512 = nullptr;   // probe-init directions fib(i, n), i < 512

@@ -166,17 +166,30 @@ struct ReflectionArgs {
 	float4* out;
 	uint64_t count;
 	float origin[3];      // camera position
+	// the chunk of texels [first, first + chunk) being traced; front-face hits of the chunk are
+	// queued (chunk-local texel index, cell key) and shaded in key order
+	uint64_t first;
+	uint32_t chunk;
+	float4* hits;         // [chunk] hit records of the queued texels
+	uint32_t* list;       // [chunk]
+	uint32_t* keys;       // [chunk]
+	uint32_t* list_count;
+	uint32_t key_shift;
 };
 
 // ---- launchers (return hipGetLastError of the launch) ----
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);
-// Orders the frame's shade list (a.shade_list with keys a.shade_keys of key_bits <= 18 bits, length
-// a.shade_count[0]) by key, in place.  Scratch: `counts` (kShadeSortCounts), `bucket_keys` and
-// `bucket_list` (the list's capacity each).
-hipError_t launch_shade_order(const TraceFrameArgs& a, uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys,
-							  uint32_t* bucket_list, hipStream_t s);
+// Orders `list` (keys `keys` of key_bits <= 18 bits, length *count <= capacity on the device) by
+// key, in place, up to the order inside each 4096-entry chunk of a coarse bucket's overflow (see
+// k_sort_fine); entries with equal keys come in any order.  Scratch: `counts` (kShadeSortCounts),
+// `bucket_keys` and `bucket_list` (capacity each).
+hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t* count, uint64_t capacity,
+						   uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys, uint32_t* bucket_list, hipStream_t s);
+// Cell-key layout of the shade ordering for a grid: coordinate shift and bits per axis.
+void shade_key_layout(const GridConst& g, uint32_t* shift, uint32_t* bits_per_axis);
+hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s);
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s);
 // Atmospheric-sky radiance of the frame's misses (the miss list).
@@ -192,7 +205,8 @@ hipError_t launch_refit_level(float4* nodes, const float4* tris, const uint32_t*
 hipError_t launch_gbuffer(const SceneDev& sc, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, hipStream_t s);
 hipError_t launch_direct_light(const DirectLightArgs& a, hipStream_t s);
-hipError_t launch_reflection(const ReflectionArgs& a, hipStream_t s);
+hipError_t launch_reflection_trace(const ReflectionArgs& a, hipStream_t s);
+hipError_t launch_reflection_shade(const ReflectionArgs& a, hipStream_t s);
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
 							  uint32_t* backface_counts, hipStream_t s);
 hipError_t launch_select(const uint32_t* states, uint32_t P, uint32_t cap, SelectState* st, uint32_t* indices,

@@ -38,6 +38,11 @@ __device__ __forceinline__ float4 frame_dir(const Rot& R, float4 f) {
 
 // `zero` (optional): two counters the next kernels start from (the shade- and miss-list lengths),
 // cleared here so that no separate memset is needed.
+__global__ void k_clear(uint32_t* __restrict__ words, uint32_t n) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < n) words[i] = 0u;
+}
+
 __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count,
 							 uint32_t* zero) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,7 +244,8 @@ __device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const 
 //  3. k_sort_bucket: every tile moves its entries into their coarse buckets: a run starts at the
 //     sum of the totals of the smaller digits plus the column prefix (positions inside a run from
 //     LDS counters);
-//  4. k_sort_fine: one workgroup per coarse bucket orders it by the fine digit, back into the list.
+//  4. k_sort_fine: fixed chunks of the bucketed list are ordered by their local keys in LDS, back
+//     into the list.
 // Every position comes from a prefix sum or an LDS counter: no global atomics, whose contention
 // on a few thousand hot counters costs more than the whole sort.
 constexpr uint32_t kSortTiles = 128, kSortDigits = 1024, kSortCoarseBits = 10, kSortUnroll = 8;
@@ -326,20 +332,6 @@ __global__ __launch_bounds__(256) void k_sort_columns(uint32_t* __restrict__ cou
 	}
 }
 
-// First entry of coarse bucket d: the sum of the totals of the digits below d (workgroup-wide).
-__device__ __forceinline__ uint32_t bucket_start(const uint32_t* __restrict__ totals, uint32_t d, uint32_t* s_wave) {
-	uint32_t part = 0;
-	for(uint32_t i = threadIdx.x; i < d; i += blockDim.x) part += totals[i];
-#pragma unroll
-	for(int off = 32; off > 0; off >>= 1) part += uint32_t(__shfl_xor(int(part), off));
-	if(__lane_id() == 0u) s_wave[threadIdx.x >> 6] = part;
-	__syncthreads();
-	uint32_t start = 0;
-	for(uint32_t w = 0; w < (blockDim.x >> 6); ++w) start += s_wave[w];
-	__syncthreads();
-	return start;
-}
-
 __global__ __launch_bounds__(1024) void k_sort_bucket(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ list,
 													  const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
 													  const uint32_t* __restrict__ counts, const uint32_t* __restrict__ totals,
@@ -371,62 +363,50 @@ __global__ __launch_bounds__(1024) void k_sort_bucket(const uint32_t* __restrict
 	}
 }
 
-// Workgroup d orders coarse bucket d by the fine digit (at most 256 fine values) into `out`.
-__global__ __launch_bounds__(256) void k_sort_fine(const uint32_t* __restrict__ bucket_keys,
-												   const uint32_t* __restrict__ bucket_list,
-												   const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
-												   const uint32_t* __restrict__ totals, uint32_t* __restrict__ out) {
-	constexpr uint32_t kU = 8;
-	__shared__ uint32_t s_cnt[256];
-	__shared__ uint32_t s_wave[4];
-	const uint32_t d = blockIdx.x, t = threadIdx.x;
+// Fine pass over fixed chunks of kFineChunk entries of the bucketed list (equal work per workgroup
+// however the keys are spread): a chunk is ordered in LDS by its local key = (coarse digit - the
+// chunk's first coarse digit, fine digit), local keys from 1023 up sharing the last counter.  A
+// chunk inside one bucket is ordered by the fine digit; one spanning buckets keeps their order.
+constexpr uint32_t kFineChunk = 4096;
+__global__ __launch_bounds__(1024) void k_sort_fine(const uint32_t* __restrict__ bucket_keys,
+													const uint32_t* __restrict__ bucket_list,
+													const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
+													uint32_t* __restrict__ out) {
+	constexpr uint32_t kU = kFineChunk / 1024u;
+	__shared__ uint32_t s_cnt[1024];
+	__shared__ uint32_t s_wave[16];
+	const uint32_t t = threadIdx.x;
 	const uint32_t count = __builtin_amdgcn_readfirstlane(*count_ptr);
-	const uint32_t size = totals[d];
-	if(size == 0u) return;  // uniform over the workgroup
-	const uint32_t start = min(count, bucket_start(totals, d, s_wave));
-	const uint32_t end = min(count, start + size);
-	const uint32_t mask = (1u << fine_bits) - 1u;
+	const uint32_t c0 = blockIdx.x * kFineChunk;
+	if(c0 >= count) return;  // uniform over the workgroup
+	const uint32_t c1 = min(count, c0 + kFineChunk);
+	const uint32_t first = bucket_keys[c0] >> fine_bits, mask = (1u << fine_bits) - 1u;
 	s_cnt[t] = 0u;
 	__syncthreads();
-	for(uint32_t i0 = start; i0 < end; i0 += 256u * kU) {
-		uint32_t k[kU];
+	uint32_t lk[kU], e[kU];
 #pragma unroll
-		for(uint32_t u = 0; u < kU; ++u) {
-			const uint32_t i = i0 + 256u * u + t;
-			k[u] = i < end ? bucket_keys[i] : 0xFFFFFFFFu;
+	for(uint32_t u = 0; u < kU; ++u) {
+		const uint32_t i = c0 + 1024u * u + t;
+		lk[u] = 0xFFFFFFFFu;
+		e[u] = 0u;
+		if(i < c1) {
+			const uint32_t k = bucket_keys[i];
+			const uint32_t d = (k >> fine_bits) - first;  // buckets come in increasing digit order
+			lk[u] = d >= (1024u >> fine_bits) ? 1023u : min(1023u, (d << fine_bits) | (k & mask));
+			e[u] = bucket_list[i];
 		}
-#pragma unroll
-		for(uint32_t u = 0; u < kU; ++u)
-			if(k[u] != 0xFFFFFFFFu) atomicAdd(&s_cnt[k[u] & mask], 1u);
 	}
+#pragma unroll
+	for(uint32_t u = 0; u < kU; ++u)
+		if(lk[u] != 0xFFFFFFFFu) atomicAdd(&s_cnt[lk[u]], 1u);
 	__syncthreads();
-	{  // exclusive scan of the 256 fine counts: wave scans, then the 4 wave totals
-		const uint32_t c = s_cnt[t], lane = __lane_id(), wave = t >> 6;
-		uint32_t inc = c;
-#pragma unroll
-		for(int off = 1; off < 64; off <<= 1) {
-			const uint32_t o = uint32_t(__shfl_up(int(inc), off));
-			if(lane >= uint32_t(off)) inc += o;
-		}
-		if(lane == 63u) s_wave[wave] = inc;
-		__syncthreads();
-		uint32_t before = 0;
-		for(uint32_t w = 0; w < wave; ++w) before += s_wave[w];
-		s_cnt[t] = before + inc - c;
-	}
+	uint32_t total;
+	const uint32_t before = block_exclusive_scan_1024(s_cnt[t], s_wave, total);
+	s_cnt[t] = before;
 	__syncthreads();
-	for(uint32_t i0 = start; i0 < end; i0 += 256u * kU) {
-		uint32_t k[kU], e[kU];
 #pragma unroll
-		for(uint32_t u = 0; u < kU; ++u) {
-			const uint32_t i = i0 + 256u * u + t;
-			k[u] = i < end ? bucket_keys[i] : 0xFFFFFFFFu;
-			e[u] = i < end ? bucket_list[i] : 0u;
-		}
-#pragma unroll
-		for(uint32_t u = 0; u < kU; ++u)
-			if(k[u] != 0xFFFFFFFFu) out[start + atomicAdd(&s_cnt[k[u] & mask], 1u)] = e[u];
-	}
+	for(uint32_t u = 0; u < kU; ++u)
+		if(lk[u] != 0xFFFFFFFFu) out[c0 + atomicAdd(&s_cnt[lk[u]], 1u)] = e[u];
 }
 
 // Entry k of the frame's shade list (the front-face hits queued by the primary traversal), so that
@@ -761,53 +741,99 @@ __global__ __launch_bounds__(kTraceBlock) void k_direct_light(DirectLightArgs a)
 // sample: the mirror direction), closest hit, mask 0xFF, tmin 0.1, tmax 10000.  The hit is shaded
 // by closesthit.glsl at recursion depth 1 (its specular term reads the probes), a back face is
 // black, a miss sees the sky; out = (colorCompression(radiance), roughness), other texels 0.
+// k_reflection_trace finishes misses, back faces and untraced texels and queues front-face hits
+// with the cell key of the hit point; k_reflection_shade shades the queue after it is ordered by
+// key (as the probe update's shade list), recomputing each ray's direction with the same operations.
+
+// The reflection ray of texel id, or false when the texel traces none.
+__device__ __forceinline__ bool reflection_ray(const ReflectionArgs& a, uint64_t id, const ddgi_gbuffer_texel& g,
+											   Vec3& dir) {
+	if(!(g.depth > 0.0f && (g.roughness < 0.4f || g.metalness > 0.01f))) return false;
+	const Vec3 position = v3(g.position[0], g.position[1], g.position[2]);
+	const Vec3 normal = v3(g.normal[0], g.normal[1], g.normal[2]);
+	const Vec3 to_origin = normalize3(v3(a.origin[0], a.origin[1], a.origin[2]) - position);
+	const Vec3 incident = v3(-to_origin.x, -to_origin.y, -to_origin.z);
+	const Vec3 rdir = normalize3(incident - normal * (2.0f * dot3(normal, incident)));
+	dir = rdir;
+	if(a.noise) {
+		const float2 nz = a.noise[id];
+		const float theta = g.roughness * (nz.x - 0.5f) * 2.0f * sky::kPi;
+		const float phi = (nz.y - 0.5f) * 2.0f * sky::kPi;
+		const Vec3 tangent = dot3(rdir, normal) < 0.9f ? normalize3(cross3(rdir, normal))
+													   : normalize3(cross3(rdir, v3(1.0f, 0.0f, 0.0f)));
+		dir = rotate_axis(rdir, tangent, theta);
+		dir = rotate_axis(dir, rdir, phi);
+	}
+	return true;
+}
+
+// colorCompression = reinhard_whitepoint(v, 1): v (1 + v / 1) / (1 + v), with the roughness in w
+__device__ __forceinline__ float4 compress_reflection(const Vec3& c, float roughness) {
+	return make_float4(c.x * (1.0f + c.x / 1.0f) / (1.0f + c.x), c.y * (1.0f + c.y / 1.0f) / (1.0f + c.y),
+					   c.z * (1.0f + c.z / 1.0f) / (1.0f + c.z), roughness);
+}
+
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_reflection(ReflectionArgs a) {
+__global__ __launch_bounds__(kTraceBlock) void k_reflection_trace(ReflectionArgs a) {
 	__shared__ uint32_t stack[STACK * kTraceBlock];
-	const uint64_t id = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
-	if(id >= a.count) return;
+	const uint32_t local = blockIdx.x * kTraceBlock + threadIdx.x;
+	const uint64_t id = a.first + local;
+	if(local >= a.chunk || id >= a.count) return;
 	const ddgi_gbuffer_texel g = a.gbuffer[id];
 	float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	if(g.depth > 0.0f && (g.roughness < 0.4f || g.metalness > 0.01f)) {
+	bool queued = false;
+	uint32_t key = 0;
+	Vec3 dir;
+	if(reflection_ray(a, id, g, dir)) {
 		const float o[3] = {g.position[0], g.position[1], g.position[2]};
-		const Vec3 position = v3(o[0], o[1], o[2]);
-		const Vec3 normal = v3(g.normal[0], g.normal[1], g.normal[2]);
-		const Vec3 to_origin = normalize3(v3(a.origin[0], a.origin[1], a.origin[2]) - position);
-		const Vec3 incident = v3(-to_origin.x, -to_origin.y, -to_origin.z);
-		const Vec3 rdir = normalize3(incident - normal * (2.0f * dot3(normal, incident)));
-		Vec3 dir = rdir;
-		if(a.noise) {
-			const float2 nz = a.noise[id];
-			const float theta = g.roughness * (nz.x - 0.5f) * 2.0f * sky::kPi;
-			const float phi = (nz.y - 0.5f) * 2.0f * sky::kPi;
-			const Vec3 tangent = dot3(rdir, normal) < 0.9f ? normalize3(cross3(rdir, normal))
-														   : normalize3(cross3(rdir, v3(1.0f, 0.0f, 0.0f)));
-			dir = rotate_axis(rdir, tangent, theta);
-			dir = rotate_axis(dir, rdir, phi);
-		}
 		RayPrep r;
 		ray_prepare(o[0], o[1], o[2], dir.x, dir.y, dir.z, r);
 		uint32_t nn, nt;
 		const HitResult h = traverse<false, STACK>(a.scene.nodes, a.scene.tris, r, 0.1f, 10000.0f, 0xFFu,
 													stack + threadIdx.x, kTraceBlock, nn, nt);
-		Vec3 c = v3(0.0f, 0.0f, 0.0f);
 		const float4 hit = pack_hit(h);
 		const uint32_t code = __float_as_uint(hit.y);
+		Vec3 c = v3(0.0f, 0.0f, 0.0f);
 		if(code == 0xFFFFFFFFu) {  // miss.rmiss
 			c = v3(a.light.sky[0], a.light.sky[1], a.light.sky[2]);
 			if(a.light.sky_model == DDGI_SKY_ATMOSPHERE)
-				c = sky_atmosphere(position, dir, v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]),
+				c = sky_atmosphere(v3(o[0], o[1], o[2]), dir, v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]),
 								   v3(a.light.color[0], a.light.color[1], a.light.color[2]), a.light.sky_brightness, true);
 		} else if(!(code & 0x80000000u)) {
-			c = shade_surface<STACK, false>(a.scene, a.grid, a.light, a.states, a.irr, a.dep, o, dir, hit.x, code, hit.z,
-											 hit.w, stack + threadIdx.x, kTraceBlock, nullptr)
-					.color;
+			queued = true;
+			a.hits[local] = hit;
+			key = cell_key(a.grid, o[0] + hit.x * dir.x, o[1] + hit.x * dir.y, o[2] + hit.x * dir.z, a.key_shift);
 		}
-		// colorCompression = reinhard_whitepoint(v, 1): v (1 + v / 1) / (1 + v)
-		out = make_float4(c.x * (1.0f + c.x / 1.0f) / (1.0f + c.x), c.y * (1.0f + c.y / 1.0f) / (1.0f + c.y),
-						  c.z * (1.0f + c.z / 1.0f) / (1.0f + c.z), g.roughness);
+		out = compress_reflection(c, g.roughness);
 	}
-	a.out[id] = out;
+	const uint32_t pos = wave_append(a.list_count, queued);
+	if(queued) {
+		a.list[pos] = local;
+		a.keys[pos] = key;
+	} else {
+		a.out[id] = out;
+	}
+}
+
+// Grid-stride over the key-ordered queue of front-face reflection hits.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock, 5) void k_reflection_shade(ReflectionArgs a) {
+	__shared__ uint32_t stack[STACK * kTraceBlock];
+	const uint32_t count = *a.list_count;
+	for(uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x; k < count; k += gridDim.x * kTraceBlock) {
+		const uint32_t local = a.list[k];
+		const uint64_t id = a.first + local;
+		const ddgi_gbuffer_texel g = a.gbuffer[id];
+		Vec3 dir;
+		reflection_ray(a, id, g, dir);  // queued texels trace a ray
+		const float4 hit = a.hits[local];
+		const float o[3] = {g.position[0], g.position[1], g.position[2]};
+		const Vec3 c = shade_surface<STACK, false>(a.scene, a.grid, a.light, a.states, a.irr, a.dep, o, dir, hit.x,
+												   __float_as_uint(hit.y), hit.z, hit.w, stack + threadIdx.x, kTraceBlock,
+												   nullptr)
+						   .color;
+		a.out[id] = compress_reflection(c, g.roughness);
+	}
 }
 
 // ---- instance moves (the TLAS instance update + refit of src/Renderer.cpp:671-742) ----
@@ -1070,18 +1096,34 @@ hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint
 	return hipGetLastError();
 }
 
-hipError_t launch_shade_order(const TraceFrameArgs& a, uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys,
-							  uint32_t* bucket_list, hipStream_t s) {
-	if(uint64_t(a.max_slots) * a.grid.n == 0) return hipSuccess;
-	if(key_bits > kSortCoarseBits + 8u) return hipErrorInvalidValue;
+hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t* count, uint64_t capacity,
+						   uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys, uint32_t* bucket_list, hipStream_t s) {
+	if(key_bits > kSortCoarseBits + 8u || capacity > 0xFFFFFFFFull) return hipErrorInvalidValue;
+	if(capacity == 0) return hipSuccess;
 	const uint32_t fine_bits = key_bits > kSortCoarseBits ? key_bits - kSortCoarseBits : 0u;
 	uint32_t* totals = counts + kSortTiles * kSortDigits;
-	hipLaunchKernelGGL(k_sort_count, dim3(kSortTiles), dim3(1024), 0, s, a.shade_keys, a.shade_count, fine_bits, counts);
+	hipLaunchKernelGGL(k_sort_count, dim3(kSortTiles), dim3(1024), 0, s, keys, count, fine_bits, counts);
 	hipLaunchKernelGGL(k_sort_columns, dim3(kSortDigits / 64u), dim3(256), 0, s, counts, totals);
-	hipLaunchKernelGGL(k_sort_bucket, dim3(kSortTiles), dim3(1024), 0, s, a.shade_keys, a.shade_list, a.shade_count,
-					   fine_bits, counts, totals, bucket_keys, bucket_list);
-	hipLaunchKernelGGL(k_sort_fine, dim3(kSortDigits), dim3(256), 0, s, bucket_keys, bucket_list, a.shade_count, fine_bits,
-					   totals, a.shade_list);
+	hipLaunchKernelGGL(k_sort_bucket, dim3(kSortTiles), dim3(1024), 0, s, keys, list, count, fine_bits, counts, totals,
+					   bucket_keys, bucket_list);
+	hipLaunchKernelGGL(k_sort_fine, dim3(uint32_t((capacity + kFineChunk - 1) / kFineChunk)), dim3(1024), 0, s, bucket_keys,
+					   bucket_list, count, fine_bits, list);
+	return hipGetLastError();
+}
+
+void shade_key_layout(const GridConst& g, uint32_t* shift, uint32_t* bits_per_axis) {
+	// half-cell coordinates below 64 per axis
+	const uint32_t max_cell = uint32_t(2 * std::max({g.res[0], g.res[1], g.res[2]}) - 1);
+	uint32_t sh = 0, b = 0;
+	while((max_cell >> sh) >= 64u) ++sh;
+	while(((max_cell >> sh) >> b) != 0u) ++b;
+	*shift = sh;
+	*bits_per_axis = b;
+}
+
+hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s) {
+	if(n == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, words, n);
 	return hipGetLastError();
 }
 
@@ -1165,10 +1207,22 @@ hipError_t launch_direct_light(const DirectLightArgs& a, hipStream_t s) {
 	return hipGetLastError();
 }
 
-hipError_t launch_reflection(const ReflectionArgs& a, hipStream_t s) {
-	if(a.count == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_reflection<kLdsStack>, dim3(blocks_for(a.count, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+hipError_t launch_reflection_trace(const ReflectionArgs& a, hipStream_t s) {
+	if(a.chunk == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_reflection_trace<kLdsStack>, dim3(blocks_for(a.chunk, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
 	return hipGetLastError();
+}
+
+hipError_t launch_reflection_shade(const ReflectionArgs& a, hipStream_t s) {
+	if(a.chunk == 0) return hipSuccess;
+	const void* f = reinterpret_cast<const void*>(&k_reflection_shade<kLdsStack>);
+	uint64_t resident = 0;  // grid-stride: as many workgroups as fit on the device at once
+	const hipError_t e = resident_blocks(f, kTraceBlock, &resident);
+	if(e != hipSuccess) return e;
+	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(a.chunk, kTraceBlock))));
+	ReflectionArgs arg = a;
+	void* args[] = {&arg};
+	return hipLaunchKernel(f, grid, dim3(kTraceBlock), args, 0, s);
 }
 
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
