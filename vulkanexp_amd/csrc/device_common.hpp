@@ -223,9 +223,12 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 		code[j] = sw_ ? ci_ : cj_;                     \
 	}
 	// Leaves are postponed (Aila and Laine's speculative traversal): a lane that reaches a leaf keeps
-	// it in `leaf` and goes on descending from its stack until every active lane of the wavefront
-	// holds a leaf (or has run out of nodes); the wavefront then tests leaves together.  Closest hit
-	// is the minimum of (t, global id) over all triangles, so the order changes no result.
+	// it in `leaf` and goes on descending from its stack until at least half of the active lanes of
+	// the wavefront hold a leaf (or it has run out of nodes); the wavefront then tests leaves
+	// together.  Waiting for every lane idles the lanes that hold one; testing at once idles the
+	// others; half measured fastest (the probe update's primary pass 0.92 ms when waiting for all,
+	// 0.86 ms at a half; an eighth 0.90, three quarters 0.89).  Closest hit is the minimum of
+	// (t, global id) over all triangles, so the order changes no result.
 	constexpr uint32_t kNone = 0xFFFFFFFFu;  // no node (empty stack) / no postponed leaf
 	uint32_t node = 0;  // root (always an inner node)
 	uint32_t leaf = kNone;
@@ -266,7 +269,8 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 				leaf = node;
 				DDGI_STACK_POP(node, node = kNone)
 			}
-			if(__ballot(leaf == kNone) == 0ull) break;  // every active lane holds a leaf
+			// at least half of the active lanes hold a leaf
+			if(uint32_t(__popcll(__ballot(leaf != kNone))) * 2u >= uint32_t(__popcll(__ballot(true)))) break;
 		}
 		// leaves: the postponed one, then any leaves next on the stack
 		while(leaf != kNone) {
