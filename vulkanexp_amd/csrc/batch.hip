@@ -3,10 +3,11 @@
 // The reference updates one probe grid per IrradianceProbes::update call
 // (src/IrradianceProbes.cpp:426-594).  Small volumes cannot fill 256 CUs and a sequence of
 // per-volume launches pays a launch gap and a tail per kernel.  Here one update of N volumes is a
-// single hipGraph launch of five kernels:
+// single hipGraph launch of nine kernels:
 //   k_batch_begin    frame directions of every volume, reset of the work-queue heads
 //   k_batch_weights  blend weights of every volume (grid: 256 rows x volumes)
 //   k_batch_trace    persistent primary traversal over the work items of all volumes
+//   k_sort_*         (four kernels, grid y = volume) every volume's shade list ordered by hit cell
 //   k_batch_trace    persistent shading + shadow rays over the same items
 //   k_batch_blend    persistent blend + border copy over the probe tiles of all volumes
 // The graph's kernel arguments are fixed; what changes per update (orientation, which atlas is
@@ -161,7 +162,10 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	BHIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
 	hipError_t e = launch_batch_begin(L, b->cap);
 	if(e == hipSuccess) e = launch_batch_weights(L, b->cap);
+	uint64_t max_list = 0;
+	for(const Handle* h : b->vols) max_list = std::max<uint64_t>(max_list, uint64_t(h->P) * h->params.rays_per_probe);
 	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, false, std::min(g_primary, L.trace_item_count), b->cap);
+	if(e == hipSuccess) e = launch_batch_sort(L, max_list, b->cap);
 	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, true, std::min(g_shade, L.trace_item_count), b->cap);
 	if(e == hipSuccess) e = launch_batch_blend(L, std::min(g_blend, L.blend_item_count), b->cap);
 	hipGraph_t graph = nullptr;
@@ -204,6 +208,12 @@ void fill_volume(const Handle& h, const Rot& R, BatchVolume& v) {
 	a.shade_list = h.shade_list;
 	a.shade_count = h.shade_count;
 	a.max_slots = h.P;
+	uint32_t key_shift = 0, key_bits = 0;  // the shade list is ordered by hit cell, as in ddgi_update
+	shade_key_layout(g, &key_shift, &key_bits);
+	a.shade_keys = h.shade_keys;
+	a.key_shift = key_shift;
+	v.sort = SortJob{h.shade_keys, h.shade_list, h.shade_count, h.sort_counts, h.sort_keys, h.sort_list,
+					 sort_fine_bits(3 * key_bits), 0u};
 	BlendArgs& bl = v.blend;
 	bl.grid = g;
 	bl.indices = h.indices;

@@ -227,6 +227,27 @@ constexpr uint32_t kBlendFP32 = 0;
 constexpr uint32_t kBlendBF16 = 1;
 constexpr uint32_t kBlendFP16 = 2;
 
+// A list to order by key (launch_key_sort): keys, entries and the device length; scratch: counts
+// (kShadeSortCounts) and bucket_keys / bucket_list (the list's capacity each).
+struct SortJob {
+	const uint32_t* keys;
+	uint32_t* list;
+	const uint32_t* count;
+	uint32_t* counts;
+	uint32_t* bucket_keys;
+	uint32_t* bucket_list;
+	uint32_t fine_bits;  // key bits below the 10-bit coarse digit
+	uint32_t pad;
+};
+// The sort kernels' argument: one job, or (many != nullptr) job y of workgroup (x, y) at
+// many + y * stride.
+struct SortJobs {
+	SortJob one;
+	const char* many;
+	uint32_t stride;
+};
+uint32_t sort_fine_bits(uint32_t key_bits);
+
 // ---- batched update of many probe volumes (one graph launch per update) ----
 constexpr uint32_t kMaxBatchVolumes = 256;
 constexpr uint32_t kMfmaProbes = 16;   // probes per MFMA blend tile
@@ -245,6 +266,7 @@ struct BatchVolume {
 	void* w_lo;
 	uint32_t threads;      // traversal threads: slots * rays
 	uint32_t n_pad;        // rays rounded up to the MFMA chunk
+	SortJob sort;          // ordering of the shade list (trace.shade_keys) before shading
 };
 
 struct BatchItem {
@@ -275,6 +297,9 @@ struct BatchLaunch {
 hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s);
 hipError_t launch_batch_weights(const BatchLaunch& b, hipStream_t s);
 hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, bool shade, uint32_t grid, hipStream_t s);
+// Orders the shade list of every volume (BatchVolume::sort) in one set of launches; max_capacity
+// = the largest list capacity.
+hipError_t launch_batch_sort(const BatchLaunch& b, uint64_t max_capacity, hipStream_t s);
 hipError_t launch_batch_blend(const BatchLaunch& b, uint32_t grid, hipStream_t s);
 // Resident workgroups per CU of the persistent kernels (for sizing their grids).
 hipError_t batch_trace_resident(uint32_t block, bool shade, int* per_cu);

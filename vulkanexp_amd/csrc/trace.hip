@@ -284,8 +284,18 @@ __device__ __forceinline__ void sort_tile(uint32_t count, uint32_t b, uint32_t& 
 	hi = min(count, lo + per);
 }
 
-__global__ __launch_bounds__(1024) void k_sort_count(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count_ptr,
-													 uint32_t fine_bits, uint32_t* __restrict__ counts) {
+// The list workgroup (x, y) works on: the single job, or job y of a batch's array.
+__device__ __forceinline__ SortJob sort_job(const SortJobs& J) {
+	if(!J.many) return J.one;
+	return *reinterpret_cast<const SortJob*>(J.many + size_t(blockIdx.y) * J.stride);
+}
+
+__global__ __launch_bounds__(1024) void k_sort_count(SortJobs J) {
+	const SortJob j = sort_job(J);
+	const uint32_t* __restrict__ keys = j.keys;
+	const uint32_t* __restrict__ count_ptr = j.count;
+	const uint32_t fine_bits = j.fine_bits;
+	uint32_t* __restrict__ counts = j.counts;
 	__shared__ uint32_t s_cnt[kSortDigits];
 	const uint32_t t = threadIdx.x;
 	s_cnt[t] = 0u;
@@ -309,7 +319,10 @@ __global__ __launch_bounds__(1024) void k_sort_count(const uint32_t* __restrict_
 
 // Column prefix sums: workgroup g takes digits 64 g .. 64 g + 63 (one per lane); wave q of the
 // workgroup walks tiles 32 q .. 32 q + 31 of them (loads coalesced over the lanes, all in flight).
-__global__ __launch_bounds__(256) void k_sort_columns(uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(256) void k_sort_columns(SortJobs J) {
+	const SortJob j = sort_job(J);
+	uint32_t* __restrict__ counts = j.counts;
+	uint32_t* __restrict__ totals = j.counts + kSortTiles * kSortDigits;
 	constexpr uint32_t kQ = kSortTiles / 4u;
 	__shared__ uint32_t s_part[4][64];
 	const uint32_t lane = __lane_id(), q = threadIdx.x >> 6;
@@ -332,10 +345,16 @@ __global__ __launch_bounds__(256) void k_sort_columns(uint32_t* __restrict__ cou
 	}
 }
 
-__global__ __launch_bounds__(1024) void k_sort_bucket(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ list,
-													  const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
-													  const uint32_t* __restrict__ counts, const uint32_t* __restrict__ totals,
-													  uint32_t* __restrict__ bucket_keys, uint32_t* __restrict__ bucket_list) {
+__global__ __launch_bounds__(1024) void k_sort_bucket(SortJobs J) {
+	const SortJob j = sort_job(J);
+	const uint32_t* __restrict__ keys = j.keys;
+	const uint32_t* __restrict__ list = j.list;
+	const uint32_t* __restrict__ count_ptr = j.count;
+	const uint32_t fine_bits = j.fine_bits;
+	const uint32_t* __restrict__ counts = j.counts;
+	const uint32_t* __restrict__ totals = j.counts + kSortTiles * kSortDigits;
+	uint32_t* __restrict__ bucket_keys = j.bucket_keys;
+	uint32_t* __restrict__ bucket_list = j.bucket_list;
 	__shared__ uint32_t s_pos[kSortDigits];
 	__shared__ uint32_t s_wave[16];
 	const uint32_t t = threadIdx.x;
@@ -368,10 +387,13 @@ __global__ __launch_bounds__(1024) void k_sort_bucket(const uint32_t* __restrict
 // chunk's first coarse digit, fine digit), local keys from 1023 up sharing the last counter.  A
 // chunk inside one bucket is ordered by the fine digit; one spanning buckets keeps their order.
 constexpr uint32_t kFineChunk = 4096;
-__global__ __launch_bounds__(1024) void k_sort_fine(const uint32_t* __restrict__ bucket_keys,
-													const uint32_t* __restrict__ bucket_list,
-													const uint32_t* __restrict__ count_ptr, uint32_t fine_bits,
-													uint32_t* __restrict__ out) {
+__global__ __launch_bounds__(1024) void k_sort_fine(SortJobs J) {
+	const SortJob j = sort_job(J);
+	const uint32_t* __restrict__ bucket_keys = j.bucket_keys;
+	const uint32_t* __restrict__ bucket_list = j.bucket_list;
+	const uint32_t* __restrict__ count_ptr = j.count;
+	const uint32_t fine_bits = j.fine_bits;
+	uint32_t* __restrict__ out = j.list;
 	constexpr uint32_t kU = kFineChunk / 1024u;
 	__shared__ uint32_t s_cnt[1024];
 	__shared__ uint32_t s_wave[16];
@@ -1096,19 +1118,32 @@ hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint
 	return hipGetLastError();
 }
 
+static hipError_t launch_sort_jobs(const SortJobs& J, uint32_t jobs, uint64_t capacity, hipStream_t s) {
+	if(capacity == 0 || jobs == 0) return hipSuccess;
+	const uint32_t chunks = uint32_t((capacity + kFineChunk - 1) / kFineChunk);
+	hipLaunchKernelGGL(k_sort_count, dim3(kSortTiles, jobs), dim3(1024), 0, s, J);
+	hipLaunchKernelGGL(k_sort_columns, dim3(kSortDigits / 64u, jobs), dim3(256), 0, s, J);
+	hipLaunchKernelGGL(k_sort_bucket, dim3(kSortTiles, jobs), dim3(1024), 0, s, J);
+	hipLaunchKernelGGL(k_sort_fine, dim3(chunks, jobs), dim3(1024), 0, s, J);
+	return hipGetLastError();
+}
+
+uint32_t sort_fine_bits(uint32_t key_bits) { return key_bits > kSortCoarseBits ? key_bits - kSortCoarseBits : 0u; }
+
 hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t* count, uint64_t capacity,
 						   uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys, uint32_t* bucket_list, hipStream_t s) {
 	if(key_bits > kSortCoarseBits + 8u || capacity > 0xFFFFFFFFull) return hipErrorInvalidValue;
-	if(capacity == 0) return hipSuccess;
-	const uint32_t fine_bits = key_bits > kSortCoarseBits ? key_bits - kSortCoarseBits : 0u;
-	uint32_t* totals = counts + kSortTiles * kSortDigits;
-	hipLaunchKernelGGL(k_sort_count, dim3(kSortTiles), dim3(1024), 0, s, keys, count, fine_bits, counts);
-	hipLaunchKernelGGL(k_sort_columns, dim3(kSortDigits / 64u), dim3(256), 0, s, counts, totals);
-	hipLaunchKernelGGL(k_sort_bucket, dim3(kSortTiles), dim3(1024), 0, s, keys, list, count, fine_bits, counts, totals,
-					   bucket_keys, bucket_list);
-	hipLaunchKernelGGL(k_sort_fine, dim3(uint32_t((capacity + kFineChunk - 1) / kFineChunk)), dim3(1024), 0, s, bucket_keys,
-					   bucket_list, count, fine_bits, list);
-	return hipGetLastError();
+	SortJobs J{};
+	J.one = SortJob{keys, list, count, counts, bucket_keys, bucket_list, sort_fine_bits(key_bits), 0u};
+	return launch_sort_jobs(J, 1, capacity, s);
+}
+
+hipError_t launch_batch_sort(const BatchLaunch& b, uint64_t max_capacity, hipStream_t s) {
+	if(max_capacity > 0xFFFFFFFFull) return hipErrorInvalidValue;
+	SortJobs J{};
+	J.many = reinterpret_cast<const char*>(b.vols) + offsetof(BatchVolume, sort);
+	J.stride = uint32_t(sizeof(BatchVolume));
+	return launch_sort_jobs(J, b.volume_count, max_capacity, s);
 }
 
 void shade_key_layout(const GridConst& g, uint32_t* shift, uint32_t* bits_per_axis) {
