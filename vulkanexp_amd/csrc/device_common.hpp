@@ -441,7 +441,7 @@ constexpr int kSamples = 64;
 
 __device__ __forceinline__ float scale_fn(float c) {
 	const float x = 1.0f - c;
-	return kScaleDepth * __expf(-0.00287f + x * (0.459f + x * (3.83f + x * (-6.80f + x * 5.25f))));
+	return kScaleDepth * __expf(fmaf(x, fmaf(x, fmaf(x, fmaf(x, 5.25f, -6.80f), 3.83f), 0.459f), -0.00287f));
 }
 // First crossing ahead of the sphere (radius r, centred at the origin) by a ray from outside, -1 if none.
 __device__ __forceinline__ float enter_sphere(Vec3 o, Vec3 d, float r) {
@@ -503,9 +503,11 @@ __device__ __forceinline__ Vec3 sky_atmosphere(Vec3 origin, Vec3 dir, Vec3 sun_d
 		int k = 0;  // samples advanced so far
 		for(int i = 0; i < kSamples; ++i) {
 			const Vec3 sp = pos + step * (float(k) + 0.5f);
-			const float h = sqrtf(dot3(sp, sp));
+			// one reciprocal square root instead of a square root and two divisions (the loop is
+			// VALU-bound; the sky's accuracy is set by the hardware exponentials, not by these)
+			const float h2 = dot3(sp, sp), inv_h = rsqrtf(h2), h = h2 * inv_h;
 			const float depth = __expf(kScale / kScaleDepth * (kInner - h));
-			const float scatter = start_offset + depth * (scale_fn(dot3(L, sp) / h) - scale_fn(dot3(dir, sp) / h));
+			const float scatter = start_offset + depth * (scale_fn(dot3(L, sp) * inv_h) - scale_fn(dot3(dir, sp) * inv_h));
 			const Vec3 att = v3(__expf(-scatter * ext.x), __expf(-scatter * ext.y), __expf(-scatter * ext.z));
 			if(!isfinite(att.x) || !isfinite(att.y) || !isfinite(att.z)) continue;
 			color = color + att * (depth * scaled);
