@@ -131,10 +131,10 @@ __device__ __forceinline__ void trace_primary_ray(const TraceFrameArgs& a, uint3
 }
 
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_primary(TraceFrameArgs a) {
-	__shared__ uint32_t stack[STACK * kTraceBlock];
-	trace_primary_ray<STACK, COUNT>(a, xcd_remap(blockIdx.x, gridDim.x) * kTraceBlock + threadIdx.x,
-									stack + threadIdx.x, kTraceBlock);
+__global__ __launch_bounds__(kPrimaryBlock) void k_trace_primary(TraceFrameArgs a) {
+	__shared__ uint32_t stack[STACK * kPrimaryBlock];
+	trace_primary_ray<STACK, COUNT>(a, xcd_remap(blockIdx.x, gridDim.x) * kPrimaryBlock + threadIdx.x,
+									stack + threadIdx.x, kPrimaryBlock);
 }
 
 // ---- closest-hit shading + shadow ray (closesthit.glsl:143-289) ----
@@ -476,12 +476,12 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 // Grid-stride over the shade list (its length is known only on the device).  At least 5 waves per
 // SIMD: the shadow traversal is latency-bound, and the register budget of 5 waves costs no spills.
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock, 5) void k_shade(TraceFrameArgs a) {
-	__shared__ uint32_t stack[STACK * kTraceBlock];
+__global__ __launch_bounds__(kShadeBlock, 5) void k_shade(TraceFrameArgs a) {
+	__shared__ uint32_t stack[STACK * kShadeBlock];
 	const uint32_t count = *a.shade_count;
-	const uint32_t stride = gridDim.x * kTraceBlock;
-	for(uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x; k < count; k += stride)
-		shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kTraceBlock);
+	const uint32_t stride = gridDim.x * kShadeBlock;
+	for(uint32_t k = blockIdx.x * kShadeBlock + threadIdx.x; k < count; k += stride)
+		shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kShadeBlock);
 }
 
 // ---- misses under the atmospheric sky (miss.rmiss:17-23 with sky.glsl:60-127) ----
@@ -839,10 +839,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_reflection_trace(ReflectionArgs
 
 // Grid-stride over the key-ordered queue of front-face reflection hits.
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock, 5) void k_reflection_shade(ReflectionArgs a) {
-	__shared__ uint32_t stack[STACK * kTraceBlock];
+__global__ __launch_bounds__(kShadeBlock, 5) void k_reflection_shade(ReflectionArgs a) {
+	__shared__ uint32_t stack[STACK * kShadeBlock];
 	const uint32_t count = *a.list_count;
-	for(uint32_t k = blockIdx.x * kTraceBlock + threadIdx.x; k < count; k += gridDim.x * kTraceBlock) {
+	for(uint32_t k = blockIdx.x * kShadeBlock + threadIdx.x; k < count; k += gridDim.x * kShadeBlock) {
 		const uint32_t local = a.list[k];
 		const uint64_t id = a.first + local;
 		const ddgi_gbuffer_texel g = a.gbuffer[id];
@@ -851,7 +851,7 @@ __global__ __launch_bounds__(kTraceBlock, 5) void k_reflection_shade(ReflectionA
 		const float4 hit = a.hits[local];
 		const float o[3] = {g.position[0], g.position[1], g.position[2]};
 		const Vec3 c = shade_surface<STACK, false>(a.scene, a.grid, a.light, a.states, a.irr, a.dep, o, dir, hit.x,
-												   __float_as_uint(hit.y), hit.z, hit.w, stack + threadIdx.x, kTraceBlock,
+												   __float_as_uint(hit.y), hit.z, hit.w, stack + threadIdx.x, kShadeBlock,
 												   nullptr)
 						   .color;
 		a.out[id] = compress_reflection(c, g.roughness);
@@ -1165,9 +1165,9 @@ hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s) {
 hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 	const uint64_t threads = uint64_t(a.max_slots) * a.grid.n;
 	if(threads == 0) return hipSuccess;
-	const dim3 grid(blocks_for(threads, kTraceBlock));
-	if(a.visit_counters) hipLaunchKernelGGL((k_trace_primary<kLdsStack, true>), grid, dim3(kTraceBlock), 0, s, a);
-	else hipLaunchKernelGGL((k_trace_primary<kLdsStack, false>), grid, dim3(kTraceBlock), 0, s, a);
+	const dim3 grid(blocks_for(threads, kPrimaryBlock));
+	if(a.visit_counters) hipLaunchKernelGGL((k_trace_primary<kLdsStack, true>), grid, dim3(kPrimaryBlock), 0, s, a);
+	else hipLaunchKernelGGL((k_trace_primary<kLdsStack, false>), grid, dim3(kPrimaryBlock), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -1186,12 +1186,12 @@ hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
 						: reinterpret_cast<const void*>(&k_shade<kLdsStack, false>);
 	// grid-stride: as many workgroups as fit on the device at once (the list length is on the device)
 	uint64_t resident = 0;
-	const hipError_t e = resident_blocks(f, kTraceBlock, &resident);
+	const hipError_t e = resident_blocks(f, kShadeBlock, &resident);
 	if(e != hipSuccess) return e;
-	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(threads, kTraceBlock))));
+	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(threads, kShadeBlock))));
 	TraceFrameArgs arg = a;
 	void* args[] = {&arg};
-	return hipLaunchKernel(f, grid, dim3(kTraceBlock), args, 0, s);
+	return hipLaunchKernel(f, grid, dim3(kShadeBlock), args, 0, s);
 }
 
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
@@ -1252,12 +1252,12 @@ hipError_t launch_reflection_shade(const ReflectionArgs& a, hipStream_t s) {
 	if(a.chunk == 0) return hipSuccess;
 	const void* f = reinterpret_cast<const void*>(&k_reflection_shade<kLdsStack>);
 	uint64_t resident = 0;  // grid-stride: as many workgroups as fit on the device at once
-	const hipError_t e = resident_blocks(f, kTraceBlock, &resident);
+	const hipError_t e = resident_blocks(f, kShadeBlock, &resident);
 	if(e != hipSuccess) return e;
-	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(a.chunk, kTraceBlock))));
+	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(a.chunk, kShadeBlock))));
 	ReflectionArgs arg = a;
 	void* args[] = {&arg};
-	return hipLaunchKernel(f, grid, dim3(kTraceBlock), args, 0, s);
+	return hipLaunchKernel(f, grid, dim3(kShadeBlock), args, 0, s);
 }
 
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
