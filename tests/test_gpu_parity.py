@@ -132,6 +132,19 @@ def _frame_compare(scene, grid, osc, frames, seed, flags=_lib.UPDATE_ALL_PROBES)
     return stats
 
 
+def test_update_long_grid_parity():
+    """A grid of 70 x 2 x 3 probes: the shade list's half-cell keys (140 half cells along x) are
+    shifted right to stay below 64 per axis, a path no cubic test grid reaches.  The ordering
+    changes no output, so the bars are the update parity's."""
+    scene, _ = scenes.config_small(seed=3, open_top=True)
+    lo, hi = scenes.inset_extent(scene)
+    grid = scenes.grid_params(lo, hi, (70, 2, 3), 64)
+    stats = _frame_compare(scene, grid, orc.OracleScene(scene), 2, seed=5)
+    for f, (ok_frac, ie_max, ie_999, de_max, st_eq) in enumerate(stats):
+        print(f"frame {f}: rays within 1e-4 {ok_frac:.5f}, irr max {ie_max:.2e}, depth max {de_max:.2e}, states eq {st_eq:.4f}")
+        assert ok_frac >= 0.995 and ie_max <= 5e-3 and de_max <= 1e-4 and st_eq >= 0.99
+
+
 def test_update_frames_small_parity(small):
     scene, grid, osc = small
     stats = _frame_compare(scene, grid, osc, 4, seed=21)
