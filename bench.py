@@ -8,8 +8,10 @@ irradiance/depth blend with hysteresis + border copy (fused) + publish.
 Prints ONE JSON line (rank 0).  `value` = primary probe rays per second of whole updates, in
 Mrays/s.  `roofline` describes the kernel with the largest share of the step; its algorithmic
 bytes come from the traversal's own fetch counters (128-byte 4-wide nodes, 48-byte triangles) of an
-untimed counting update.  `cpu_baseline` times the CPU oracle (FP64 shading, FP32 intersection
-replica) through its thread pool on a bounded sample of the same frame.
+untimed counting update; `achievable_peak` / `frac_of_achievable` set it against a 2 GiB device
+copy measured in the same run (`achievable_peaks` also holds an FP32 GEMM rate for the blend).
+`cpu_baseline` times the CPU oracle (FP64 shading, FP32 intersection replica) through its thread
+pool on a bounded sample of the same frame.
 """
 from __future__ import annotations
 
@@ -46,6 +48,7 @@ def parse():
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-dynamic", action="store_true")
     ap.add_argument("--no-sky", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the achievable-peak measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 3 elementwise copies of 256 MiB (torch.add(x, 0)) so that profiler byte "
                          "counters can be scaled against a kernel of known traffic")
@@ -342,6 +345,36 @@ def batch_bench(lib, frames=5):
     }
 
 
+def achievable_peaks(torch):
+    """Peaks measured on this GPU, for the roofline fractions beside the published ones: a device
+    copy of 2 GiB (beyond the 256 MB Infinity Cache; read + written bytes per second) and an FP32
+    GEMM of 8192^3 through torch (hipBLASLt / rocBLAS on the matrix cores)."""
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+    src = torch.ones(512 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    t_copy = timed(lambda: dst.copy_(src), 10)
+    copy_gbs = 2 * src.numel() * 4 / t_copy / 1e9
+    del src, dst
+    a = torch.randn(8192, 8192, device="cuda", dtype=torch.float32)
+    b = torch.randn(8192, 8192, device="cuda", dtype=torch.float32)
+    t_mm = timed(lambda: torch.mm(a, b), 5)
+    gemm_tf = 2 * 8192 ** 3 / t_mm / 1e12
+    del a, b
+    torch.cuda.empty_cache()
+    return {"copy_GB/s": round(copy_gbs, 1), "fp32_gemm_TFLOP/s": round(gemm_tf, 1),
+            "copy": "2 GiB device-to-device torch copy_, read + written bytes",
+            "gemm": "torch.mm 8192x8192x8192 float32"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -414,6 +447,14 @@ def main():
         v["GB/s"] = round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
         v["ms"] = round(v["ms"], 4)
     kernels["blend"]["TFLOP/s"] = round(blend_flops / (avg["blend_ms"] * 1e-3) / 1e12, 2)
+    peaks = None if args.no_peaks else achievable_peaks(torch)
+    if roofline["traffic"]:  # bytes that left L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE) per second
+        roofline["traffic_GB/s"] = round(roofline["traffic"] / (dk["ms"] * 1e-3) / 1e9, 1)
+    if peaks:
+        roofline["achievable_peak"] = peaks["copy_GB/s"]
+        roofline["frac_of_achievable"] = round(achieved / peaks["copy_GB/s"], 4)
+        kernels["blend"]["frac_of_fp32_peak"] = round(kernels["blend"]["TFLOP/s"] / FP32_PEAK_TFLOPS, 4)
+        kernels["blend"]["frac_of_fp32_gemm"] = round(kernels["blend"]["TFLOP/s"] / peaks["fp32_gemm_TFLOP/s"], 4)
     trace_stats = {"nodes_per_primary_ray": st["primary_nodes"] / rays, "tris_per_primary_ray": st["primary_tris"] / rays,
                    "shadow_rays": st["shadow_rays"], "nodes_per_shadow_ray": st["shadow_nodes"] / max(1, st["shadow_rays"])}
 
@@ -448,7 +489,7 @@ def main():
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
         "full_frame": frame, "blend_precision": blend, "differentiable": diff, "batched_volumes": batched,
-        "dynamic_scene": dynamic, "atmospheric_sky": atmosphere,
+        "dynamic_scene": dynamic, "atmospheric_sky": atmosphere, "achievable_peaks": peaks,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
