@@ -145,6 +145,22 @@ def test_update_long_grid_parity():
         assert ok_frac >= 0.995 and ie_max <= 5e-3 and de_max <= 1e-4 and st_eq >= 0.99
 
 
+def test_update_with_no_front_face_hits():
+    """Every probe ray misses (the scene's one quad lies 500 m above the grid, beyond the rays'
+    tmax of the grid diagonal): the shade list is empty, so the key sort and the shading run on a
+    list of length 0, and every ray is sky."""
+    sb = scenes.SceneBuilder()
+    m = sb.add_material((0.5, 0.5, 0.5))
+    pos = np.array([[0, 500, 0], [1, 500, 0], [1, 500, 1], [0, 500, 1]], np.float32)
+    nrm = np.tile(np.array([0, 1, 0], np.float32), (4, 1))
+    sb.add_instance(sb.add_mesh(pos, nrm, np.array([[0, 2, 1], [0, 3, 2]]), m))
+    scene = sb.build((-10, -10, -10), (10, 10, 10))
+    grid = scenes.grid_params((-8, -8, -8), (8, 8, 8), (4, 2, 4), 64)
+    stats = _frame_compare(scene, grid, orc.OracleScene(scene), 2, seed=9)
+    for f, (ok_frac, ie_max, ie_999, de_max, st_eq) in enumerate(stats):
+        assert ok_frac == 1.0 and ie_max <= 5e-3 and de_max <= 1e-4 and st_eq >= 0.99
+
+
 def test_update_frames_small_parity(small):
     scene, grid, osc = small
     stats = _frame_compare(scene, grid, osc, 4, seed=21)

@@ -90,3 +90,22 @@ def test_reflection_pass_matches_oracle(setup, jitter, sky):
     assert np.mean(err.max(1) <= 1e-4) >= 0.995
     if sky == "atmosphere":  # a ray within ~1e-3 rad of the horizon may flip ground/sky in FP32
         assert np.quantile(err[miss].max(1), 0.999) <= 2e-4
+
+
+def test_reflection_pass_chunks_are_seamless(setup):
+    """More than 4 Mi texels go through the pass in chunks (each with its own hit queue and key
+    sort): a G-buffer of 203 copies of the test view (4.2 M texels, the second chunk starting
+    inside copy 202) must give 203 copies of the single view's output, bit for bit."""
+    import torch
+    s = setup
+    p = s["p"]
+    one = p.reflection(s["gbt"], EYE).cpu().numpy()
+    copies = 203
+    assert copies * W * H > 1 << 22
+    big = s["gbt"].repeat(copies, 1)
+    got = p.reflection(big, EYE).cpu().numpy().reshape(copies, W * H, 4)
+    del big
+    torch.cuda.empty_cache()
+    bad = [c for c in range(copies) if not np.array_equal(got[c].view(np.uint32), one.view(np.uint32))]
+    assert not bad, f"copies {bad[:5]} differ from the single view"
+
