@@ -183,9 +183,10 @@ struct ReflectionArgs {
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);
-// Orders `list` (keys `keys` of key_bits <= 18 bits, length *count <= capacity on the device) by
-// key, in place, up to the order inside each 4096-entry chunk of a coarse bucket's overflow (see
-// k_sort_fine); entries with equal keys come in any order.  Scratch: `counts` (kShadeSortCounts),
+// Orders `list` (keys `keys` of key_bits <= 18 bits, length *count <= capacity on the device) in
+// place: by the keys' high 10 bits over the whole list, and by the whole key inside each fixed
+// chunk of 4096 entries (a bucket of equal high bits longer than a chunk is ordered chunk by
+// chunk); entries with equal keys come in any order.  Scratch: `counts` (kShadeSortCounts),
 // `bucket_keys` and `bucket_list` (capacity each).
 hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t* count, uint64_t capacity,
 						   uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys, uint32_t* bucket_list, hipStream_t s);
