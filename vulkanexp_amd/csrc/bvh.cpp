@@ -277,10 +277,10 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		d[7] = u2f(inst_of[t]);
 		d[11] = u2f(meta[t]);
 	}
-	// ---- collapse to a 4-wide tree: each wide node takes the children of a binary node and keeps
-	// opening its largest-area inner child until it has four ----
+	// ---- collapse to a kWide-wide tree: each wide node takes the children of a binary node and
+	// keeps opening its largest-area inner child until it has kWide ----
 	auto leaf_code = [&](const BuildNode& n) { return kLeafFlag | (n.first << 3) | (n.count - 1); };
-	auto wide_children = [&](int32_t n, int32_t ch[4]) -> int {
+	auto wide_children = [&](int32_t n, int32_t ch[kWide]) -> int {
 		int cnt = 0;
 		if(b.nodes[n].leaf()) {  // the whole scene is one leaf: a root with that leaf as its only child
 			ch[cnt++] = n;
@@ -288,7 +288,7 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		}
 		ch[cnt++] = b.nodes[n].left;
 		ch[cnt++] = b.nodes[n].right;
-		while(cnt < 4) {
+		while(cnt < int(kWide)) {
 			int pick = -1;
 			double area = -1.0;
 			for(int k = 0; k < cnt; ++k) {
@@ -308,25 +308,29 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 	struct Wide {
 		int32_t bin;       // binary node it was made from
 		uint32_t depth;
-		int32_t kids[4];   // binary child nodes, -1 = empty slot
+		int32_t kids[kWide];  // binary child nodes, -1 = empty slot
 	};
 	// depth first: the children of a node get consecutive indices, then each child's subtree in turn
-	std::vector<Wide> wide{{0, 1, {-1, -1, -1, -1}}};
+	Wide root{0, 1, {}};
+	for(uint32_t k = 0; k < kWide; ++k) root.kids[k] = -1;
+	std::vector<Wide> wide{root};
 	std::vector<uint32_t> todo{0};
 	uint32_t wide_depth = 0;
 	while(!todo.empty()) {
 		const uint32_t w = todo.back();
 		todo.pop_back();
-		int32_t ch[4];
+		int32_t ch[kWide];
 		const int cnt = wide_children(wide[w].bin, ch);
 		wide_depth = std::max(wide_depth, wide[w].depth);
-		uint32_t created[4];
+		uint32_t created[kWide];
 		int ncreated = 0;
-		for(int k = 0; k < 4; ++k) {
+		for(int k = 0; k < int(kWide); ++k) {
 			wide[w].kids[k] = k < cnt ? ch[k] : -1;
 			if(k < cnt && !b.nodes[ch[k]].leaf()) {
 				created[ncreated++] = uint32_t(wide.size());
-				wide.push_back({ch[k], wide[w].depth + 1, {-1, -1, -1, -1}});
+				Wide child{ch[k], wide[w].depth + 1, {}};
+				for(uint32_t q = 0; q < kWide; ++q) child.kids[q] = -1;
+				wide.push_back(child);
 			}
 		}
 		for(int k = ncreated - 1; k >= 0; --k) todo.push_back(created[k]);
@@ -338,7 +342,7 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 	out.nodes.assign(wide.size() * kNodeFloats, 0.0f);
 	for(size_t w = 0; w < wide.size(); ++w) {
 		float* rec = &out.nodes[w * kNodeFloats];
-		for(int k = 0; k < 4; ++k) {
+		for(int k = 0; k < int(kWide); ++k) {
 			const int32_t c = wide[w].kids[k];
 			float lo[3], hi[3];
 			uint32_t code = kEmptyChild;
@@ -353,10 +357,10 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 				for(int a = 0; a < 3; ++a) lo[a] = hi[a] = 1e30f;  // unreachable point box
 			}
 			for(int a = 0; a < 3; ++a) {
-				rec[(2 * a) * 4 + k] = lo[a];
-				rec[(2 * a + 1) * 4 + k] = hi[a];
+				rec[(2 * a) * kWide + k] = lo[a];
+				rec[(2 * a + 1) * kWide + k] = hi[a];
 			}
-			rec[24 + k] = u2f(code);
+			rec[kCodeRow + k] = u2f(code);
 		}
 	}
 	out.node_count = uint32_t(wide.size());

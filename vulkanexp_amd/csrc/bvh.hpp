@@ -3,13 +3,14 @@
 // so traversal is single-level.
 //
 // Device layout (all float4 / 16-byte aligned, read with 128-bit loads):
-//   node k  : 8 float4 = 128 bytes, a 4-wide node collapsed from the binary SAH tree, holding the
-//             boxes of its (up to) four children side by side so that one node fetch tests all four:
-//               [0] = lo.x of children 0..3     [1] = hi.x     [2] = lo.y
-//               [3] = hi.y                      [4] = lo.z     [5] = hi.z
-//               [6] = child codes 0..3 as int bits: inner node index, or
-//                     leaf = 0x80000000 | first_triangle << 3 | (count - 1), or kEmptyChild
-//               [7] = unused
+//   node k  : kNodeFloats floats, a kWide-wide node collapsed from the binary SAH tree, holding the
+//             boxes of its (up to) kWide children side by side so that one node fetch tests all of
+//             them, in rows of kWide floats (one float4 per 4 children):
+//               row 0 = lo.x of children 0..kWide-1   row 1 = hi.x   row 2 = lo.y
+//               row 3 = hi.y                          row 4 = lo.z   row 5 = hi.z
+//               row 6 = child codes as int bits: inner node index, or
+//                       leaf = 0x80000000 | first_triangle << 3 | (count - 1), or kEmptyChild
+//             then padding (4-wide: 128 bytes, 8-wide: 256 bytes)
 //   triangle: 3 float4 = 48 bytes, leaf order:
 //               [0] = (v0.xyz, global triangle id)   global id = instance-major flattened index
 //               [1] = (v1.xyz, instance index)
@@ -28,8 +29,13 @@ namespace ddgi {
 constexpr uint32_t kLeafFlag = 0x80000000u;
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr uint32_t kMaxLeafSize = 8;
-constexpr uint32_t kNodeFloats = 32;     // 128-byte wide node
-constexpr uint32_t kMaxWideDepth = 32;   // traversal stack: 3 entries per level (16 LDS + 80 private)
+// children per node: 4 (8-wide nodes visit 26 % fewer nodes per probe ray but trace slower: the
+// wider fetch, slab tests and sorting network cost registers and occupancy)
+constexpr uint32_t kWide = 4;
+constexpr uint32_t kNodeFloats = kWide == 4 ? 32 : 64;  // 6 box rows + 1 code row of kWide, padded
+constexpr uint32_t kCodeRow = 6 * kWide;               // float offset of the child codes
+constexpr uint32_t kMaxWideDepth = 32;   // traversal stack: kWide - 1 entries per level
+static_assert(kWide == 4 || kWide == 8, "nodes are 4- or 8-wide");
 
 struct BvhHost {
 	std::vector<float> nodes;     // kNodeFloats per node
@@ -38,7 +44,7 @@ struct BvhHost {
 	uint32_t node_count = 0;
 	uint32_t tri_count = 0;
 	uint32_t max_depth = 0;       // binary tree
-	uint32_t wide_depth = 0;      // levels of the 4-wide tree
+	uint32_t wide_depth = 0;      // levels of the wide tree
 	double sah_cost = 0.0;
 	// for refits after instance moves: object-space vertices of each triangle slot (9 floats, leaf
 	// order) and the wide nodes grouped by depth (level d = nodes level_nodes[level_offsets[d-1] ..

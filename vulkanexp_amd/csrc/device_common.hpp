@@ -143,18 +143,20 @@ struct HitResult {
 // Stack: the newest STACK entries live in an LDS ring laid out [entry][thread] (conflict-free;
 // `stack` points at this thread's column, `stride` is the block size); when the ring is full its
 // oldest entry moves to a private spill array, and pops drain the ring first, then the spill, so
-// the order stays LIFO.  At most 3 entries per level of the wide tree are pending, and
+// the order stays LIFO.  At most kWide - 1 entries per level of the wide tree are pending, and
 // STACK + kSpill covers the deepest tree accepted (kMaxWideDepth levels), while the LDS footprint
 // stays small enough for full occupancy.
 //
 // Leaves are tested two triangles at a time, each triangle's 48 bytes loaded in one go, so that
 // two loads are in flight and the mask word costs no extra round trip.
 //
-// With COUNT, n_nodes / n_tris receive the number of 128-byte node and 48-byte triangle fetches
+// With COUNT, n_nodes / n_tris receive the number of node (kNodeFloats * 4 bytes) and 48-byte
+// triangle fetches
 // (the algorithmic bytes of the traversal).
 constexpr int kLdsStack = 16;  // LDS ring entries per thread
-constexpr int kSpill = 80;      // private spill entries: kLdsStack + kSpill = 3 * kMaxWideDepth
-static_assert(kLdsStack + kSpill >= 3 * int(kMaxWideDepth), "traversal stack too small for the deepest tree");
+// private spill entries: kLdsStack + kSpill = (kWide - 1) * kMaxWideDepth
+constexpr int kSpill = (int(kWide) - 1) * int(kMaxWideDepth) - kLdsStack;
+static_assert(kLdsStack + kSpill >= (int(kWide) - 1) * int(kMaxWideDepth), "traversal stack too small for the deepest tree");
 
 // CULL: back-facing triangles are ignored (rasterizer back-face culling).
 template <bool ANY, bool CULL>
@@ -238,32 +240,73 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 		// descend through inner nodes
 		while(!(node & kLeafFlag)) {
 			const float4* nr = nodes + size_t(node) * (kNodeFloats / 4);
-			const float4 lx = nr[0], hx = nr[1], ly = nr[2], hy = nr[3], lz = nr[4], hz = nr[5], cc = nr[6];
+			constexpr int G = int(kWide) / 4;  // float4s per row
+			float bx[6][kWide];
+			uint32_t code[kWide];
+#pragma unroll
+			for(int row = 0; row < 6; ++row)
+#pragma unroll
+				for(int g = 0; g < G; ++g) {
+					const float4 v = nr[row * G + g];
+					bx[row][4 * g] = v.x;
+					bx[row][4 * g + 1] = v.y;
+					bx[row][4 * g + 2] = v.z;
+					bx[row][4 * g + 3] = v.w;
+				}
+#pragma unroll
+			for(int g = 0; g < G; ++g) {
+				const float4 v = nr[6 * G + g];
+				code[4 * g] = __float_as_uint(v.x);
+				code[4 * g + 1] = __float_as_uint(v.y);
+				code[4 * g + 2] = __float_as_uint(v.z);
+				code[4 * g + 3] = __float_as_uint(v.w);
+			}
 			++n_nodes;
-			float key[4];
-			uint32_t code[4];
-			code[0] = __float_as_uint(cc.x);
-			code[1] = __float_as_uint(cc.y);
-			code[2] = __float_as_uint(cc.z);
-			code[3] = __float_as_uint(cc.w);
-			float tn;
-			key[0] = (slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, best.t, tn) && code[0] != kEmptyChild) ? tn : kInf;
-			key[1] = (slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, best.t, tn) && code[1] != kEmptyChild) ? tn : kInf;
-			key[2] = (slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, best.t, tn) && code[2] != kEmptyChild) ? tn : kInf;
-			key[3] = (slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, best.t, tn) && code[3] != kEmptyChild) ? tn : kInf;
-			const int nh = int(key[0] != kInf) + int(key[1] != kInf) + int(key[2] != kInf) + int(key[3] != kInf);
+			float key[kWide];
+			int nh = 0;
+#pragma unroll
+			for(int k = 0; k < int(kWide); ++k) {
+				float tn;
+				key[k] = (slab(r, bx[0][k], bx[1][k], bx[2][k], bx[3][k], bx[4][k], bx[5][k], tmin, best.t, tn) &&
+						  code[k] != kEmptyChild)
+							 ? tn
+							 : kInf;
+				nh += int(key[k] != kInf);
+			}
 			if(nh == 0) {
 				DDGI_STACK_POP(node, node = kNone)
 			} else {
-				DDGI_CSWAP(0, 1)
-				DDGI_CSWAP(2, 3)
-				DDGI_CSWAP(0, 2)
-				DDGI_CSWAP(1, 3)
-				DDGI_CSWAP(1, 2)
+				if constexpr(kWide == 4) {
+					DDGI_CSWAP(0, 1)
+					DDGI_CSWAP(2, 3)
+					DDGI_CSWAP(0, 2)
+					DDGI_CSWAP(1, 3)
+					DDGI_CSWAP(1, 2)
+				} else {  // Batcher's odd-even merge network for 8 keys (19 comparators)
+					DDGI_CSWAP(0, 1)
+					DDGI_CSWAP(2, 3)
+					DDGI_CSWAP(4, 5)
+					DDGI_CSWAP(6, 7)
+					DDGI_CSWAP(0, 2)
+					DDGI_CSWAP(1, 3)
+					DDGI_CSWAP(4, 6)
+					DDGI_CSWAP(5, 7)
+					DDGI_CSWAP(1, 2)
+					DDGI_CSWAP(5, 6)
+					DDGI_CSWAP(0, 4)
+					DDGI_CSWAP(1, 5)
+					DDGI_CSWAP(2, 6)
+					DDGI_CSWAP(3, 7)
+					DDGI_CSWAP(2, 4)
+					DDGI_CSWAP(3, 5)
+					DDGI_CSWAP(1, 2)
+					DDGI_CSWAP(3, 4)
+					DDGI_CSWAP(5, 6)
+				}
 				node = code[0];
-				if(nh > 3) DDGI_STACK_PUSH(code[3])
-				if(nh > 2) DDGI_STACK_PUSH(code[2])
-				if(nh > 1) DDGI_STACK_PUSH(code[1])
+#pragma unroll
+				for(int k = int(kWide) - 1; k >= 1; --k)
+					if(nh > k) DDGI_STACK_PUSH(code[k])
 			}
 			if(leaf == kNone && node != kNone && (node & kLeafFlag)) {  // postpone the leaf
 				leaf = node;

@@ -907,8 +907,8 @@ __global__ void k_refit_level(float4* __restrict__ nodes, const float4* __restri
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= count) return;
 	float* rec = reinterpret_cast<float*>(nodes + size_t(level_nodes[i]) * (kNodeFloats / 4));
-	for(int k = 0; k < 4; ++k) {
-		const uint32_t code = __float_as_uint(rec[24 + k]);
+	for(int k = 0; k < int(kWide); ++k) {
+		const uint32_t code = __float_as_uint(rec[kCodeRow + k]);
 		if(code == kEmptyChild) continue;
 		float lo[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
 		float hi[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
@@ -929,17 +929,17 @@ __global__ void k_refit_level(float4* __restrict__ nodes, const float4* __restri
 			}
 		} else {  // the child's padded boxes already bound it
 			const float* ch = reinterpret_cast<const float*>(nodes + size_t(code) * (kNodeFloats / 4));
-			for(int c = 0; c < 4; ++c) {
-				if(__float_as_uint(ch[24 + c]) == kEmptyChild) continue;
+			for(int c = 0; c < int(kWide); ++c) {
+				if(__float_as_uint(ch[kCodeRow + c]) == kEmptyChild) continue;
 				for(int a = 0; a < 3; ++a) {
-					lo[a] = fminf(lo[a], ch[(2 * a) * 4 + c]);
-					hi[a] = fmaxf(hi[a], ch[(2 * a + 1) * 4 + c]);
+					lo[a] = fminf(lo[a], ch[(2 * a) * kWide + c]);
+					hi[a] = fmaxf(hi[a], ch[(2 * a + 1) * kWide + c]);
 				}
 			}
 		}
 		for(int a = 0; a < 3; ++a) {
-			rec[(2 * a) * 4 + k] = lo[a];
-			rec[(2 * a + 1) * 4 + k] = hi[a];
+			rec[(2 * a) * kWide + k] = lo[a];
+			rec[(2 * a + 1) * kWide + k] = hi[a];
 		}
 	}
 }
