@@ -7,6 +7,7 @@
 // Built with -ffp-contract=off (see device_common.hpp).
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "device_common.hpp"
@@ -1137,6 +1138,8 @@ hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t*
 	J.one = SortJob{keys, list, count, counts, bucket_keys, bucket_list, sort_fine_bits(key_bits), 0u};
 	return launch_sort_jobs(J, 1, capacity, s);
 }
+
+static_assert(std::is_standard_layout<BatchVolume>::value, "the batched sort finds each volume's job by offsetof");
 
 hipError_t launch_batch_sort(const BatchLaunch& b, uint64_t max_capacity, hipStream_t s) {
 	if(max_capacity > 0xFFFFFFFFull) return hipErrorInvalidValue;
