@@ -396,7 +396,7 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->rays, P * R * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->hits, P * R * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->shade_list, P * R * sizeof(uint32_t));
-	if(e == hipSuccess) e = dalloc(&h->shade_count, 16);
+	if(e == hipSuccess) e = dalloc(&h->shade_count, kListCounterBytes);
 	if(e == hipSuccess) e = dalloc(&h->shade_keys, P * R * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->sort_keys, P * R * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->sort_list, P * R * sizeof(uint32_t));

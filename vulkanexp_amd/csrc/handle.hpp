@@ -37,7 +37,7 @@ struct Handle {
 	float4* rays = nullptr;
 	float4* hits = nullptr;
 	uint32_t* shade_list = nullptr;   // P * max_rays entries
-	uint32_t* shade_count = nullptr;  // [0] shade list, [1] miss list
+	uint32_t* shade_count = nullptr;  // [0] shade list, [kMissCountSlot] miss list
 	uint32_t* miss_list = nullptr;    // P * max_rays entries, allocated with the atmospheric sky
 	uint32_t* shade_keys = nullptr;   // P * max_rays: cell key of each shade-list entry
 	uint32_t* sort_keys = nullptr;    // P * max_rays: keys / entries bucketed by coarse digit

@@ -71,6 +71,11 @@ struct SceneDev {
 
 // Internal hit record: (t, slot | back << 31, u, v); slot = triangle position in the BVH array,
 // 0xFFFFFFFF = miss (t = -1).
+// The miss list's counter sits a cache line after the shade list's: every wavefront of the primary
+// pass appends to both, and atomics on one line serialise.
+constexpr uint32_t kMissCountSlot = 32;
+constexpr uint32_t kListCounterBytes = 256;
+
 struct TraceFrameArgs {
 	SceneDev scene;
 	GridConst grid;
@@ -85,7 +90,7 @@ struct TraceFrameArgs {
 	float4* hits;
 	float4* rays;              // (rgb, depth)
 	uint32_t* shade_list;      // ray positions (slot * n + k) of the front-face hits, filled by the primary pass
-	uint32_t* shade_count;     // [0]: its length, [1]: the miss list's (both cleared before the primary pass)
+	uint32_t* shade_count;     // [0]: its length, [kMissCountSlot]: the miss list's (both cleared before the primary pass)
 	uint32_t* miss_list;       // atmospheric sky only: ray positions of the misses
 	uint32_t max_slots;        // grid sized for this many slots
 	// optional (statistics runs only): [0]/[1] primary node / triangle fetches, [2]/[3] shadow

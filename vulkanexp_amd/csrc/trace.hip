@@ -47,7 +47,7 @@ __global__ void k_clear(uint32_t* __restrict__ words, uint32_t n) {
 __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count,
 							 uint32_t* zero) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if(zero && i == 0) zero[0] = zero[1] = 0u;
+	if(zero && i == 0) zero[0] = zero[kMissCountSlot] = 0u;
 	if(i >= count) return;
 	dirs[i] = frame_dir(R, fib[i]);
 }
@@ -98,7 +98,7 @@ __device__ __forceinline__ void finish_primary_ray(const TraceFrameArgs& a, uint
 	}
 	const bool miss = code == 0xFFFFFFFFu;
 	if(a.light.sky_model == DDGI_SKY_ATMOSPHERE) {  // misses are queued for the sky kernel
-		const uint32_t mpos = wave_append(a.shade_count + 1, miss);
+		const uint32_t mpos = wave_append(a.shade_count + kMissCountSlot, miss);
 		if(miss) a.miss_list[mpos] = ri;
 		else if(!front) write_unshaded(a, ri, false, h.t);
 	} else if(!front) {
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(kShadeBlock, 5) void k_shade(TraceFrameArgs a) {
 // Entry k of the miss list: the sky seen from the probe along the ray, with the light's colour
 // scaled by its 4th component and the sun's Mie lobe; depth -1.
 __global__ __launch_bounds__(256) void k_sky(TraceFrameArgs a) {
-	const uint32_t count = a.shade_count[1];
+	const uint32_t count = a.shade_count[kMissCountSlot];
 	const uint32_t n = a.grid.n;
 	const Vec3 L = v3(a.light.dir[0], a.light.dir[1], a.light.dir[2]);
 	const Vec3 C = v3(a.light.color[0], a.light.color[1], a.light.color[2]);
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(512) void k_batch_begin(BatchLaunch b) {
 	const BatchVolume& v = b.vols[blockIdx.x];
 	const uint32_t n = v.trace.grid.n;
 	for(uint32_t i = threadIdx.x; i < n; i += 512) v.dirs[i] = frame_dir(v.R, v.fib[i]);
-	if(threadIdx.x == 0) v.trace.shade_count[0] = v.trace.shade_count[1] = 0u;
+	if(threadIdx.x == 0) v.trace.shade_count[0] = v.trace.shade_count[kMissCountSlot] = 0u;
 	if(blockIdx.x == 0) {
 		if(threadIdx.x < 4) b.ctl->next[threadIdx.x] = 0u;
 		for(uint32_t k = threadIdx.x; k < kMaxBatchVolumes; k += 512) b.ctl->done[k] = 0ull;
