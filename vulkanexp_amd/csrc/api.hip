@@ -395,9 +395,11 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 	if(e == hipSuccess) e = dalloc(&h->sel, sizeof(SelectState));
 	if(e == hipSuccess) e = dalloc(&h->rays, P * R * sizeof(float4));
 	if(e == hipSuccess) e = dalloc(&h->hits, P * R * sizeof(float4));
-	if(e == hipSuccess) e = dalloc(&h->shade_list, P * R * sizeof(uint32_t));
+	// shade list and keys: the primary pass's segments hold up to kShadeSegments * kPrimaryBlock
+	// entries beyond P * R (each segment is sized for whole workgroups)
+	if(e == hipSuccess) e = dalloc(&h->shade_list, (P * R + kShadeSegments * kPrimaryBlock) * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->shade_count, kListCounterBytes);
-	if(e == hipSuccess) e = dalloc(&h->shade_keys, P * R * sizeof(uint32_t));
+	if(e == hipSuccess) e = dalloc(&h->shade_keys, (P * R + kShadeSegments * kPrimaryBlock) * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->sort_keys, P * R * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->sort_list, P * R * sizeof(uint32_t));
 	if(e == hipSuccess) e = dalloc(&h->sort_counts, size_t(kShadeSortCounts) * sizeof(uint32_t));
@@ -755,11 +757,16 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	}
 	a.shade_keys = h->shade_keys;
 	a.key_shift = key_shift;
+	{  // segment s of the shade list takes the front-face hits of workgroups b with b % 8 == s
+		const uint32_t blocks = uint32_t((uint64_t(max_slots) * g.n + kPrimaryBlock - 1) / kPrimaryBlock);
+		a.seg_cap = (blocks + kShadeSegments - 1) / kShadeSegments * kPrimaryBlock;
+	}
 	HIP_TRY(h, launch_trace_primary(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[2], s));
 	// shade the front-face hits ordered by the grid cell they lie in
-	HIP_TRY(h, launch_key_sort(h->shade_keys, h->shade_list, h->shade_count, uint64_t(max_slots) * g.n, 3 * key_bits,
-							   h->sort_counts, h->sort_keys, h->sort_list, s));
+	HIP_TRY(h, launch_key_sort(SortJob{h->shade_keys, h->shade_list, h->shade_count, h->sort_counts, h->sort_keys,
+									   h->sort_list, h->shade_count + kShadeTotalSlot, 0u, kShadeSegments, a.seg_cap, 0u},
+							   uint64_t(max_slots) * g.n, 3 * key_bits, s));
 	HIP_TRY(h, launch_shade(a, s));
 	if(h->light.sky_model == DDGI_SKY_ATMOSPHERE) HIP_TRY(h, launch_sky(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
@@ -1150,8 +1157,9 @@ ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uin
 		a.chunk = uint32_t(std::min<uint64_t>(kChunk, count - first));
 		HIP_TRY(h, launch_clear(h->refl_count, 1, s));
 		HIP_TRY(h, launch_reflection_trace(a, s));
-		HIP_TRY(h, launch_key_sort(h->refl_keys, h->refl_list, h->refl_count, a.chunk, 3 * key_bits, h->refl_counts,
-								   h->refl_bkeys, h->refl_blist, s));
+		HIP_TRY(h, launch_key_sort(SortJob{h->refl_keys, h->refl_list, h->refl_count, h->refl_counts, h->refl_bkeys,
+										   h->refl_blist, nullptr, 0u, 1u, 0u, 0u},
+								   a.chunk, 3 * key_bits, s));
 		HIP_TRY(h, launch_reflection_shade(a, s));
 	}
 	h->last_stream = s;

@@ -212,8 +212,9 @@ void fill_volume(const Handle& h, const Rot& R, BatchVolume& v) {
 	shade_key_layout(g, &key_shift, &key_bits);
 	a.shade_keys = h.shade_keys;
 	a.key_shift = key_shift;
+	a.seg_cap = 0;  // one shade list per volume
 	v.sort = SortJob{h.shade_keys, h.shade_list, h.shade_count, h.sort_counts, h.sort_keys, h.sort_list,
-					 sort_fine_bits(3 * key_bits), 0u};
+					 h.shade_count + kShadeTotalSlot, sort_fine_bits(3 * key_bits), 1u, 0u, 0u};
 	BlendArgs& bl = v.blend;
 	bl.grid = g;
 	bl.indices = h.indices;

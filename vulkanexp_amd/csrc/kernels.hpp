@@ -71,10 +71,16 @@ struct SceneDev {
 
 // Internal hit record: (t, slot | back << 31, u, v); slot = triangle position in the BVH array,
 // 0xFFFFFFFF = miss (t = -1).
-// The miss list's counter sits a cache line after the shade list's: every wavefront of the primary
-// pass appends to both, and atomics on one line serialise.
-constexpr uint32_t kMissCountSlot = 32;
-constexpr uint32_t kListCounterBytes = 256;
+// List counters, each on its own 128-byte line (device-scope atomics on one line serialise): the
+// primary pass appends its front-face hits to kShadeSegments segments of the shade list, segment s
+// counted at [kCounterStride * s] (workgroup b appends to segment b % kShadeSegments, so each XCD
+// has its own counter); the miss list is counted at [kMissCountSlot]; the key sort stores the
+// ordered list's length at [kShadeTotalSlot], which the shading reads.
+constexpr uint32_t kShadeSegments = 8;
+constexpr uint32_t kCounterStride = 32;
+constexpr uint32_t kMissCountSlot = kCounterStride * kShadeSegments;
+constexpr uint32_t kShadeTotalSlot = kMissCountSlot + kCounterStride;
+constexpr uint32_t kListCounterBytes = (kShadeTotalSlot + kCounterStride) * 4;
 
 struct TraceFrameArgs {
 	SceneDev scene;
@@ -90,7 +96,7 @@ struct TraceFrameArgs {
 	float4* hits;
 	float4* rays;              // (rgb, depth)
 	uint32_t* shade_list;      // ray positions (slot * n + k) of the front-face hits, filled by the primary pass
-	uint32_t* shade_count;     // [0]: its length, [kMissCountSlot]: the miss list's (both cleared before the primary pass)
+	uint32_t* shade_count;     // list counters (kShadeSegments segments, miss list, ordered total)
 	uint32_t* miss_list;       // atmospheric sky only: ray positions of the misses
 	uint32_t max_slots;        // grid sized for this many slots
 	// optional (statistics runs only): [0]/[1] primary node / triangle fetches, [2]/[3] shadow
@@ -103,6 +109,8 @@ struct TraceFrameArgs {
 	// (cell coordinates shifted right by key_shift, so that they stay below 64 and keys below 2^18)
 	uint32_t* shade_keys;
 	uint32_t key_shift;
+	// > 0: the shade list is kShadeSegments segments of seg_cap entries (k_trace_primary); 0: one list
+	uint32_t seg_cap;
 };
 constexpr uint32_t kShadeSortCounts = 1024 * 128 + 1024;  // (list tile, coarse digit) counters + digit totals
 
@@ -184,17 +192,41 @@ struct ReflectionArgs {
 	uint32_t key_shift;
 };
 
+// A list to order by key (launch_key_sort): keys, entries and the device length; scratch: counts
+// (kShadeSortCounts) and bucket_keys / bucket_list (the list's capacity each).
+struct SortJob {
+	const uint32_t* keys;
+	uint32_t* list;
+	const uint32_t* count;  // segment s's length at count[kCounterStride * s]
+	uint32_t* counts;
+	uint32_t* bucket_keys;
+	uint32_t* bucket_list;
+	uint32_t* total;        // optional: receives the list's length
+	uint32_t fine_bits;     // key bits below the 10-bit coarse digit
+	uint32_t segments;      // the input list: `segments` segments of seg_cap entries (1: contiguous)
+	uint32_t seg_cap;
+	uint32_t pad;
+};
+// The sort kernels' argument: one job, or (many != nullptr) job y of workgroup (x, y) at
+// many + y * stride.
+struct SortJobs {
+	SortJob one;
+	const char* many;
+	uint32_t stride;
+};
+uint32_t sort_fine_bits(uint32_t key_bits);
+
 // ---- launchers (return hipGetLastError of the launch) ----
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);
-// Orders `list` (keys `keys` of key_bits <= 18 bits, length *count <= capacity on the device) in
-// place: by the keys' high 10 bits over the whole list, and by the whole key inside each fixed
-// chunk of 4096 entries (a bucket of equal high bits longer than a chunk is ordered chunk by
-// chunk); entries with equal keys come in any order.  Scratch: `counts` (kShadeSortCounts),
-// `bucket_keys` and `bucket_list` (capacity each).
-hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t* count, uint64_t capacity,
-						   uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys, uint32_t* bucket_list, hipStream_t s);
+// Orders job.list (keys job.keys of key_bits <= 18 bits; input in job.segments segments, output
+// contiguous, at most `capacity` entries) in place: by the keys' high 10 bits over the whole list,
+// and by the whole key inside each fixed chunk of 4096 entries (a bucket of equal high bits longer
+// than a chunk is ordered chunk by chunk); entries with equal keys come in any order.  Scratch:
+// job.counts (kShadeSortCounts), job.bucket_keys and job.bucket_list (capacity each).  The
+// caller's job.fine_bits is ignored (set from key_bits).
+hipError_t launch_key_sort(SortJob job, uint64_t capacity, uint32_t key_bits, hipStream_t s);
 // Cell-key layout of the shade ordering for a grid: coordinate shift and bits per axis.
 void shade_key_layout(const GridConst& g, uint32_t* shift, uint32_t* bits_per_axis);
 hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s);
@@ -235,26 +267,6 @@ constexpr uint32_t kBlendFP32 = 0;
 constexpr uint32_t kBlendBF16 = 1;
 constexpr uint32_t kBlendFP16 = 2;
 
-// A list to order by key (launch_key_sort): keys, entries and the device length; scratch: counts
-// (kShadeSortCounts) and bucket_keys / bucket_list (the list's capacity each).
-struct SortJob {
-	const uint32_t* keys;
-	uint32_t* list;
-	const uint32_t* count;
-	uint32_t* counts;
-	uint32_t* bucket_keys;
-	uint32_t* bucket_list;
-	uint32_t fine_bits;  // key bits below the 10-bit coarse digit
-	uint32_t pad;
-};
-// The sort kernels' argument: one job, or (many != nullptr) job y of workgroup (x, y) at
-// many + y * stride.
-struct SortJobs {
-	SortJob one;
-	const char* many;
-	uint32_t stride;
-};
-uint32_t sort_fine_bits(uint32_t key_bits);
 
 // ---- batched update of many probe volumes (one graph launch per update) ----
 constexpr uint32_t kMaxBatchVolumes = 256;

@@ -47,7 +47,8 @@ __global__ void k_clear(uint32_t* __restrict__ words, uint32_t n) {
 __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __restrict__ dirs, uint32_t count,
 							 uint32_t* zero) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if(zero && i == 0) zero[0] = zero[kMissCountSlot] = 0u;
+	if(zero && i < kShadeSegments) zero[kCounterStride * i] = 0u;
+	if(zero && i == 0) zero[kMissCountSlot] = 0u;
 	if(i >= count) return;
 	dirs[i] = frame_dir(R, fib[i]);
 }
@@ -91,10 +92,13 @@ __device__ __forceinline__ void finish_primary_ray(const TraceFrameArgs& a, uint
 	a.hits[ri] = packed;
 	const uint32_t code = __float_as_uint(packed.y);
 	const bool front = code != 0xFFFFFFFFu && !(code & 0x80000000u);
-	const uint32_t pos = wave_append(a.shade_count, front);  // front-face hits are queued for shading
+	// front-face hits are queued for shading, in this workgroup's segment of the list
+	const uint32_t seg = a.seg_cap ? blockIdx.x % kShadeSegments : 0u;
+	const size_t base = size_t(seg) * a.seg_cap;
+	const uint32_t pos = wave_append(a.shade_count + kCounterStride * seg, front);
 	if(front) {
-		a.shade_list[pos] = ri;
-		if(a.shade_keys) a.shade_keys[pos] = key;
+		a.shade_list[base + pos] = ri;
+		if(a.shade_keys) a.shade_keys[base + pos] = key;
 	}
 	const bool miss = code == 0xFFFFFFFFu;
 	if(a.light.sky_model == DDGI_SKY_ATMOSPHERE) {  // misses are queued for the sky kernel
@@ -238,8 +242,8 @@ __device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const 
 // the order changes no output, and entries with equal keys may come in any order.
 //
 // Keys of up to 18 bits = a coarse digit (the high 10 bits) and a fine digit (the rest):
-//  1. k_sort_count: kSortTiles workgroups, one per contiguous tile of the list, count the coarse
-//     digits of their tile in LDS and store them as counts[tile][digit];
+//  1. k_sort_count: kSortTiles workgroups, one per contiguous tile of the list (of one of its
+//     segments), count the coarse digits of their tile in LDS and store them as counts[tile][digit];
 //  2. k_sort_columns: per digit, the exclusive prefix sum over the tiles in place, and the digit's
 //     total;
 //  3. k_sort_bucket: every tile moves its entries into their coarse buckets: a run starts at the
@@ -278,10 +282,20 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_1024(uint32_t v, uint32
 	return before + inc - v;
 }
 
-// Tile b of the list: entries [lo, hi).
-__device__ __forceinline__ void sort_tile(uint32_t count, uint32_t b, uint32_t& lo, uint32_t& hi) {
-	const uint32_t per = (count + kSortTiles - 1u) / kSortTiles;
-	lo = min(count, b * per);
+// Length of a job's input list (the sum of its segments).
+__device__ __forceinline__ uint32_t job_length(const SortJob& j) {
+	uint32_t n = 0;
+	for(uint32_t q = 0; q < j.segments; ++q) n += __builtin_amdgcn_readfirstlane(j.count[kCounterStride * q]);
+	return n;
+}
+// Tile b of a job's input: entries [lo, hi) of segment `seg` (kSortTiles / segments tiles per
+// segment; the tiles only have to partition the entries, in any order).
+__device__ __forceinline__ void sort_tile(const SortJob& j, uint32_t b, size_t& base, uint32_t& lo, uint32_t& hi) {
+	const uint32_t per_seg = kSortTiles / j.segments, seg = b / per_seg, sub = b - seg * per_seg;
+	const uint32_t count = seg < j.segments ? __builtin_amdgcn_readfirstlane(j.count[kCounterStride * seg]) : 0u;
+	const uint32_t per = (count + per_seg - 1u) / per_seg;
+	base = size_t(seg) * j.seg_cap;
+	lo = min(count, sub * per);
 	hi = min(count, lo + per);
 }
 
@@ -294,21 +308,21 @@ __device__ __forceinline__ SortJob sort_job(const SortJobs& J) {
 __global__ __launch_bounds__(1024) void k_sort_count(SortJobs J) {
 	const SortJob j = sort_job(J);
 	const uint32_t* __restrict__ keys = j.keys;
-	const uint32_t* __restrict__ count_ptr = j.count;
 	const uint32_t fine_bits = j.fine_bits;
 	uint32_t* __restrict__ counts = j.counts;
 	__shared__ uint32_t s_cnt[kSortDigits];
 	const uint32_t t = threadIdx.x;
 	s_cnt[t] = 0u;
 	__syncthreads();
+	size_t base;
 	uint32_t lo, hi;
-	sort_tile(__builtin_amdgcn_readfirstlane(*count_ptr), blockIdx.x, lo, hi);
+	sort_tile(j, blockIdx.x, base, lo, hi);
 	for(uint32_t i0 = lo; i0 < hi; i0 += 1024u * kSortUnroll) {
 		uint32_t k[kSortUnroll];
 #pragma unroll
 		for(uint32_t u = 0; u < kSortUnroll; ++u) {
 			const uint32_t i = i0 + 1024u * u + t;
-			k[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+			k[u] = i < hi ? keys[base + i] : 0xFFFFFFFFu;
 		}
 #pragma unroll
 		for(uint32_t u = 0; u < kSortUnroll; ++u)
@@ -324,6 +338,9 @@ __global__ __launch_bounds__(256) void k_sort_columns(SortJobs J) {
 	const SortJob j = sort_job(J);
 	uint32_t* __restrict__ counts = j.counts;
 	uint32_t* __restrict__ totals = j.counts + kSortTiles * kSortDigits;
+	if(j.total && blockIdx.x == 0 && threadIdx.x == 0) {  // the ordered list's length, for the shading
+		*j.total = job_length(j);
+	}
 	constexpr uint32_t kQ = kSortTiles / 4u;
 	__shared__ uint32_t s_part[4][64];
 	const uint32_t lane = __lane_id(), q = threadIdx.x >> 6;
@@ -350,7 +367,6 @@ __global__ __launch_bounds__(1024) void k_sort_bucket(SortJobs J) {
 	const SortJob j = sort_job(J);
 	const uint32_t* __restrict__ keys = j.keys;
 	const uint32_t* __restrict__ list = j.list;
-	const uint32_t* __restrict__ count_ptr = j.count;
 	const uint32_t fine_bits = j.fine_bits;
 	const uint32_t* __restrict__ counts = j.counts;
 	const uint32_t* __restrict__ totals = j.counts + kSortTiles * kSortDigits;
@@ -363,15 +379,16 @@ __global__ __launch_bounds__(1024) void k_sort_bucket(SortJobs J) {
 	const uint32_t digit_start = block_exclusive_scan_1024(totals[t], s_wave, all);
 	s_pos[t] = digit_start + counts[blockIdx.x * kSortDigits + t];
 	__syncthreads();
+	size_t base;
 	uint32_t lo, hi;
-	sort_tile(__builtin_amdgcn_readfirstlane(*count_ptr), blockIdx.x, lo, hi);
+	sort_tile(j, blockIdx.x, base, lo, hi);
 	for(uint32_t i0 = lo; i0 < hi; i0 += 1024u * kSortUnroll) {
 		uint32_t k[kSortUnroll], e[kSortUnroll];
 #pragma unroll
 		for(uint32_t u = 0; u < kSortUnroll; ++u) {
 			const uint32_t i = i0 + 1024u * u + t;
-			k[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
-			e[u] = i < hi ? list[i] : 0u;
+			k[u] = i < hi ? keys[base + i] : 0xFFFFFFFFu;
+			e[u] = i < hi ? list[base + i] : 0u;
 		}
 #pragma unroll
 		for(uint32_t u = 0; u < kSortUnroll; ++u) {
@@ -392,14 +409,13 @@ __global__ __launch_bounds__(1024) void k_sort_fine(SortJobs J) {
 	const SortJob j = sort_job(J);
 	const uint32_t* __restrict__ bucket_keys = j.bucket_keys;
 	const uint32_t* __restrict__ bucket_list = j.bucket_list;
-	const uint32_t* __restrict__ count_ptr = j.count;
 	const uint32_t fine_bits = j.fine_bits;
 	uint32_t* __restrict__ out = j.list;
 	constexpr uint32_t kU = kFineChunk / 1024u;
 	__shared__ uint32_t s_cnt[1024];
 	__shared__ uint32_t s_wave[16];
 	const uint32_t t = threadIdx.x;
-	const uint32_t count = __builtin_amdgcn_readfirstlane(*count_ptr);
+	const uint32_t count = job_length(j);
 	const uint32_t c0 = blockIdx.x * kFineChunk;
 	if(c0 >= count) return;  // uniform over the workgroup
 	const uint32_t c1 = min(count, c0 + kFineChunk);
@@ -437,7 +453,7 @@ __global__ __launch_bounds__(1024) void k_sort_fine(SortJobs J) {
 // record: the order of the list does not change any output.
 template <int STACK, bool COUNT>
 __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, uint32_t* stack, int stride) {
-	if(k >= *a.shade_count) return;
+	if(k >= a.shade_count[kShadeTotalSlot]) return;
 	const uint32_t n = a.grid.n;
 	const uint32_t ri = a.shade_list[k];  // hits and rays are stored in thread (ray_order) order
 	const uint32_t slot = ri / n, j = ri - slot * n;
@@ -479,7 +495,7 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kShadeBlock, 5) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kShadeBlock];
-	const uint32_t count = *a.shade_count;
+	const uint32_t count = a.shade_count[kShadeTotalSlot];
 	const uint32_t stride = gridDim.x * kShadeBlock;
 	for(uint32_t k = blockIdx.x * kShadeBlock + threadIdx.x; k < count; k += stride)
 		shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kShadeBlock);
@@ -1131,11 +1147,13 @@ static hipError_t launch_sort_jobs(const SortJobs& J, uint32_t jobs, uint64_t ca
 
 uint32_t sort_fine_bits(uint32_t key_bits) { return key_bits > kSortCoarseBits ? key_bits - kSortCoarseBits : 0u; }
 
-hipError_t launch_key_sort(const uint32_t* keys, uint32_t* list, const uint32_t* count, uint64_t capacity,
-						   uint32_t key_bits, uint32_t* counts, uint32_t* bucket_keys, uint32_t* bucket_list, hipStream_t s) {
-	if(key_bits > kSortCoarseBits + 8u || capacity > 0xFFFFFFFFull) return hipErrorInvalidValue;
+hipError_t launch_key_sort(SortJob job, uint64_t capacity, uint32_t key_bits, hipStream_t s) {
+	if(key_bits > kSortCoarseBits + 8u || capacity > 0xFFFFFFFFull || job.segments == 0 ||
+	   job.segments > kShadeSegments || kSortTiles % job.segments != 0)
+		return hipErrorInvalidValue;
+	job.fine_bits = sort_fine_bits(key_bits);
 	SortJobs J{};
-	J.one = SortJob{keys, list, count, counts, bucket_keys, bucket_list, sort_fine_bits(key_bits), 0u};
+	J.one = job;
 	return launch_sort_jobs(J, 1, capacity, s);
 }
 
