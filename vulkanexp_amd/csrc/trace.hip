@@ -6,7 +6,6 @@
 //
 // Built with -ffp-contract=off (see device_common.hpp).
 #include <algorithm>
-#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -490,15 +489,16 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 	}
 }
 
-// Grid-stride over the shade list (its length is known only on the device).  At least 5 waves per
-// SIMD: the shadow traversal is latency-bound, and the register budget of 5 waves costs no spills.
+// One thread per entry of the ordered shade list, over a grid sized for the list's capacity (its
+// length is known only on the device; workgroups past it end at once).  The hardware refills a CU
+// as soon as one of its workgroups ends, so the CUs stay busy to the end of the list, which a
+// resident grid striding over the list does not do.  At least 5 waves per SIMD: the shadow
+// traversal is latency-bound, and the register budget of 5 waves costs no spills.
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kShadeBlock, 5) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kShadeBlock];
-	const uint32_t count = a.shade_count[kShadeTotalSlot];
-	const uint32_t stride = gridDim.x * kShadeBlock;
-	for(uint32_t k = blockIdx.x * kShadeBlock + threadIdx.x; k < count; k += stride)
-		shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kShadeBlock);
+	const uint32_t k = blockIdx.x * kShadeBlock + threadIdx.x;
+	if(k < a.shade_count[kShadeTotalSlot]) shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kShadeBlock);
 }
 
 // ---- misses under the atmospheric sky (miss.rmiss:17-23 with sky.glsl:60-127) ----
@@ -854,7 +854,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_reflection_trace(ReflectionArgs
 	}
 }
 
-// Grid-stride over the key-ordered queue of front-face reflection hits.
+// One thread per entry of the key-ordered queue of front-face reflection hits, over a grid sized
+// for the chunk (as k_shade).
 template <int STACK>
 __global__ __launch_bounds__(kShadeBlock, 5) void k_reflection_shade(ReflectionArgs a) {
 	__shared__ uint32_t stack[STACK * kShadeBlock];
@@ -1096,37 +1097,6 @@ __global__ void k_sample(GridConst g, const uint32_t* __restrict__ states, const
 
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
 
-// Workgroups of `kernel` that fit on the current device at once (CUs x resident per CU), cached per
-// (device, kernel): the occupancy query costs more host time than a small launch.
-hipError_t resident_blocks(const void* kernel, uint32_t block, uint64_t* out) {
-	struct Entry {
-		int device;
-		const void* kernel;
-		uint64_t blocks;
-	};
-	static std::mutex mu;
-	static std::vector<Entry> cache;
-	int dev = 0;
-	hipError_t e = hipGetDevice(&dev);
-	if(e != hipSuccess) return e;
-	{
-		std::lock_guard<std::mutex> lock(mu);
-		for(const Entry& c : cache)
-			if(c.device == dev && c.kernel == kernel) {
-				*out = c.blocks;
-				return hipSuccess;
-			}
-	}
-	int cus = 0, per_cu = 0;
-	e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	if(e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, int(block), 0);
-	if(e != hipSuccess) return e;
-	*out = uint64_t(std::max(per_cu, 1)) * uint64_t(std::max(cus, 1));
-	std::lock_guard<std::mutex> lock(mu);
-	cache.push_back(Entry{dev, kernel, *out});
-	return hipSuccess;
-}
-
 }  // namespace
 
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
@@ -1193,9 +1163,9 @@ hipError_t launch_trace_primary(const TraceFrameArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_sky(const TraceFrameArgs& a, hipStream_t s) {
-	// grid-stride: the miss count is known only on the device; enough blocks for every CU
+	// one thread per entry the miss list can hold (the miss count is known only on the device)
 	const uint32_t cap = a.max_slots * a.grid.n;
-	hipLaunchKernelGGL(k_sky, dim3(std::max(1u, std::min(blocks_for(cap, 256), 2048u))), dim3(256), 0, s, a);
+	hipLaunchKernelGGL(k_sky, dim3(std::max(1u, blocks_for(cap, 256))), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -1205,11 +1175,7 @@ hipError_t launch_shade(const TraceFrameArgs& a, hipStream_t s) {
 	const bool cnt = a.visit_counters != nullptr;
 	const void* f = cnt ? reinterpret_cast<const void*>(&k_shade<kLdsStack, true>)
 						: reinterpret_cast<const void*>(&k_shade<kLdsStack, false>);
-	// grid-stride: as many workgroups as fit on the device at once (the list length is on the device)
-	uint64_t resident = 0;
-	const hipError_t e = resident_blocks(f, kShadeBlock, &resident);
-	if(e != hipSuccess) return e;
-	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(threads, kShadeBlock))));
+	const dim3 grid(blocks_for(threads, kShadeBlock));  // one thread per entry the list can hold
 	TraceFrameArgs arg = a;
 	void* args[] = {&arg};
 	return hipLaunchKernel(f, grid, dim3(kShadeBlock), args, 0, s);
@@ -1271,14 +1237,8 @@ hipError_t launch_reflection_trace(const ReflectionArgs& a, hipStream_t s) {
 
 hipError_t launch_reflection_shade(const ReflectionArgs& a, hipStream_t s) {
 	if(a.chunk == 0) return hipSuccess;
-	const void* f = reinterpret_cast<const void*>(&k_reflection_shade<kLdsStack>);
-	uint64_t resident = 0;  // grid-stride: as many workgroups as fit on the device at once
-	const hipError_t e = resident_blocks(f, kShadeBlock, &resident);
-	if(e != hipSuccess) return e;
-	const dim3 grid(uint32_t(std::min<uint64_t>(resident, blocks_for(a.chunk, kShadeBlock))));
-	ReflectionArgs arg = a;
-	void* args[] = {&arg};
-	return hipLaunchKernel(f, grid, dim3(kShadeBlock), args, 0, s);
+	hipLaunchKernelGGL(k_reflection_shade<kLdsStack>, dim3(blocks_for(a.chunk, kShadeBlock)), dim3(kShadeBlock), 0, s, a);
+	return hipGetLastError();
 }
 
 hipError_t launch_init_probes(const SceneDev& sc, const GridConst& g, const float4* dirs512, uint32_t* states,
