@@ -44,6 +44,8 @@ struct Batch {
 	int ring_pos = 0;
 	BatchItem* d_trace_items = nullptr;
 	size_t trace_cap = 0;
+	BatchItem* d_shade_items = nullptr;
+	size_t shade_cap = 0;
 	BatchItem* d_blend_items = nullptr;
 	size_t blend_cap = 0;
 	BatchControl* d_ctl = nullptr;
@@ -118,16 +120,20 @@ ddgi_status build_items(Batch* b, uint32_t mode, BatchLaunch& L) {
 	});
 	const uint32_t per_item = b->cfg.trace_block * b->cfg.trace_item_blocks;
 	const uint32_t probes = mode == kBlendFP32 ? b->cfg.blend_probes : kMfmaProbes;
-	std::vector<BatchItem> ti, bi;
+	std::vector<BatchItem> ti, si, bi;
 	for(uint32_t v : order) {
 		const Handle* h = b->vols[v];
-		const uint64_t threads = uint64_t(h->P) * h->params.rays_per_probe;
+		const uint64_t threads = uint64_t(h->P) * h->params.rays_per_probe;  // also the shade list's capacity
 		for(uint64_t f = 0; f < threads; f += per_item) ti.push_back(BatchItem{v, uint32_t(f)});
+		for(uint64_t f = 0; f < threads; f += kShadeBlock) si.push_back(BatchItem{v, uint32_t(f)});
 		for(uint32_t s = 0; s < h->P; s += probes) bi.push_back(BatchItem{v, s});
 	}
 	ddgi_status st = ensure_items(&b->d_trace_items, &b->trace_cap, ti.size());
-	if(st != DDGI_OK || (st = ensure_items(&b->d_blend_items, &b->blend_cap, bi.size())) != DDGI_OK) return st;
+	if(st != DDGI_OK || (st = ensure_items(&b->d_shade_items, &b->shade_cap, si.size())) != DDGI_OK ||
+	   (st = ensure_items(&b->d_blend_items, &b->blend_cap, bi.size())) != DDGI_OK)
+		return st;
 	BHIP_TRY(hipMemcpy(b->d_trace_items, ti.data(), ti.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+	BHIP_TRY(hipMemcpy(b->d_shade_items, si.data(), si.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
 	BHIP_TRY(hipMemcpy(b->d_blend_items, bi.data(), bi.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
 	L.vols = b->d_vols;
 	L.trace_items = b->d_trace_items;
@@ -139,6 +145,8 @@ ddgi_status build_items(Batch* b, uint32_t mode, BatchLaunch& L) {
 	L.blend_item_count = uint32_t(bi.size());
 	L.blend_probes = probes;
 	L.blend_mode = mode;
+	L.shade_items = b->d_shade_items;
+	L.shade_item_count = uint32_t(si.size());
 	return DDGI_OK;
 }
 
@@ -152,12 +160,10 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	BatchLaunch L{};
 	ddgi_status st = build_items(b, mode, L);
 	if(st != DDGI_OK) return st;
-	int r_primary = 0, r_shade = 0, r_blend = 0;
+	int r_primary = 0, r_blend = 0;
 	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, false, &r_primary));
-	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, true, &r_shade));
 	BHIP_TRY(batch_blend_resident(mode, L.blend_probes, &r_blend));
 	const uint32_t g_primary = uint32_t(std::max(1, r_primary) * b->cus);
-	const uint32_t g_shade = uint32_t(std::max(1, r_shade) * b->cus);
 	const uint32_t g_blend = uint32_t(std::max(1, r_blend) * b->cus);
 	BHIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
 	hipError_t e = launch_batch_begin(L, b->cap);
@@ -166,7 +172,7 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	for(const Handle* h : b->vols) max_list = std::max<uint64_t>(max_list, uint64_t(h->P) * h->params.rays_per_probe);
 	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, false, std::min(g_primary, L.trace_item_count), b->cap);
 	if(e == hipSuccess) e = launch_batch_sort(L, max_list, b->cap);
-	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, true, std::min(g_shade, L.trace_item_count), b->cap);
+	if(e == hipSuccess) e = launch_batch_shade(L, b->cap);
 	if(e == hipSuccess) e = launch_batch_blend(L, std::min(g_blend, L.blend_item_count), b->cap);
 	hipGraph_t graph = nullptr;
 	const hipError_t e_end = hipStreamEndCapture(b->cap, &graph);
@@ -248,6 +254,7 @@ void destroy_batch(Batch* b) {
 	for(Handle* h : b->vols) --h->batch_members;
 	(void)hipFree(b->d_vols);
 	(void)hipFree(b->d_trace_items);
+	(void)hipFree(b->d_shade_items);
 	(void)hipFree(b->d_blend_items);
 	(void)hipFree(b->d_ctl);
 	if(b->h_ring) (void)hipHostFree(b->h_ring);

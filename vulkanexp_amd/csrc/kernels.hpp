@@ -312,6 +312,8 @@ struct BatchLaunch {
 	uint32_t blend_item_count;
 	uint32_t blend_probes;        // slots per blend item
 	uint32_t blend_mode;
+	const BatchItem* shade_items;  // one workgroup of kShadeBlock shade-list entries each
+	uint32_t shade_item_count;
 };
 
 hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s);
@@ -320,6 +322,8 @@ hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, bool shade, 
 // Orders the shade list of every volume (BatchVolume::sort) in one set of launches; max_capacity
 // = the largest list capacity.
 hipError_t launch_batch_sort(const BatchLaunch& b, uint64_t max_capacity, hipStream_t s);
+// Shades every volume's ordered shade list: one workgroup per shade item, one thread per entry.
+hipError_t launch_batch_shade(const BatchLaunch& b, hipStream_t s);
 hipError_t launch_batch_blend(const BatchLaunch& b, uint32_t grid, hipStream_t s);
 // Resident workgroups per CU of the persistent kernels (for sizing their grids).
 hipError_t batch_trace_resident(uint32_t block, bool shade, int* per_cu);

@@ -567,6 +567,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_trace(BatchLaunch b) {
 	}
 }
 
+// Shading of every volume's ordered shade list, one thread per entry that the list can hold (as
+// k_shade): workgroup i takes the kShadeBlock entries of shade item i; entries past a list's length
+// return at once.
+template <int STACK>
+__global__ __launch_bounds__(kShadeBlock, 5) void k_batch_shade(BatchLaunch b) {
+	__shared__ uint32_t stack[STACK * kShadeBlock];
+	const uint32_t vol = __builtin_amdgcn_readfirstlane(b.shade_items[blockIdx.x].volume);
+	const uint32_t first = __builtin_amdgcn_readfirstlane(b.shade_items[blockIdx.x].first);
+	shade_ray<STACK, false>(b.vols[vol].trace, first + threadIdx.x, stack + threadIdx.x, kShadeBlock);
+}
+
 // ---- arbitrary ray queries (public ddgi_hit records) ----
 template <int STACK, bool ANY, bool CULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneDev sc, const ddgi_ray* __restrict__ rays,
@@ -1295,6 +1306,12 @@ hipError_t batch_trace_resident(uint32_t block, bool shade, int* per_cu) {
 	const void* f = shade ? batch_trace_kernel<true>(block) : batch_trace_kernel<false>(block);
 	if(!f) return hipErrorInvalidValue;
 	return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, int(block), 0);
+}
+
+hipError_t launch_batch_shade(const BatchLaunch& b, hipStream_t s) {
+	if(b.shade_item_count == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_batch_shade<kLdsStack>, dim3(b.shade_item_count), dim3(kShadeBlock), 0, s, b);
+	return hipGetLastError();
 }
 
 hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, bool shade, uint32_t grid, hipStream_t s) {
