@@ -403,7 +403,7 @@ ddgi_status ddgi_synchronize(void* handle);
  * on each handle.  The handles stay usable on their own; ddgi_destroy is refused while a batch
  * holds the handle.  Errors of the batch calls are reported by ddgi_last_error(NULL). */
 typedef struct ddgi_batch_config {
-	uint32_t trace_block;       /* traversal workgroup size: 64, 128 or 256 threads */
+	uint32_t trace_block;       /* primary traversal workgroup size: 64, 128 or 256 threads */
 	uint32_t trace_item_blocks; /* work item = trace_block * trace_item_blocks consecutive rays of one volume (1..64) */
 	uint32_t blend_probes;      /* FP32 blend tile: probes per workgroup, 4, 8, 16 or 32 (MFMA blends use 16) */
 	uint32_t reserved[5];

@@ -8,7 +8,7 @@
 //   k_batch_weights  blend weights of every volume (grid: 256 rows x volumes)
 //   k_batch_trace    persistent primary traversal over the work items of all volumes
 //   k_sort_*         (four kernels, grid y = volume) every volume's shade list ordered by hit cell
-//   k_batch_trace    persistent shading + shadow rays over the same items
+//   k_batch_shade    shading + shadow rays, one thread per entry every volume's list can hold
 //   k_batch_blend    persistent blend + border copy over the probe tiles of all volumes
 // The graph's kernel arguments are fixed; what changes per update (orientation, which atlas is
 // exposed, hysteresis, light, scene pointers) lives in a device array of BatchVolume records that
@@ -161,7 +161,7 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	ddgi_status st = build_items(b, mode, L);
 	if(st != DDGI_OK) return st;
 	int r_primary = 0, r_blend = 0;
-	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, false, &r_primary));
+	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, &r_primary));
 	BHIP_TRY(batch_blend_resident(mode, L.blend_probes, &r_blend));
 	const uint32_t g_primary = uint32_t(std::max(1, r_primary) * b->cus);
 	const uint32_t g_blend = uint32_t(std::max(1, r_blend) * b->cus);
@@ -170,7 +170,7 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	if(e == hipSuccess) e = launch_batch_weights(L, b->cap);
 	uint64_t max_list = 0;
 	for(const Handle* h : b->vols) max_list = std::max<uint64_t>(max_list, uint64_t(h->P) * h->params.rays_per_probe);
-	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, false, std::min(g_primary, L.trace_item_count), b->cap);
+	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, std::min(g_primary, L.trace_item_count), b->cap);
 	if(e == hipSuccess) e = launch_batch_sort(L, max_list, b->cap);
 	if(e == hipSuccess) e = launch_batch_shade(L, b->cap);
 	if(e == hipSuccess) e = launch_batch_blend(L, std::min(g_blend, L.blend_item_count), b->cap);

@@ -541,14 +541,14 @@ __global__ __launch_bounds__(512) void k_batch_begin(BatchLaunch b) {
 	}
 }
 
-// Persistent traversal over the work items of all volumes: each workgroup takes the next item
-// (a run of consecutive traversal threads of one volume) from a queue until the queue is empty.
-// Every workgroup leaves once the head passes the item count, so the grid always drains.
-template <int STACK, int BLOCK, bool SHADE>
+// Persistent primary traversal over the work items of all volumes: each workgroup takes the next
+// item (a run of consecutive traversal threads of one volume) from a queue until the queue is
+// empty.  Every workgroup leaves once the head passes the item count, so the grid always drains.
+template <int STACK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch_trace(BatchLaunch b) {
 	__shared__ uint32_t stack[STACK * BLOCK];
 	__shared__ uint32_t s_item;
-	uint32_t* head = &b.ctl->next[SHADE ? 1 : 0];
+	uint32_t* head = &b.ctl->next[0];
 	// bounded: a workgroup can take at most every item once, plus the fetch that ends the loop
 	for(uint32_t round = 0; round <= b.trace_item_count; ++round) {
 		if(threadIdx.x == 0) s_item = atomicAdd(head, 1u);
@@ -560,10 +560,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_trace(BatchLaunch b) {
 		const uint32_t first = __builtin_amdgcn_readfirstlane(b.trace_items[it].first);
 		const BatchVolume& v = b.vols[vol];
 		const uint32_t end = min(first + b.trace_item_threads, v.threads);
-		for(uint32_t g = first + threadIdx.x; g < end; g += BLOCK) {
-			if(SHADE) shade_ray<STACK, false>(v.trace, g, stack + threadIdx.x, BLOCK);
-			else trace_primary_ray<STACK, false>(v.trace, g, stack + threadIdx.x, BLOCK);
-		}
+		for(uint32_t g = first + threadIdx.x; g < end; g += BLOCK)
+			trace_primary_ray<STACK, false>(v.trace, g, stack + threadIdx.x, BLOCK);
 	}
 }
 
@@ -1285,12 +1283,11 @@ hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float
 // ---- batched update ----
 namespace {
 
-template <bool SHADE>
 const void* batch_trace_kernel(uint32_t block) {
 	switch(block) {
-		case 64: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 64, SHADE>);
-		case 128: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 128, SHADE>);
-		case 256: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 256, SHADE>);
+		case 64: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 64>);
+		case 128: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 128>);
+		case 256: return reinterpret_cast<const void*>(&k_batch_trace<kLdsStack, 256>);
 	}
 	return nullptr;
 }
@@ -1302,8 +1299,8 @@ hipError_t launch_batch_begin(const BatchLaunch& b, hipStream_t s) {
 	return hipGetLastError();
 }
 
-hipError_t batch_trace_resident(uint32_t block, bool shade, int* per_cu) {
-	const void* f = shade ? batch_trace_kernel<true>(block) : batch_trace_kernel<false>(block);
+hipError_t batch_trace_resident(uint32_t block, int* per_cu) {
+	const void* f = batch_trace_kernel(block);
 	if(!f) return hipErrorInvalidValue;
 	return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, int(block), 0);
 }
@@ -1314,8 +1311,8 @@ hipError_t launch_batch_shade(const BatchLaunch& b, hipStream_t s) {
 	return hipGetLastError();
 }
 
-hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, bool shade, uint32_t grid, hipStream_t s) {
-	const void* f = shade ? batch_trace_kernel<true>(block) : batch_trace_kernel<false>(block);
+hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, uint32_t grid, hipStream_t s) {
+	const void* f = batch_trace_kernel(block);
 	if(!f || grid == 0) return hipErrorInvalidValue;
 	BatchLaunch arg = b;
 	void* args[] = {&arg};
