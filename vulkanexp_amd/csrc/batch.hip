@@ -528,7 +528,7 @@ ddgi_status ddgi_batch_autotune(void* batch, const float* orientations, uint32_t
 	float win_ms = 3.4e38f;
 	// phase 0: traversal workgroup size x work-item size
 	const uint32_t blocks[] = {64, 128, 256};
-	const uint32_t items[] = {1, 4, 16};
+	const uint32_t items[] = {8, 16, 32};
 	for(uint32_t bs : blocks)
 		for(uint32_t it : items) {
 			ddgi_batch_config c = start;
