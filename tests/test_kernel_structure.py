@@ -4,9 +4,9 @@ A persistent kernel's workgroup takes items in a loop: thread 0 bumps the queue 
 publishes the item, a second barrier protects the slot.  If the compiler's control-flow
 structurizer moves thread 0's queue access out of the loop that holds those barriers (it did once,
 when thread 0 had a second block at the bottom of the loop), the other threads spin over the
-barriers on a stale item and the grid never drains.  This test compiles trace.hip and blend.hip to
-assembly (hipcc, no GPU needed) and checks, for every persistent kernel instantiation, that the
-queue atomic and the first barriers after it sit in the same outermost loop."""
+barriers on a stale item and the grid never drains.  This test compiles trace.hip to assembly (hipcc,
+no GPU needed) and checks, for every instantiation of the persistent primary traversal of batched
+updates, that the queue atomic and the first barriers after it sit in the same outermost loop."""
 import os
 import re
 import shutil
@@ -55,7 +55,7 @@ def kernel_events(asm: str, name_re: str):
 def asm(tmp_path_factory):
     d = tmp_path_factory.mktemp("asm")
     texts = {}
-    for src, contract in (("trace.hip", "-ffp-contract=off"), ("blend.hip", "-ffp-contract=fast")):
+    for src, contract in (("trace.hip", "-ffp-contract=off"),):
         out = d / (src + ".s")
         subprocess.run([HIPCC, *FLAGS, contract, os.path.join(CSRC, src), "-o", str(out)], check=True,
                        capture_output=True, timeout=600)
@@ -66,8 +66,7 @@ def asm(tmp_path_factory):
 def test_persistent_kernels_fetch_inside_the_barrier_loop(asm):
     kernels = {}
     kernels.update(kernel_events(asm["trace.hip"], "k_batch_trace"))
-    kernels.update(kernel_events(asm["blend.hip"], "k_batch_blend"))
-    assert len(kernels) >= 3 + 6, sorted(kernels)  # 3 primary-traversal block sizes, 6 blend variants
+    assert len(kernels) >= 3, sorted(kernels)  # 3 workgroup sizes
     for name, ev in kernels.items():
         atomics = [i for i, (e, _) in enumerate(ev) if e == "A"]
         assert atomics, f"{name}: no queue atomic"

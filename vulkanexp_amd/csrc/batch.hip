@@ -9,7 +9,7 @@
 //   k_batch_trace    persistent primary traversal over the work items of all volumes
 //   k_sort_*         (four kernels, grid y = volume) every volume's shade list ordered by hit cell
 //   k_batch_shade    shading + shadow rays, one thread per entry every volume's list can hold
-//   k_batch_blend    persistent blend + border copy over the probe tiles of all volumes
+//   k_batch_blend    blend + border copy, one workgroup per probe tile of every volume
 // The graph's kernel arguments are fixed; what changes per update (orientation, which atlas is
 // exposed, hysteresis, light, scene pointers) lives in a device array of BatchVolume records that
 // is copied from a pinned ring of host records before each launch.  Each per-ray and per-probe
@@ -110,7 +110,7 @@ ddgi_status ensure_items(T** buf, size_t* cap, size_t need) {
 }
 
 // Work items of the current configuration.  Volumes are queued largest BVH first (their rays
-// cost the most), so the small volumes fill the tail of each persistent kernel.
+// cost the most), so the small volumes fill the tail of the persistent traversal.
 ddgi_status build_items(Batch* b, uint32_t mode, BatchLaunch& L) {
 	const uint32_t nv = uint32_t(b->vols.size());
 	std::vector<uint32_t> order(nv);
@@ -160,11 +160,9 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	BatchLaunch L{};
 	ddgi_status st = build_items(b, mode, L);
 	if(st != DDGI_OK) return st;
-	int r_primary = 0, r_blend = 0;
+	int r_primary = 0;
 	BHIP_TRY(batch_trace_resident(b->cfg.trace_block, &r_primary));
-	BHIP_TRY(batch_blend_resident(mode, L.blend_probes, &r_blend));
 	const uint32_t g_primary = uint32_t(std::max(1, r_primary) * b->cus);
-	const uint32_t g_blend = uint32_t(std::max(1, r_blend) * b->cus);
 	BHIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
 	hipError_t e = launch_batch_begin(L, b->cap);
 	if(e == hipSuccess) e = launch_batch_weights(L, b->cap);
@@ -173,7 +171,7 @@ ddgi_status ensure_graph(Batch* b, uint32_t mode) {
 	if(e == hipSuccess) e = launch_batch_trace(L, b->cfg.trace_block, std::min(g_primary, L.trace_item_count), b->cap);
 	if(e == hipSuccess) e = launch_batch_sort(L, max_list, b->cap);
 	if(e == hipSuccess) e = launch_batch_shade(L, b->cap);
-	if(e == hipSuccess) e = launch_batch_blend(L, std::min(g_blend, L.blend_item_count), b->cap);
+	if(e == hipSuccess) e = launch_batch_blend(L, b->cap);
 	hipGraph_t graph = nullptr;
 	const hipError_t e_end = hipStreamEndCapture(b->cap, &graph);
 	if(e != hipSuccess || e_end != hipSuccess) {

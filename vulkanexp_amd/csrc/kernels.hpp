@@ -295,7 +295,7 @@ struct BatchItem {
 };
 
 struct BatchControl {
-	uint32_t next[4];        // work-queue heads: [0] primary traversal, [2] blend
+	uint32_t next[4];        // work-queue heads: [0] primary traversal
 	unsigned long long t0;   // wall clock when the update started on the device
 	unsigned long long pad;
 	unsigned long long done[kMaxBatchVolumes];  // wall clock when a volume's last blend tile ended
@@ -324,9 +324,8 @@ hipError_t launch_batch_trace(const BatchLaunch& b, uint32_t block, uint32_t gri
 hipError_t launch_batch_sort(const BatchLaunch& b, uint64_t max_capacity, hipStream_t s);
 // Shades every volume's ordered shade list: one workgroup per shade item, one thread per entry.
 hipError_t launch_batch_shade(const BatchLaunch& b, hipStream_t s);
-hipError_t launch_batch_blend(const BatchLaunch& b, uint32_t grid, hipStream_t s);
+hipError_t launch_batch_blend(const BatchLaunch& b, hipStream_t s);
 // Resident workgroups per CU of the persistent kernels (for sizing their grids).
 hipError_t batch_trace_resident(uint32_t block, int* per_cu);
-hipError_t batch_blend_resident(uint32_t mode, uint32_t probes, int* per_cu);
 
 }  // namespace ddgi
