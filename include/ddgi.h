@@ -397,8 +397,9 @@ ddgi_status ddgi_synchronize(void* handle);
  * No counterpart in the reference, which updates one IrradianceProbes per scene
  * (src/IrradianceProbes.cpp:426-594).  A batch groups handles (same device, each with a scene, no
  * handle twice) and updates all of them with ONE hipGraph launch: frame directions and blend
- * weights of every volume, then persistent work-queue kernels for the primary traversal, the
- * shading + shadow rays and the blend, each over the work items of all volumes.  The atlases,
+ * weights of every volume, a persistent work-queue kernel for the primary traversal over the work
+ * items of all volumes, the shade-list ordering, then the shading + shadow rays (one thread per
+ * list entry) and the blend (one workgroup per probe tile) over all volumes.  The atlases,
  * states and ray records are bit-identical to ddgi_update(h, R_h, DDGI_UPDATE_ALL_PROBES | mode)
  * on each handle.  The handles stay usable on their own; ddgi_destroy is refused while a batch
  * holds the handle.  Errors of the batch calls are reported by ddgi_last_error(NULL). */
