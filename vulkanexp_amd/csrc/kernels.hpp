@@ -18,7 +18,7 @@ constexpr uint32_t kBlendTexels = kIrrTexels + kDepTexels;
 constexpr uint32_t kWeightRows = 256;  // blend weight rows: 36 irradiance + 12 zero + 196 depth + 12 zero
 constexpr uint32_t kTraceBlock = 128;  // threads per traversal block
 constexpr uint32_t kPrimaryBlock = 256;  // threads per block of the probe update's primary traversal
-constexpr uint32_t kShadeBlock = 64;     // threads per block of the shading kernels (k_shade, k_reflection_shade)
+constexpr uint32_t kShadeBlock = 64;     // threads per block of the shading kernels (k_shade, k_reflection_shade, k_batch_shade)
 
 // Grid constants derived on the host (FP32, same operations as the oracle's ray generation).
 struct GridConst {
