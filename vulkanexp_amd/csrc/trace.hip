@@ -492,10 +492,11 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 // One thread per entry of the ordered shade list, over a grid sized for the list's capacity (its
 // length is known only on the device; workgroups past it end at once).  The hardware refills a CU
 // as soon as one of its workgroups ends, so the CUs stay busy to the end of the list, which a
-// resident grid striding over the list does not do.  At least 5 waves per SIMD: the shadow
-// traversal is latency-bound, and the register budget of 5 waves costs no spills.
+// resident grid striding over the list does not do.  At least 6 waves per SIMD: the shadow
+// traversal is latency-bound, and the sixth wave hides more latency than the 48 bytes per lane its
+// register budget spills cost (0.461 against 0.469 ms at 5 waves).
 template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kShadeBlock, 5) void k_shade(TraceFrameArgs a) {
+__global__ __launch_bounds__(kShadeBlock, 6) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kShadeBlock];
 	const uint32_t k = blockIdx.x * kShadeBlock + threadIdx.x;
 	if(k < a.shade_count[kShadeTotalSlot]) shade_ray<STACK, COUNT>(a, k, stack + threadIdx.x, kShadeBlock);
