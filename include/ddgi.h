@@ -249,6 +249,12 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, const float* transforms, void* stream);
 /* Node / triangle counts and bytes of the uploaded BVH. */
 ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes);
+/* For every BVH triangle slot (count = triangle_count of ddgi_get_bvh_info; host arrays): the TLAS
+ * instance and the primitive index within its mesh (gl_InstanceID / gl_PrimitiveID of a hit on it,
+ * src/shaders/closesthit.glsl:147-150).  The internal hit records of an update (the `hits` of
+ * ddgi_device_view: t, slot | back-face << 31, u, v) name the slot; this maps it to the ids a
+ * ddgi_trace_rays record carries.  Synchronous. */
+ddgi_status ddgi_get_slot_ids(void* handle, uint32_t* instance, uint32_t* primitive, uint64_t count);
 
 /* ---- scene ingestion (no counterpart in IrradianceProbes: the reference's Scene::load / save,
  * src/Scene.cpp:30-49 (.scene, .gltf, .glb, .obj), :115-533 (glTF 2.0), :534-646 (OBJ), :647-684 (.mat), :710-934

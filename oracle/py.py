@@ -144,6 +144,34 @@ def select(states: np.ndarray, cap: int, last_offset: int, loop_index: int):
     return out[:cnt].copy(), int(lo[0]), int(li[0])
 
 
+def hysteresis_ramp(probe_count: int, counts, target: float = 0.98, hysteresis: float = 0.0,
+                    uploaded: float = 0.0, updated: int = 0):
+    """The hysteresis each of a run of updates blends with (the grid UBO's value when the update is
+    recorded), src/IrradianceProbes.cpp:462-476: ``counts`` are the probes each update selects;
+    ``hysteresis`` is GridParameters.hysteresis (0 after init, IrradianceProbes.hpp:53 and
+    .cpp:358), ``uploaded`` the value last written to the UBO (updateUniforms, .cpp:106-108),
+    ``updated`` the function-static counter (.cpp:463).  Before every update, once ``updated``
+    reaches the probe count: while |h - target| > 0.05 (a float distance against the double 0.05)
+    the UBO receives h BEFORE h moves a tenth of the way to the target (float arithmetic), so the
+    kernels lag one step; then h snaps to the target and is uploaded at once.  Returns the list of
+    values used and the final (hysteresis, uploaded, updated)."""
+    f = np.float32
+    h, up, t = f(hysteresis), f(uploaded), f(target)
+    used = []
+    for c in counts:
+        if updated >= probe_count:
+            if float(abs(f(h - t))) > 0.05:
+                up = h
+                h = f(h + f(f(0.1) * f(t - h)))
+            elif h != t:
+                h = t
+                up = t
+            updated -= probe_count
+        updated += int(c)
+        used.append(float(up))
+    return used, (float(h), float(up), updated)
+
+
 class Grid:
     def __init__(self, params: np.ndarray):
         self.params = np.array(params)  # 0-d structured GRID_DTYPE

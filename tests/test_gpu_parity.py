@@ -1,12 +1,13 @@
 """GPU parity: the HIP kernels (through the C-ABI) against the CPU oracle on the same seeded inputs.
 
 Bars
-  * ray/triangle decisions: bit-exact (t, barycentrics as FP32 bits; instance, primitive, facing)
-  * probe states from init: identical
-  * ray radiance/depth (FP32 GPU vs FP64 oracle): per-ray relative error <= 1e-4 for >= 99.5 % of
-    rays (the rest are branch flips of sampleProbes near a discrete threshold), and <= 5e-3 on the
-    cosine-weighted irradiance texels after the blend
-  * depth texels: relative error <= 1e-4 (max over texels, 99.9th percentile)
+  * ray/triangle decisions: bit-exact (t, barycentrics as FP32 bits; instance, primitive, facing),
+    for ddgi_trace_rays and for the update's own hit records (BVH slot -> instance / primitive)
+  * probe states from init: identical; after an update: >= 99 % identical (a state moves when the
+    probe's max change crosses 0.02 / state or 0.04 / state, which FP32 and FP64 may see apart)
+  * ray radiance/depth (FP32 GPU vs FP64 oracle): per-ray relative error <= 1e-4 for every ray
+  * irradiance and depth texels: relative error <= 1e-5 (measured: <= 1.4e-6)
+  (relative error = |gpu - oracle| / max(|oracle|, 1e-2))
 """
 import numpy as np
 import pytest
@@ -102,34 +103,52 @@ def test_init_probes_states_identical(small):
     probes.destroy()
 
 
-def _frame_compare(scene, grid, osc, frames, seed, flags=_lib.UPDATE_ALL_PROBES):
+RAY_BAR = 1e-4
+TEXEL_BAR = 1e-5
+
+
+def _frame_compare(scene, grid, osc, frames, seed, flags=_lib.UPDATE_ALL_PROBES, probes_per_update=0, states=None):
+    """Updates teacher-forced from the GPU state (the oracle starts every frame from the GPU's
+    atlases and states).  Per frame: the update's hit records must equal the oracle's; returns the
+    fraction of rays within RAY_BAR, the max texel errors over the WHOLE atlases (probes that were
+    not updated must keep their tiles) and the fraction of equal states."""
     og = orc.Grid(grid)
-    probes = _make(scene, grid)
-    irr, dep = og.zero_atlases()
-    states = np.ones(og.count, np.uint32)
+    probes = IrradianceProbes(grid, max_rays_per_probe=int(grid["rays_per_probe"]),
+                              probes_per_update=probes_per_update)
+    probes.set_scene(scene)
+    probes.set_light(scenes.sun_light())
+    probes.set_sky(scenes.SKY_RADIANCE)
+    if states is not None:
+        probes.set_states(states)
+    slot_ids = probes.slot_ids()
     light = scenes.sun_light()
     stats = []
     for f, R in enumerate(scenes.orientations(seed, frames)):
-        # teacher forcing: the oracle starts every frame from the GPU's state
         irr = probes.irradiance()[..., :3].astype(np.float64)
         dep = probes.depth().astype(np.float64)
         states = probes.states()
         probes.update(R, flags)
         probes.synchronize()
         gray, gidx = probes.last_update_rays()
-        ghit = probes.last_update_hits()
+        ghit = probes.last_update_hit_records(slot_ids)
         res = osc.update(og, light, scenes.SKY_RADIANCE.astype(np.float64), R, gidx, states, irr, dep, want_hits=True)
-        oh = res["hits"]
-        assert np.array_equal(ghit[..., 0], oh[..., 0]) and np.array_equal(ghit[..., 2], oh[..., 4]) and \
-            np.array_equal(ghit[..., 3], oh[..., 5]), f"frame {f}: hit records differ"
+        bad = np.any(ghit != res["hits"], axis=-1)
+        assert not bad.any(), f"frame {f}: {bad.sum()} hit records differ, first gpu {ghit[bad][0]} oracle {res['hits'][bad][0]}"
         ray_err = np.abs(gray - res["rays"]) / np.maximum(np.abs(res["rays"]), 1e-2)
         gi, gd = probes.irradiance(), probes.depth()
         ie = np.abs(gi[..., :3] - irr) / np.maximum(np.abs(irr), 1e-2)
         de = np.abs(gd - dep) / np.maximum(np.abs(dep), 1e-2)
-        stats.append((np.mean(ray_err.max(-1) <= 1e-4), ie.max(), np.quantile(ie, 0.999), de.max(),
-                      np.mean(probes.states() == states)))
+        stats.append((np.mean(ray_err.max(-1) <= RAY_BAR), ie.max(), np.quantile(ie, 0.999), de.max(),
+                      np.mean(probes.states() == states), len(gidx)))
     probes.destroy()
     return stats
+
+
+def _check(stats, what):
+    for f, (ok_frac, ie_max, ie_999, de_max, st_eq, count) in enumerate(stats):
+        print(f"{what} frame {f}: {count} probes, rays within {RAY_BAR:g} {ok_frac:.5f}, irr max {ie_max:.2e} "
+              f"p99.9 {ie_999:.2e}, depth max {de_max:.2e}, states eq {st_eq:.4f}", flush=True)
+        assert ok_frac == 1.0 and ie_max <= TEXEL_BAR and de_max <= TEXEL_BAR and st_eq >= 0.99
 
 
 def test_update_long_grid_parity():
@@ -139,10 +158,7 @@ def test_update_long_grid_parity():
     scene, _ = scenes.config_small(seed=3, open_top=True)
     lo, hi = scenes.inset_extent(scene)
     grid = scenes.grid_params(lo, hi, (70, 2, 3), 64)
-    stats = _frame_compare(scene, grid, orc.OracleScene(scene), 2, seed=5)
-    for f, (ok_frac, ie_max, ie_999, de_max, st_eq) in enumerate(stats):
-        print(f"frame {f}: rays within 1e-4 {ok_frac:.5f}, irr max {ie_max:.2e}, depth max {de_max:.2e}, states eq {st_eq:.4f}")
-        assert ok_frac >= 0.995 and ie_max <= 5e-3 and de_max <= 1e-4 and st_eq >= 0.99
+    _check(_frame_compare(scene, grid, orc.OracleScene(scene), 2, seed=5), "70x2x3")
 
 
 def test_update_with_no_front_face_hits():
@@ -156,20 +172,29 @@ def test_update_with_no_front_face_hits():
     sb.add_instance(sb.add_mesh(pos, nrm, np.array([[0, 2, 1], [0, 3, 2]]), m))
     scene = sb.build((-10, -10, -10), (10, 10, 10))
     grid = scenes.grid_params((-8, -8, -8), (8, 8, 8), (4, 2, 4), 64)
-    stats = _frame_compare(scene, grid, orc.OracleScene(scene), 2, seed=9)
-    for f, (ok_frac, ie_max, ie_999, de_max, st_eq) in enumerate(stats):
-        assert ok_frac == 1.0 and ie_max <= 5e-3 and de_max <= 1e-4 and st_eq >= 0.99
+    _check(_frame_compare(scene, grid, orc.OracleScene(scene), 2, seed=9), "all-miss")
 
 
 def test_update_frames_small_parity(small):
     scene, grid, osc = small
-    stats = _frame_compare(scene, grid, osc, 4, seed=21)
-    for f, (ok_frac, ie_max, ie_999, de_max, st_eq) in enumerate(stats):
-        print(f"frame {f}: rays within 1e-4 {ok_frac:.5f}, irr max {ie_max:.2e} p99.9 {ie_999:.2e}, depth max {de_max:.2e}, states eq {st_eq:.4f}")
-        assert ok_frac >= 0.995
-        assert ie_max <= 5e-3
-        assert de_max <= 1e-4
-        assert st_eq >= 0.99
+    _check(_frame_compare(scene, grid, osc, 4, seed=21), "all probes")
+
+
+def test_update_reference_schedule_parity(small):
+    """The reference's per-frame path: update() with the probes selectProbesToUpdate picks
+    (src/IrradianceProbes.cpp:396-424) from random states, without and with a ProbesPerUpdate cap
+    below P.  Rays, hit records, the whole atlases (tiles of probes not picked are carried over)
+    and the states are compared with the oracle."""
+    scene, grid, osc = small
+    og = orc.Grid(grid)
+    rng = np.random.Generator(np.random.PCG64(17))
+    states = rng.choice(np.array([0, 1, 2, 3, 5, 8], np.uint32), og.count)
+    stats = _frame_compare(scene, grid, osc, 4, seed=23, flags=0, states=states)
+    assert all(0 < s[-1] < og.count for s in stats)
+    _check(stats, "reference schedule")
+    stats = _frame_compare(scene, grid, osc, 4, seed=24, flags=0, probes_per_update=37, states=states)
+    assert all(s[-1] == 37 for s in stats), [s[-1] for s in stats]
+    _check(stats, "ProbesPerUpdate 37")
 
 
 def test_reference_schedule_selection_matches_oracle(small):

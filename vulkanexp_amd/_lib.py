@@ -94,6 +94,7 @@ _SIGNATURES = {
     "ddgi_set_sky_model": (C.c_int, [_P, C.c_uint32]),
     "ddgi_set_scene": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ]),
     "ddgi_get_bvh_info": (C.c_int, [_P, _P, _P, _P]),
+    "ddgi_get_slot_ids": (C.c_int, [_P, _P, _P, _U64]),
     "ddgi_update_instances": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
     "ddgi_scene_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "ddgi_scene_load_material": (C.c_int, [_P, C.c_char_p]),

@@ -299,11 +299,12 @@ ddgi_status enqueue_blend_and_publish(Handle* h, const Rot& R, const float4* ray
 }
 
 // src/IrradianceProbes.cpp:462-476 with its order: in the first branch the uniforms are uploaded
-// BEFORE the value moves, so the change reaches the kernels one ramp step later.
+// BEFORE the value moves, so the change reaches the kernels one ramp step later.  The distance is
+// a float compared with the double 0.05, as there (0.05f would differ when it is exactly 0.05f).
 void advance_hysteresis(Handle* h, uint32_t updated) {
 	if(h->updated_probes >= h->P) {
 		const float target = h->opts.target_hysteresis;
-		if(std::fabs(h->params.hysteresis - target) > 0.05f) {
+		if(double(std::fabs(h->params.hysteresis - target)) > 0.05) {
 			h->uploaded_hysteresis = h->params.hysteresis;
 			h->params.hysteresis += 0.1f * (target - h->params.hysteresis);
 		} else if(h->params.hysteresis != target) {
@@ -665,6 +666,28 @@ ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* tria
 	if(node_count) *node_count = h->sd.node_count;
 	if(triangle_count) *triangle_count = h->sd.tri_count;
 	if(bytes) *bytes = h->bvh_bytes;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_get_slot_ids(void* handle, uint32_t* instance, uint32_t* primitive, uint64_t count) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "no scene");
+	if(!instance || !primitive || count != h->sd.tri_count)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null output or count != triangle count");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	// the global id (row 0 .w) and the instance (row 1 .w) of every triangle record, in slot order
+	std::vector<float4> tris(size_t(count) * 3);
+	HIP_TRY(h, hipMemcpy(tris.data(), h->sd.tris, tris.size() * sizeof(float4), hipMemcpyDeviceToHost));
+	for(uint64_t k = 0; k < count; ++k) {
+		uint32_t gid, inst;
+		std::memcpy(&gid, &tris[k * 3].w, 4);
+		std::memcpy(&inst, &tris[k * 3 + 1].w, 4);
+		if(inst >= h->bvh_meta.inst_prim_base.size()) return fail(h, DDGI_ERROR_INVALID_STATE, "corrupt triangle record");
+		instance[k] = inst;
+		primitive[k] = gid - h->bvh_meta.inst_prim_base[inst];
+	}
 	return DDGI_OK;
 }
 

@@ -288,6 +288,32 @@ class IrradianceProbes:
         hits[:, self._ray_order(v, n)] = stored
         return hits
 
+    def slot_ids(self):
+        """(instance, primitive) uint32 arrays indexed by BVH triangle slot."""
+        tc = self.bvh_info()["triangles"]
+        inst = np.zeros(tc, np.uint32)
+        prim = np.zeros(tc, np.uint32)
+        check(self._lib.ddgi_get_slot_ids(self.handle, _p(inst), _p(prim), tc), self.handle)
+        return inst, prim
+
+    def last_update_hit_records(self, slot_ids=None):
+        """Hit records of the last update as (count, n, 6) uint32 (t bits, instance, primitive,
+        back-face flag, u bits, v bits) in ray-index order, the layout of a ddgi_trace_rays record
+        reordered as (t, instance, primitive, flags, u, v); misses: t = -1, ids 0xFFFFFFFF, u = v = 0."""
+        h = self.last_update_hits()
+        inst, prim = self.slot_ids() if slot_ids is None else slot_ids
+        code = h[..., 1]
+        miss = code == 0xFFFFFFFF
+        slot = np.where(miss, 0, code & 0x7FFFFFFF)
+        out = np.empty(h.shape[:-1] + (6,), np.uint32)
+        out[..., 0] = h[..., 0]
+        out[..., 1] = np.where(miss, 0xFFFFFFFF, inst[slot])
+        out[..., 2] = np.where(miss, 0xFFFFFFFF, prim[slot])
+        out[..., 3] = np.where(miss, 0, code >> 31)
+        out[..., 4] = h[..., 2]
+        out[..., 5] = h[..., 3]
+        return out
+
 
 def _device_copy(ptr: int, numel: int, dtype):
     """Copy `numel` elements at device address `ptr` to a host tensor (synchronous)."""
