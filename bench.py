@@ -7,12 +7,16 @@ irradiance/depth blend with hysteresis + border copy (fused) + publish.
 
 Prints ONE JSON line (rank 0).  `value` = primary probe rays per second of whole updates, in
 Mrays/s, over `--steps` updates enqueued back to back (no timing events, no host wait between
-them); `phases_ms` come from a second pass over the same updates with per-phase HIP events.  `roofline` describes the kernel with the largest share of the step; its algorithmic
-bytes come from the traversal's own fetch counters (128-byte 4-wide nodes, 48-byte triangles) of an
-untimed counting update; `achievable_peak` / `frac_of_achievable` set it against a 2 GiB device
-copy measured in the same run (`achievable_peaks` also holds an FP32 GEMM rate for the blend).
+them); `phases_ms` come from a second pass over the same updates with per-phase HIP events.
+`roofline` describes the kernel with the largest share of the step: its algorithmic bytes come from
+the traversal's own fetch counters (128-byte 4-wide nodes, 48-byte triangles) of an untimed
+counting update, set against the HBM peak; `traffic` (bytes leaving L2 per launch, from the
+committed rocprofv3 counter summary) shows how few of them reach HBM, and `cache_ceiling` is the
+fastest device copy that stays in L2 / the Infinity Cache, measured in the same run.
 `cpu_baseline` times the CPU oracle (FP64 shading, FP32 intersection replica) through its thread
-pool on a bounded sample of the same frame.
+pool on a bounded sample of the same frame (1 warm-up + median of 5, FP32 variant beside it);
+the same leg checks the mixed-precision blend workload (`blend_config3`) against the oracle's FP64
+blend.
 """
 from __future__ import annotations
 
@@ -40,7 +44,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-probes", type=int, default=4096, help="probes in the CPU-baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="cap on the baseline's threads (the host CPU share of one GPU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blend", choices=["fp32", "bf16", "fp16"], default="fp32")
     ap.add_argument("--no-blend-compare", action="store_true")
@@ -56,28 +61,64 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(scene, grid, light, sky, R, n_probes, threads):
+def host_cpu():
+    """The host's CPU as the baseline reports it: lscpu's model name and physical cores (cores per
+    socket x sockets), std::thread::hardware_concurrency() (= os.cpu_count()) and the CPUs this
+    process may run on."""
+    info = {"model": None, "physical_cores": None, "hardware_concurrency": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        import subprocess
+        kv = {}
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        if kv.get("Core(s) per socket") and kv.get("Socket(s)"):
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_baseline(scene, grid, light, sky, R, n_probes, max_threads):
+    """The oracle (FP64 shading over the FP32 intersection replica) on the host through its thread
+    pool, on a bounded sample of the same frame: `n_probes` probes spread evenly over the grid, each
+    traced, shaded, blended and border-copied as in the update.  Threads: hardware_concurrency(),
+    capped by the CPUs this process may use and by `max_threads` (the host share of one GPU).  One
+    warm-up, then the median of 5 runs, per phase and in total; the oracle's FP32 instantiation is
+    timed the same way beside it.  Returns the baseline object and the states of the sample."""
     from oracle import py as orc
+    cpu = host_cpu()
+    threads = max(1, min(cpu["hardware_concurrency"] or 1, cpu["affinity_cpus"], max_threads))
     osc = orc.OracleScene(scene)
     og = orc.Grid(grid)
     used = orc.set_threads(threads)
     states, _ = osc.init_probes(og, R)
     idx = np.linspace(0, og.count - 1, n_probes).astype(np.uint32)
     n = int(grid["rays_per_probe"])
-    # one warm-up on a small slice, then the median of 3 timed runs of the whole sample
-    irr, dep = og.zero_atlases()
-    osc.update(og, light, sky, R, idx[:64], states.copy(), irr, dep)
-    times = []
-    for _ in range(3):
-        irr, dep = og.zero_atlases()
-        t0 = time.perf_counter()
-        osc.update(og, light, sky, R, idx, states.copy(), irr, dep)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    rate = n_probes * n / t / 1e6
-    return {"value": round(rate, 3), "unit": "Mrays/s", "cores": used, "kind": "port",
-            "sample": f"{n_probes} of {og.count} probes x {n} rays, one frame (trace+shade+shadow+blend+border), "
-                      f"FP64 oracle, {used} threads, median of 3 = {t * 1e3:.0f} ms",
+
+    def run(fp32):
+        res = []
+        for k in range(6):  # 1 warm-up + 5 timed
+            irr, dep = og.zero_atlases()
+            t0 = time.perf_counter()
+            r = osc.update(og, light, sky, R, idx, states.copy(), irr, dep, fp32=fp32)
+            if k:
+                res.append((time.perf_counter() - t0, r["trace_ms"], r["blend_ms"], r["border_ms"]))
+        med = np.median(np.array(res), axis=0)
+        return {"total_ms": round(med[0] * 1e3, 1), "trace_ms": round(med[1], 1), "blend_ms": round(med[2], 1),
+                "border_ms": round(med[3], 2), "Mrays/s": round(n_probes * n / med[0] / 1e6, 3)}
+
+    fp64 = run(False)
+    fp32 = run(True)
+    rate = fp64["Mrays/s"]
+    return {"value": rate, "unit": "Mrays/s", "cores": used, "kind": "port",
+            "sample": f"{n_probes} of {og.count} probes (evenly spaced) x {n} rays of one frame: trace + shade + "
+                      f"shadow + blend + border, FP64 oracle, {used} threads, 1 warm-up + median of 5 = "
+                      f"{fp64['total_ms']:.0f} ms",
+            "threads": used, "cpu": cpu, "phases_ms": fp64, "fp32_variant": fp32,
             "update_ms_extrapolated": round(og.count * n / (rate * 1e6) * 1e3, 1)}
 
 
@@ -141,6 +182,64 @@ def blend_compare(probes, R, lib):
     out["bf16_speedup_vs_fp32"] = round(out["fp32"]["blend_ms"] / out["bf16"]["blend_ms"], 2)
     out["fp16_speedup_vs_fp32"] = round(out["fp32"]["blend_ms"] / out["fp16"]["blend_ms"], 2)
     return out
+
+
+def blend_config3(lib):
+    """The mixed-precision blend workload: the closed room + 64 boxes (780 triangles), 8x4x8 probes
+    x 512 rays, cosine weights [36 x 512] for irradiance and cos^50 weights [196 x 512] for depth.
+    One ray buffer from the update is blended by each mode with hysteresis 0 (the new-frame
+    estimate), timed over 10 blends; the atlases are kept for the error check against the FP64
+    blend (done in the CPU leg)."""
+    import torch
+    from vulkanexp_amd import IrradianceProbes, scenes
+    scene, grid = scenes.config_small(seed=1, rays=512)
+    grid = np.array(grid)
+    grid["depth_sharpness"] = 50.0
+    p = IrradianceProbes(grid, max_rays_per_probe=512)
+    p.set_scene(scene)
+    p.set_light(scenes.sun_light())
+    p.set_sky(scenes.SKY_RADIANCE)
+    Rs = scenes.orientations(33, 4)
+    for R in Rs[:3]:
+        p.update(R, lib.UPDATE_ALL_PROBES)
+    p.GridParameters["hysteresis"] = 0.0
+    p.update_uniforms()
+    state = p.save_state()
+    p.update(Rs[3], lib.UPDATE_ALL_PROBES)
+    p.synchronize()
+    rays, idx = p.last_update_rays()
+    p.load_state(state)
+    check = {"grid": np.array(p.GridParameters), "R": Rs[3], "rays": rays, "idx": idx, "states": p.states(),
+             "irr": p.irradiance()[..., :3].astype(np.float64), "dep": p.depth().astype(np.float64), "out": {}}
+    t_rays = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    t_idx = torch.from_numpy(idx.view(np.int32).copy()).cuda()
+    out = {"workload": "closed room + 64 boxes (780 tris), 8x4x8 probes x 512 rays, depthSharpness 50, hysteresis 0"}
+    for mode, flag in (("fp32", 0), ("bf16", lib.UPDATE_BLEND_BF16), ("fp16", lib.UPDATE_BLEND_FP16)):
+        ms = []
+        for _ in range(10):
+            p.load_state(state)
+            p.blend_rays(t_rays, t_idx, Rs[3], flag)
+            ms.append(p.timings()["blend_ms"])
+        p.synchronize()
+        check["out"][mode] = (p.irradiance()[..., :3].astype(np.float64), p.depth().astype(np.float64))
+        out[mode] = {"blend_ms": round(float(np.median(ms)), 4)}
+    p.destroy()
+    return out, check
+
+
+def config3_errors(check):
+    """Per-texel relative error of each blend mode against the oracle's FP64 blend of the same ray
+    buffer: |gpu - fp64| / max(|fp64|, 1e-3 max |fp64|), max and mean, irradiance and depth."""
+    from oracle import py as orc
+    og = orc.Grid(check["grid"])
+    irr, dep = check["irr"].copy(), check["dep"].copy()
+    orc.blend(og, orc.frame_dirs(check["R"], int(og.params["rays_per_probe"])), check["rays"].astype(np.float64),
+              check["idx"], check["states"].copy(), irr, dep)
+
+    def rel(g, o):
+        e = np.abs(g - o) / np.maximum(np.abs(o), 1e-3 * max(np.abs(o).max(), 1e-12))
+        return {"max": float(f"{e.max():.3e}"), "mean": float(f"{e.mean():.3e}")}
+    return {mode: {"irradiance": rel(gi, irr), "depth": rel(gd, dep)} for mode, (gi, gd) in check["out"].items()}
 
 
 def frame_bench(probes, scene, width=1920, height=1080):
@@ -365,6 +464,13 @@ def achievable_peaks(torch):
     t_copy = timed(lambda: dst.copy_(src), 10)
     copy_gbs = 2 * src.numel() * 4 / t_copy / 1e9
     del src, dst
+    cache = {}
+    for mib in (8, 32, 64):  # copies that stay in L2 (8 x 4 MB) and in the 256 MB Infinity Cache
+        src = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        t = timed(lambda: dst.copy_(src), 200)
+        cache[f"{mib}MiB"] = round(2 * src.numel() * 4 / t / 1e9, 1)
+        del src, dst
     a = torch.randn(8192, 8192, device="cuda", dtype=torch.float32)
     b = torch.randn(8192, 8192, device="cuda", dtype=torch.float32)
     t_mm = timed(lambda: torch.mm(a, b), 5)
@@ -372,7 +478,10 @@ def achievable_peaks(torch):
     del a, b
     torch.cuda.empty_cache()
     return {"copy_GB/s": round(copy_gbs, 1), "fp32_gemm_TFLOP/s": round(gemm_tf, 1),
+            "cache_copy_GB/s": cache, "cache_ceiling_GB/s": max(cache.values()),
             "copy": "2 GiB device-to-device torch copy_, read + written bytes",
+            "cache_copy": "device-to-device copies of 8 / 32 / 64 MiB repeated 200 times (resident in L2 or the "
+                          "Infinity Cache), read + written bytes; the fastest is the cache-level ceiling",
             "gemm": "torch.mm 8192x8192x8192 float32"}
 
 
@@ -439,27 +548,44 @@ def main():
     blend_bytes = P * n * 16 + P * (36 * 12 + 196 * 8) * 2
     blend_flops = 2 * P * n * (36 * 3 + 196 * 2)
     kernels = {
-        "trace_primary": {"ms": avg["primary_ms"], "bytes": prim_bytes},
-        "shade_shadow": {"ms": avg["shade_ms"], "bytes": shade_bytes},
-        "blend": {"ms": avg["blend_ms"], "bytes": blend_bytes, "flops": blend_flops},
+        "trace_primary": {"ms": avg["primary_ms"], "algorithmic_bytes": prim_bytes},
+        "shade_shadow": {"ms": avg["shade_ms"], "algorithmic_bytes": shade_bytes},
+        "blend": {"ms": avg["blend_ms"], "algorithmic_bytes": blend_bytes, "flops": blend_flops},
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     dk = kernels[dom]
-    achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
+    achieved = dk["algorithmic_bytes"] / (dk["ms"] * 1e-3) / 1e9
+    traffic = profiled_traffic(dom)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": os.path.relpath(PROFILE_SUMMARY, ROOT),
-                "algorithmic_bytes_per_launch": int(dk["bytes"]), "avg_ms": round(dk["ms"], 4)}
+                "algorithmic_bytes_per_launch": int(dk["algorithmic_bytes"]), "avg_ms": round(dk["ms"], 4),
+                "algorithmic_bytes_definition": "per primary ray: 128 B per 4-wide node fetched + 48 B per triangle "
+                                                "fetched (the traversal's own fetch counters) + the 16 B hit record"}
     for k, v in kernels.items():
-        v["GB/s"] = round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
+        v["algorithmic_GB/s"] = round(v["algorithmic_bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
+        t = profiled_traffic(k)
+        if t:
+            v["traffic_bytes"] = t
+            v["traffic_GB/s"] = round(t / (v["ms"] * 1e-3) / 1e9, 1)
         v["ms"] = round(v["ms"], 4)
     kernels["blend"]["TFLOP/s"] = round(blend_flops / (avg["blend_ms"] * 1e-3) / 1e12, 2)
     peaks = None if args.no_peaks else achievable_peaks(torch)
-    if roofline["traffic"]:  # bytes that left L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE) per second
-        roofline["traffic_GB/s"] = round(roofline["traffic"] / (dk["ms"] * 1e-3) / 1e9, 1)
+    if traffic:
+        # what leaves L2 (rocprofv3 FETCH_SIZE + WRITE_SIZE, MALL hits included) is the HBM-side load;
+        # the algorithmic bytes above are mostly served by L2 and the Infinity Cache
+        tr_gbs = traffic / (dk["ms"] * 1e-3) / 1e9
+        roofline["traffic_GB/s"] = round(tr_gbs, 1)
+        roofline["traffic_frac_of_peak"] = round(tr_gbs / HBM_PEAK_GBS, 4)
+        roofline["fetched_bytes_beyond_L2"] = round(traffic / dk["algorithmic_bytes"], 4)
+    if dom == "trace_primary":
+        roofline["limiter"] = ("latency of dependent node fetches under lane divergence: a wavefront runs until its "
+                               "longest ray ends (VALU lane utilisation in the profile summary), not HBM bandwidth")
     if peaks:
-        roofline["achievable_peak"] = peaks["copy_GB/s"]
-        roofline["frac_of_achievable"] = round(achieved / peaks["copy_GB/s"], 4)
+        roofline["cache_ceiling"] = peaks["cache_ceiling_GB/s"]
+        roofline["frac_of_cache_ceiling"] = round(achieved / peaks["cache_ceiling_GB/s"], 4)
+        if traffic:
+            roofline["traffic_frac_of_achievable_hbm"] = round(roofline["traffic_GB/s"] / peaks["copy_GB/s"], 4)
         kernels["blend"]["frac_of_fp32_peak"] = round(kernels["blend"]["TFLOP/s"] / FP32_PEAK_TFLOPS, 4)
         kernels["blend"]["frac_of_fp32_gemm"] = round(kernels["blend"]["TFLOP/s"] / peaks["fp32_gemm_TFLOP/s"], 4)
     trace_stats = {"nodes_per_primary_ray": st["primary_nodes"] / rays, "tris_per_primary_ray": st["primary_tris"] / rays,
@@ -467,6 +593,7 @@ def main():
 
     frame = None if args.no_frame else frame_bench(probes, scene)
     blend = None if args.no_blend_compare else blend_compare(probes, Rs[-1], _lib)
+    config3, config3_check = (None, None) if args.no_blend_compare else blend_config3(_lib)
     diff = None if args.no_diff else diff_bench(_lib)
     batched = None if args.no_batch else batch_bench(_lib)
     dynamic = None if args.no_dynamic else dynamic_bench(probes, scene, Rs[-1], _lib)
@@ -482,6 +609,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, grid, light, sky.astype(np.float64), Rs[1], args.cpu_probes, args.cpu_threads)
+        if config3 is not None:  # the oracle's FP64 blend as the checker of the config-3 blends
+            for mode, err in config3_errors(config3_check).items():
+                config3[mode]["rel_err_vs_fp64"] = err
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -495,7 +625,7 @@ def main():
         "phases_ms": {k: round(v, 4) for k, v in avg.items()},
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
-        "full_frame": frame, "blend_precision": blend, "differentiable": diff, "batched_volumes": batched,
+        "full_frame": frame, "blend_precision": blend, "blend_config3": config3, "differentiable": diff, "batched_volumes": batched,
         "dynamic_scene": dynamic, "atmospheric_sky": atmosphere, "achievable_peaks": peaks,
     }
     if cpu:

@@ -79,122 +79,130 @@ __global__ __launch_bounds__(256) void k_blend_weights(GridConst g, const float4
 	blend_weight_row(g, dirs, order, wt, inv_sum, w_hi_v, w_lo_v, mode, n_pad, blockIdx.x);
 }
 
-// Border destinations mirroring interior texel (x, y) of an R x R tile (probesCopyBorders.comp
-// tables == octahedral mirror).  Returns the count, writes up to 3 destinations.
+// The interior texel whose blended value texel (x, y) of an R x R tile holds after the border copy:
+// itself inside; on the border, the octahedral mirror that probesCopyBorders.comp's tables encode
+// (an edge texel copies the interior texel next to it, reflected along the edge; a corner copies
+// the opposite interior corner).
 template <int R>
-__device__ __forceinline__ int border_targets(int x, int y, int dx[3], int dy[3]) {
-	int k = 0;
-	if(x == 1) {
-		dx[k] = 0;
-		dy[k] = R - 1 - y;
-		++k;
+__device__ __forceinline__ void border_source(int x, int y, int& sx, int& sy) {
+	const bool bx = x == 0 || x == R - 1, by = y == 0 || y == R - 1;
+	if(bx && by) {
+		sx = x == 0 ? R - 2 : 1;
+		sy = y == 0 ? R - 2 : 1;
+	} else if(bx) {
+		sx = x == 0 ? 1 : R - 2;
+		sy = R - 1 - y;
+	} else if(by) {
+		sx = R - 1 - x;
+		sy = y == 0 ? 1 : R - 2;
+	} else {
+		sx = x;
+		sy = y;
 	}
-	if(x == R - 2) {
-		dx[k] = R - 1;
-		dy[k] = R - 1 - y;
-		++k;
-	}
-	if(y == 1) {
-		dx[k] = R - 1 - x;
-		dy[k] = 0;
-		++k;
-	}
-	if(y == R - 2) {
-		dx[k] = R - 1 - x;
-		dy[k] = R - 1;
-		++k;
-	}
-	// corners
-	if(x == R - 2 && y == R - 2 && k < 3) {
-		dx[k] = 0;
-		dy[k] = 0;
-		++k;
-	}
-	if(x == R - 2 && y == 1 && k < 3) {
-		dx[k] = 0;
-		dy[k] = R - 1;
-		++k;
-	}
-	if(x == 1 && y == R - 2 && k < 3) {
-		dx[k] = R - 1;
-		dy[k] = 0;
-		++k;
-	}
-	if(x == 1 && y == 1 && k < 3) {
-		dx[k] = R - 1;
-		dy[k] = R - 1;
-		++k;
-	}
-	return k;
 }
 
-// Epilogue shared by the blends, for the block's np probes (atlas indices in s_idx): normalise the
-// sums the K loop left in LDS ([48][ci] irradiance rows, [208][cd] depth rows; probe p's channels in
-// columns 3p + c and 2p + c), apply hysteresis against the exposed atlas, store interior + mirrored
-// border texels into the next atlas and collect maxChange per probe in s_maxc.  A thread takes four
-// items at a time and issues their four exposed-atlas loads before any of their stores, so the
-// load latency is paid once per four items.
+// Accumulator layout in LDS between the K loop and the epilogue, per probe slot of the block:
+// irradiance sums [texel][4] (rgb + pad) and depth sums [texel][2], each probe's block padded by
+// 4 floats so that consecutive probes start on different banks.
+constexpr int kIrrStride = 4 * int(kIrrTexels) + 4;
+constexpr int kDepStride = 2 * int(kDepTexels) + 4;
+
+// Accumulator element (texel row, data column) of a blend GEMM into the LDS layout above: irradiance
+// columns are 3 probe + channel, depth columns 2 probe + channel; padding rows and columns are
+// dropped.
+__device__ __forceinline__ void store_irr_sum(float* s_irr, int tex, int col, int np_max, float v) {
+	const int p = col / 3, ch = col - 3 * p;
+	if(tex < int(kIrrTexels) && p < np_max) s_irr[p * kIrrStride + tex * 4 + ch] = v;
+}
+__device__ __forceinline__ void store_dep_sum(float* s_dep, int tex, int col, int np_max, float v) {
+	const int p = col >> 1, ch = col & 1;
+	if(tex < int(kDepTexels) && p < np_max) s_dep[p * kDepStride + tex * 2 + ch] = v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+	for(int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+	return v;
+}
+
+// Epilogue shared by the blends, for the block's np probe slots (atlas indices in s_idx): whole
+// tiles, borders included.  Each texel takes the interior texel it holds after the border copy
+// (border_source), normalises that texel's sum, applies hysteresis against that texel's exposed
+// value and stores into the next atlas; a wavefront writes one tile row-block at a time, so the
+// stores are contiguous.  Irradiance: one wavefront per 8x8 tile (lane = texel), maxChange per
+// probe over the interior texels by a wave reduction.  Depth: a wavefront per 16x16 tile, four
+// texels per lane.  The exposed-atlas loads of kTilesPerPass tiles per wavefront are issued before
+// their stores.
+template <uint32_t kTilesPerPass = 4>
 __device__ __forceinline__ void blend_epilogue(const BlendArgs& a, const uint32_t* s_idx, uint32_t np,
-											   const float* s_out_irr, int ci, const float* s_out_dep, int cd,
-											   uint32_t* s_maxc, bool irr_only) {
-	constexpr uint32_t kU = 4;
+											   const float* s_irr, const float* s_dep, uint32_t* s_maxc, bool irr_only) {
 	const float h = a.grid.hysteresis;
 	const float4* __restrict__ irr_prev = a.irr_prev;
 	const float2* __restrict__ dep_prev = a.dep_prev;
 	float4* __restrict__ irr_out = a.irr_out;
 	float2* __restrict__ dep_out = a.dep_out;
-	const uint32_t ni = np * kIrrTexels;
-	for(uint32_t e0 = threadIdx.x; e0 < ni; e0 += 256u * kU) {
-		float4 prev[kU];
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+	{
+		int sx, sy;
+		border_source<8>(int(lane & 7u), int(lane >> 3), sx, sy);
+		const bool interior = sx == int(lane & 7u) && sy == int(lane >> 3);
+		const uint32_t stex = uint32_t(sy - 1) * 6u + uint32_t(sx - 1);
+		const uint32_t src = uint32_t(sy) * 8u + uint32_t(sx);
+		const float inv = a.inv_sum[stex];
+		for(uint32_t p0 = wave; p0 < np; p0 += waves * kTilesPerPass) {
+			float4 prev[kTilesPerPass];
 #pragma unroll
-		for(uint32_t u = 0; u < kU; ++u) {
-			const uint32_t e = e0 + 256u * u, p = e / kIrrTexels, tex = e % kIrrTexels;
-			if(e < ni) prev[u] = irr_prev[size_t(s_idx[p]) * 64 + (tex / 6u + 1u) * 8 + tex % 6u + 1u];
-		}
+			for(uint32_t u = 0; u < kTilesPerPass; ++u) {
+				const uint32_t p = p0 + waves * u;
+				if(p < np) prev[u] = irr_prev[size_t(s_idx[p]) * 64 + src];
+			}
 #pragma unroll
-		for(uint32_t u = 0; u < kU; ++u) {
-			const uint32_t e = e0 + 256u * u, p = e / kIrrTexels, tex = e % kIrrTexels;
-			if(e >= ni) break;
-			const int x = int(tex % 6u) + 1, y = int(tex / 6u) + 1;
-			const float inv = a.inv_sum[tex];
-			const float* sum = s_out_irr + tex * ci + p * 3;
-			const float r0 = sum[0] * inv, r1 = sum[1] * inv, r2 = sum[2] * inv;
-			const float4 pv = prev[u];
-			const float mc = fmaxf(fmaxf(fabsf(r0 - pv.x), fabsf(r1 - pv.y)), fabsf(r2 - pv.z));
-			const float4 v = make_float4(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h,
-										 r2 * (1.0f - h) + pv.z * h, 1.0f);
-			float4* tile = irr_out + size_t(s_idx[p]) * 64;
-			tile[y * 8 + x] = v;
-			int dx[3], dy[3];
-			const int nb = border_targets<8>(x, y, dx, dy);
-			for(int k = 0; k < nb; ++k) tile[dy[k] * 8 + dx[k]] = v;
-			atomicMax(&s_maxc[p], __float_as_uint(mc));
+			for(uint32_t u = 0; u < kTilesPerPass; ++u) {
+				const uint32_t p = p0 + waves * u;
+				if(p >= np) break;
+				const float4 sum = *reinterpret_cast<const float4*>(s_irr + p * kIrrStride + stex * 4);
+				const float r0 = sum.x * inv, r1 = sum.y * inv, r2 = sum.z * inv;
+				const float4 pv = prev[u];
+				const float mc = interior ? fmaxf(fmaxf(fabsf(r0 - pv.x), fabsf(r1 - pv.y)), fabsf(r2 - pv.z)) : 0.0f;
+				irr_out[size_t(s_idx[p]) * 64 + lane] = make_float4(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h,
+																	  r2 * (1.0f - h) + pv.z * h, 1.0f);
+				const float m = wave_max(mc);
+				if(lane == 0) s_maxc[p] = __float_as_uint(m);
+			}
 		}
 	}
 	if(irr_only) return;
-	const uint32_t nd = np * kDepTexels;
-	for(uint32_t e0 = threadIdx.x; e0 < nd; e0 += 256u * kU) {
-		float2 prev[kU];
+	uint32_t src[4], stex[4];
+	float inv[4];
 #pragma unroll
-		for(uint32_t u = 0; u < kU; ++u) {
-			const uint32_t e = e0 + 256u * u, p = e / kDepTexels, tex = e % kDepTexels;
-			if(e < nd) prev[u] = dep_prev[size_t(s_idx[p]) * 256 + (tex / 14u + 1u) * 16 + tex % 14u + 1u];
+	for(uint32_t i = 0; i < 4; ++i) {
+		int sx, sy;
+		border_source<16>(int(lane & 15u), int(lane >> 4) + 4 * int(i), sx, sy);
+		stex[i] = uint32_t(sy - 1) * 14u + uint32_t(sx - 1);
+		src[i] = uint32_t(sy) * 16u + uint32_t(sx);
+		inv[i] = a.inv_sum[kIrrTexels + stex[i]];
+	}
+	for(uint32_t p0 = wave; p0 < np; p0 += waves * kTilesPerPass) {
+		float2 prev[kTilesPerPass][4];
+#pragma unroll
+		for(uint32_t u = 0; u < kTilesPerPass; ++u) {
+			const uint32_t p = p0 + waves * u;
+			if(p < np) {
+#pragma unroll
+				for(uint32_t i = 0; i < 4; ++i) prev[u][i] = dep_prev[size_t(s_idx[p]) * 256 + src[i]];
+			}
 		}
 #pragma unroll
-		for(uint32_t u = 0; u < kU; ++u) {
-			const uint32_t e = e0 + 256u * u, p = e / kDepTexels, tex = e % kDepTexels;
-			if(e >= nd) break;
-			const int x = int(tex % 14u) + 1, y = int(tex / 14u) + 1;
-			const float inv = a.inv_sum[kIrrTexels + tex];
-			const float* sum = s_out_dep + tex * cd + p * 2;
-			const float r0 = sum[0] * inv, r1 = sum[1] * inv;
-			const float2 pv = prev[u];
-			const float2 v = make_float2(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h);
-			float2* tile = dep_out + size_t(s_idx[p]) * 256;
-			tile[y * 16 + x] = v;
-			int dx[3], dy[3];
-			const int nb = border_targets<16>(x, y, dx, dy);
-			for(int k = 0; k < nb; ++k) tile[dy[k] * 16 + dx[k]] = v;
+		for(uint32_t u = 0; u < kTilesPerPass; ++u) {
+			const uint32_t p = p0 + waves * u;
+			if(p >= np) break;
+#pragma unroll
+			for(uint32_t i = 0; i < 4; ++i) {
+				const float2 sum = *reinterpret_cast<const float2*>(s_dep + p * kDepStride + stex[i] * 2);
+				const float r0 = sum.x * inv[i], r1 = sum.y * inv[i];
+				const float2 pv = prev[u][i];
+				dep_out[size_t(s_idx[p]) * 256 + lane + 64 * i] = make_float2(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h);
+			}
 		}
 	}
 }
@@ -234,14 +242,14 @@ __device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slo
 	constexpr int kCI = 16 * kNI, kCD = 16 * kND;                       // padded columns
 	constexpr int kStride = kMfmaChunk + 4;  // floats per staged column: 16-byte skew between columns
 	constexpr int kStageFloats = (kCI + kCD) * kStride;
-	constexpr int kOutFloats = 48 * kCI + 208 * kCD;
+	constexpr int kOutFloats = NP * (kIrrStride + kDepStride);
 	__shared__ __attribute__((aligned(16))) float smem[kStageFloats > kOutFloats ? kStageFloats : kOutFloats];
 	__shared__ uint32_t s_oor[NP];
 	__shared__ uint32_t s_maxc[NP];
 	__shared__ uint32_t s_idx[NP];
-	float* sb = smem;                    // [column][kStride]: irradiance 3p + c, then depth kCI + 2p + c
-	float* s_out_irr = smem;             // after the K loop: [48][kCI]
-	float* s_out_dep = smem + 48 * kCI;  //                   [208][kCD]
+	float* sb = smem;                            // [column][kStride]: irradiance 3p + c, then depth kCI + 2p + c
+	float* s_irr = smem;                         // after the K loop: [probe][kIrrStride]
+	float* s_dep = smem + NP * kIrrStride;       //                   [probe][kDepStride]
 
 	// count and n steer loops that contain barriers: make them provably uniform (count is read
 	// through a pointer the compiler cannot prove to be global, which it would treat as divergent)
@@ -255,7 +263,6 @@ __device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slo
 	const int lrow = lane & 15, lgrp = lane >> 4;
 	if(t < NP) {
 		s_oor[t] = 0;
-		s_maxc[t] = 0;
 		s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
 	}
 	floatx4_t acc[4][kNI];
@@ -339,17 +346,18 @@ __device__ __forceinline__ void blend_fp32_tile(const BlendArgs& a, uint32_t slo
 		const int mt = wave + 4 * q;
 #pragma unroll
 		for(int c = 0; c < kNI; ++c) {
+			const int col = c * 16 + lrow;
 #pragma unroll
 			for(int r = 0; r < 4; ++r) {
 				const int row = 4 * lgrp + r;
-				if(mt < 3) s_out_irr[(mt * 16 + row) * kCI + c * 16 + lrow] = acc[q][c][r];
-				else if(c < kND) s_out_dep[((mt - 3) * 16 + row) * kCD + c * 16 + lrow] = acc[q][c][r];
+				if(mt < 3) store_irr_sum(s_irr, mt * 16 + row, col, NP, acc[q][c][r]);
+				else if(c < kND) store_dep_sum(s_dep, (mt - 3) * 16 + row, col, NP, acc[q][c][r]);
 			}
 		}
 	}
 	__syncthreads();
 	const bool irr_only = (a.flags & kBlendIrradianceOnly) != 0u;
-	blend_epilogue(a, s_idx, np, s_out_irr, kCI, s_out_dep, kCD, s_maxc, irr_only);
+	blend_epilogue<>(a, s_idx, np, s_irr, s_dep, s_maxc, irr_only);
 	__syncthreads();
 	if(uint32_t(t) < np && !irr_only) update_state(a.states, s_idx[t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
@@ -360,24 +368,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // ---------------------------------------------------------------------------------------------
-// MFMA blend.  Per block: 16 probe slots.  GEMM shapes per block, as 16x16 output tiles of
-// v_mfma_f32_16x16x32_{bf16,f16}:
-//   irradiance: M = 48 (36 texels + 12 zero rows) x N = 48 (16 probes x rgb)   -> 3 x 3 tiles
-//   depth:      M = 208 (196 + 12)                x N = 32 (16 probes x d, d^2) -> 13 x 2 tiles
-// K = rays, in steps of 32.  The 16 M-tiles (3 irradiance, 13 depth) are dealt to the 4 waves
-// (wave w: tiles w, w+4, w+8, w+12), so each wave loads a weight fragment once per K step and
-// reuses it for all N-tiles of its row.  Weights (A) come straight from L2 in the padded
-// [256][n_pad] 16-bit layout; ray data (B) is staged in LDS per 64-ray chunk, column-major so a
-// lane's 8 consecutive K values are one 16-byte read.
+// 16-bit MFMA operand types (v_mfma_f32_16x16x32_{bf16,f16}: 8 K values per lane and operand).
 //   BF16: w = w_hi + w_lo, x = x_hi + x_lo, acc += w_hi x_hi + w_hi x_lo + w_lo x_hi (3 MFMAs;
 //         relative error ~2^-16), normalised by the sum of the split weights.
 //   FP16: one product of fp16 weights and data (relative error ~2^-11 per term).
 // Accumulator layout (verified for gfx950): lane l, register r holds D[4*(l/16)+r][l%16].
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
-
-constexpr int kBStride = kMfmaChunk + 8;    // 16-bit elements per LDS column (padded)
-constexpr int kIrrCols = kProbesPerBlock * 3, kDepCols = kProbesPerBlock * 2;
 
 template <uint32_t MODE>
 struct MfmaTypes;
@@ -396,125 +393,240 @@ struct MfmaTypes<kBlendFP16> {
 	__device__ static floatx4_t mma(V a, V b, floatx4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 };
 
-template <uint32_t MODE>
-__device__ __forceinline__ void blend_mfma_tile(const BlendArgs& a, uint32_t slot0, uint32_t n_pad) {
+// ---------------------------------------------------------------------------------------------
+// MFMA blend, transposed and pipelined (BF16 / FP16 modes).  Per channel the blend of a wave's 16
+// probes is C^T [16 probes x texels] = X^T [16 probes x n rays] W^T [n x texels]: the ray data is
+// the A operand, read by each lane straight from HBM (row = one of the wave's probes, 8
+// consecutive rays per lane and K step = 128 contiguous bytes), and the weights are the B operand,
+// shared by the workgroup's four waves through LDS.  A workgroup owns 64 probe slots (16 per wave)
+// and keeps 140 accumulator registers per lane (irradiance: 3 channels x 3 texel tiles, depth: 2
+// channels x 13 texel tiles), so it runs one wave per SIMD and hides latency by prefetching: the
+// ray data of K step s + 2 is in flight in registers and the weights of step s + 1 are being
+// copied from L2 into LDS (global_load_lds, no registers) while the matrix cores work on step s.
+// Products and their order are those of the blend above (hi x hi, lo x hi, hi x lo into FP32).
+// LDS weight layout per K step: [part][texel tile][K block (4)][row (16)] x 8 elements, so that a
+// wave's B fragment (lane l: texel row l % 16, K block l / 16) is one contiguous 1 KB read and one
+// 64-lane global_load_lds instruction fills one (part, tile).
+// One 16-byte global -> LDS copy per lane (global_load_lds_dwordx4): the lanes' data lands at
+// lds + 16 * lane (lds must be wave-uniform), without passing through registers.
+__device__ __forceinline__ void lds_copy16(const void* src, void* lds) {
+	__builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+}
+
+// Workgroup barrier for LDS traffic: waits for this wave's LDS reads and writes (lgkmcnt) and
+// then meets the other waves.  Unlike __syncthreads() it has no release fence, which would also
+// wait for every global load in flight (the prefetches); LDS filled by global_load_lds must be
+// waited for by the caller (vmcnt) before it.  The empty asm statements keep the compiler from
+// moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+	__asm__ volatile("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt and expcnt untouched
+	__builtin_amdgcn_s_barrier();
+	__asm__ volatile("" ::: "memory");
+}
+
+constexpr int kWideProbes = 64;  // probe slots per workgroup (16 per wave)
+constexpr int kKStep = 32;       // rays per MFMA K step
+
+constexpr int kWideWBytes = 2 * 16 * 64 * 16;  // weights of one K step: 2 parts x 16 tiles x 64 lanes x 16 B
+constexpr int kWideHalf = kWideProbes / 2;      // probes whose sums the epilogue takes at a time
+
+// LDS of a wide-blend workgroup, declared by the kernels (one copy however many K-step variants
+// a kernel holds).  The two weight buffers are separate objects, so that the compiler sees that
+// reading one does not have to wait for the copy into the other; the sums of the epilogue (half of
+// the workgroup's probes at a time) have their own.
+struct WideLds {
+	unsigned char* w[2];  // kWideWBytes each
+	float* sums;          // kWideHalf x (kIrrStride + kDepStride)
+	uint32_t* s_oor;      // [kWideProbes]
+	uint32_t* s_maxc;
+	uint32_t* s_idx;
+};
+
+template <uint32_t MODE, int STEPS>
+__device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slot0, const WideLds& lds) {
 	typedef MfmaTypes<MODE> M;
 	typedef typename M::T T;
 	typedef typename M::V V;
 	constexpr int P = M::kParts;
-	// LDS: staging (B operand) and, after the K loop, the accumulator tiles (aliased)
-	constexpr int kStageBytes = P * (kIrrCols + kDepCols) * kBStride * 2;
-	constexpr int kOutBytes = (48 * kIrrCols + 208 * kDepCols) * 4;
-	__shared__ __attribute__((aligned(16))) unsigned char smem[kStageBytes > kOutBytes ? kStageBytes : kOutBytes];
-	__shared__ uint32_t s_oor[kProbesPerBlock];
-	__shared__ uint32_t s_maxc[kProbesPerBlock];
-	__shared__ uint32_t s_idx[kProbesPerBlock];
-	T* sb = reinterpret_cast<T*>(smem);  // [part][col][kBStride], irr cols first then depth cols
-	float* s_out_irr = reinterpret_cast<float*>(smem);          // [48][48]
-	float* s_out_dep = s_out_irr + 48 * kIrrCols;               // [208][32]
+	uint32_t* s_oor = lds.s_oor;
+	uint32_t* s_maxc = lds.s_maxc;
+	uint32_t* s_idx = lds.s_idx;
 
 	const uint32_t count = __builtin_amdgcn_readfirstlane(*a.count);  // uniform: see blend_fp32_tile
 	if(slot0 >= count) return;
-	const uint32_t np = min(uint32_t(kProbesPerBlock), count - slot0);
+	const uint32_t np = min(uint32_t(kWideProbes), count - slot0);
 	const uint32_t n = __builtin_amdgcn_readfirstlane(a.grid.n);
-	n_pad = __builtin_amdgcn_readfirstlane(n_pad);
+	constexpr uint32_t steps = STEPS;
+	const uint32_t n_pad = STEPS * kKStep;
 	const float cell_len = a.grid.cell_len;
 	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 	const int lrow = lane & 15, lgrp = lane >> 4;
-	const T* __restrict__ wparts[2] = {static_cast<const T*>(a.w_hi16), static_cast<const T*>(a.w_lo16)};
+	const uint32_t my_slot = uint32_t(16 * wave + lrow);  // this lane's A row: a probe slot of the workgroup
+	const bool my_valid = my_slot < np;
+	const float4* __restrict__ my_rays = a.rays + size_t(slot0 + (my_valid ? my_slot : 0u)) * n;
+	const T* __restrict__ wsrc[2] = {static_cast<const T*>(a.w_hi16), static_cast<const T*>(a.w_lo16)};
 
-	floatx4_t acc[4][3];
+	// weights of K step s into LDS buffer b: (part, tile) j by wave j % 4, lane l copies texel row
+	// 16 tile + l % 16, rays 32 s + 8 (l / 16) .. +7
+	auto stage_weights = [&](uint32_t s, int b) {
 #pragma unroll
-	for(int q = 0; q < 4; ++q)
+		for(int i = 0; i < P * 4; ++i) {
+			const int j = wave + 4 * i, part = j >> 4, tile = j & 15;
+			const T* src = wsrc[part] + size_t(16 * tile + lrow) * n_pad + s * kKStep + lgrp * 8;
+			unsigned char* dst = lds.w[b] + (part * 16 + tile) * 1024;
+			lds_copy16(src, dst);
+		}
+	};
+	// ray data of K step s: rays 32 s + 8 (l / 16) .. +7 of this lane's probe.  The loads are
+	// unconditional (clamped to the probe's last ray, or to slot0's rays for an absent probe) so
+	// that they issue back to back without branches; the rays that do not exist (beyond n, absent
+	// probes) are replaced by zero where they are used, ray j of the step being valid when
+	// j < valid_rays(s).
+	auto load_rays = [&](uint32_t s, float4* r) {
 #pragma unroll
-		for(int c = 0; c < 3; ++c) acc[q][c] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+		for(int j = 0; j < 8; ++j) {
+			const uint32_t k = s * kKStep + uint32_t(lgrp * 8 + j);
+			r[j] = my_rays[min(k, n - 1u)];
+		}
+	};
+	auto valid_rays = [&](uint32_t s) -> int {
+		const int first = int(s * kKStep) + lgrp * 8;
+		return my_valid ? max(0, min(8, int(n) - first)) : 0;
+	};
 
-	// staging assignment: thread t loads rays 4*(t%16) .. +3 of probe t/16 of the chunk
-	const int sp = t >> 4, sk = (t & 15) * 4;
+	floatx4_t acc_i[3][3], acc_d[2][13];
+#pragma unroll
+	for(int c = 0; c < 3; ++c)
+#pragma unroll
+		for(int tt = 0; tt < 3; ++tt) acc_i[c][tt] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+	for(int c = 0; c < 2; ++c)
+#pragma unroll
+		for(int tt = 0; tt < 13; ++tt) acc_d[c][tt] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
 	uint32_t my_oor = 0;
-	for(uint32_t k0 = 0; k0 < n_pad; k0 += kMfmaChunk) {
-		__syncthreads();
-		for(int q = 0; q < 4; ++q) {
-			const uint32_t k = k0 + sk + q;
-			float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-			float dc = 0.0f;
-			if(uint32_t(sp) < np && k < n) {
-				r = a.rays[size_t(slot0 + sp) * n + k];
-				if(r.w < 0.0f || r.w > cell_len) ++my_oor;
-				dc = fminf(cell_len, r.w);
-				if(dc < 0.0f) dc = cell_len;
-			}
-			const float v[5] = {r.x, r.y, r.z, dc, dc * dc};
-			const int col[5] = {sp * 3, sp * 3 + 1, sp * 3 + 2, kIrrCols + sp * 2, kIrrCols + sp * 2 + 1};
+
+	// one K step on the data in r (rays of step s) and LDS buffer b; then r receives step s + 2
+	auto step = [&](uint32_t s, int b, float4* r) {
+		// the weights of step s (and everything but the newest ray prefetch) have landed; the other
+		// waves are done with buffer b ^ 1
+		if(s + 1 < steps) __builtin_amdgcn_s_waitcnt(0x0F78);  // vmcnt(8): the step s + 1 rays may stay in flight
+		else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
+		lds_barrier();
+		if(s + 1 < steps) stage_weights(s + 1, b ^ 1);
+		// A fragments (hi / lo) of the five data channels
+		V A[5][2];
+		const int nvalid = valid_rays(s);
+#pragma unroll
+		for(int j = 0; j < 8; ++j) {
+			// rays that do not exist are zero: in range, and nothing in any column
+			const float4 x = j < nvalid ? r[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			if(x.w < 0.0f || x.w > cell_len) ++my_oor;
+			float dc = fminf(cell_len, x.w);
+			if(dc < 0.0f) dc = cell_len;
+			const float v[5] = {x.x, x.y, x.z, dc, dc * dc};
 #pragma unroll
 			for(int c = 0; c < 5; ++c) {
 				const T hi = T(v[c]);
-				sb[(0 * (kIrrCols + kDepCols) + col[c]) * kBStride + sk + q] = hi;
-				if(P == 2) sb[(1 * (kIrrCols + kDepCols) + col[c]) * kBStride + sk + q] = T(v[c] - float(hi));
+				A[c][0][j] = hi;
+				if(P == 2) A[c][1][j] = T(v[c] - float(hi));
 			}
 		}
-		__syncthreads();
+		if(s + 2 < steps) load_rays(s + 2, r);
+		const unsigned char* wb = lds.w[b] + lane * 16;
 #pragma unroll
-		for(int ks = 0; ks < kMfmaChunk; ks += 32) {
-			const uint32_t kg = k0 + ks + lgrp * 8;  // this lane's 8 K values (same mapping for A and B)
+		for(int tt = 0; tt < 16; ++tt) {
+			V B[2];
 #pragma unroll
-			for(int q = 0; q < 4; ++q) {
-				const int mt = wave + 4 * q;            // 0..2 irradiance, 3..15 depth
-				const bool irr = mt < 3;
-				const int wrow = irr ? mt * 16 + lrow : kDepRow0 + (mt - 3) * 16 + lrow;
-				V A[2];
+			for(int p = 0; p < P; ++p) B[p] = *reinterpret_cast<const V*>(wb + (p * 16 + tt) * 1024);
+			if(tt < 3) {
 #pragma unroll
-				for(int p = 0; p < P; ++p) A[p] = *reinterpret_cast<const V*>(wparts[p] + size_t(wrow) * n_pad + kg);
-				const int nt_count = irr ? 3 : 2;
-#pragma unroll
-				for(int nt = 0; nt < 3; ++nt) {
-					if(nt >= nt_count) break;
-					const int col = irr ? nt * 16 + lrow : kIrrCols + nt * 16 + lrow;
-					V B[2];
-#pragma unroll
-					for(int p = 0; p < P; ++p)
-						B[p] = *reinterpret_cast<const V*>(sb + (p * (kIrrCols + kDepCols) + col) * kBStride + ks + lgrp * 8);
-					acc[q][nt] = M::mma(A[0], B[0], acc[q][nt]);
+				for(int c = 0; c < 3; ++c) {
+					acc_i[c][tt] = M::mma(A[c][0], B[0], acc_i[c][tt]);
 					if(P == 2) {
-						acc[q][nt] = M::mma(A[0], B[1], acc[q][nt]);
-						acc[q][nt] = M::mma(A[1], B[0], acc[q][nt]);
+						acc_i[c][tt] = M::mma(A[c][1], B[0], acc_i[c][tt]);
+						acc_i[c][tt] = M::mma(A[c][0], B[1], acc_i[c][tt]);
+					}
+				}
+			} else {
+#pragma unroll
+				for(int c = 0; c < 2; ++c) {
+					acc_d[c][tt - 3] = M::mma(A[3 + c][0], B[0], acc_d[c][tt - 3]);
+					if(P == 2) {
+						acc_d[c][tt - 3] = M::mma(A[3 + c][1], B[0], acc_d[c][tt - 3]);
+						acc_d[c][tt - 3] = M::mma(A[3 + c][0], B[1], acc_d[c][tt - 3]);
 					}
 				}
 			}
 		}
+	};
+
+	float4 ra[8], rb[8];
+	stage_weights(0, 0);
+	load_rays(0, ra);
+	if(steps > 1) load_rays(1, rb);
+#pragma unroll
+	for(uint32_t s = 0; s < steps; s += 2) {
+		step(s, 0, ra);
+		if(s + 1 < steps) step(s + 1, 1, rb);
 	}
-	__syncthreads();  // staging done: reuse LDS for the accumulators
+
+	// out-of-range counts: the four lanes of a probe row (l % 16 equal)
+	my_oor += __shfl_xor(my_oor, 16);
+	my_oor += __shfl_xor(my_oor, 32);
+	if(lgrp == 0) s_oor[my_slot] = my_oor;
+	if(t < kWideProbes) s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
+	// epilogue, half of the probes at a time: waves 2h, 2h + 1 store their sums (D row 4 (l / 16) + r
+	// = probe, column l % 16 = texel), then all four waves blend, store and reduce those tiles
+	float* s_irr = lds.sums;
+	float* s_dep = s_irr + kWideHalf * kIrrStride;
 #pragma unroll
-	for(int q = 0; q < 4; ++q) {
-		const int mt = wave + 4 * q;
-		const bool irr = mt < 3;
-#pragma unroll
-		for(int nt = 0; nt < 3; ++nt) {
-			if(!irr && nt == 2) break;
+	for(int half = 0; half < 2; ++half) {
+		const uint32_t first = uint32_t(half * kWideHalf);
+		if(first >= np) break;
+		__syncthreads();
+		if((wave >> 1) == half) {
 #pragma unroll
 			for(int r = 0; r < 4; ++r) {
-				const int row = 4 * lgrp + r;
-				if(irr) s_out_irr[(mt * 16 + row) * kIrrCols + nt * 16 + lrow] = acc[q][nt][r];
-				else s_out_dep[((mt - 3) * 16 + row) * kDepCols + nt * 16 + lrow] = acc[q][nt][r];
+				const int slot = 16 * (wave & 1) + 4 * lgrp + r;
+#pragma unroll
+				for(int tt = 0; tt < 3; ++tt) {
+					const int tex = 16 * tt + lrow;
+					if(tex < int(kIrrTexels)) {
+#pragma unroll
+						for(int c = 0; c < 3; ++c) s_irr[slot * kIrrStride + tex * 4 + c] = acc_i[c][tt][r];
+					}
+				}
+#pragma unroll
+				for(int tt = 0; tt < 13; ++tt) {
+					const int tex = 16 * tt + lrow;
+					if(tex < int(kDepTexels)) {
+						s_dep[slot * kDepStride + tex * 2 + 0] = acc_d[0][tt][r];
+						s_dep[slot * kDepStride + tex * 2 + 1] = acc_d[1][tt][r];
+					}
+				}
 			}
 		}
+		__syncthreads();
+		blend_epilogue<16>(a, s_idx + first, min(uint32_t(kWideHalf), np - first), s_irr, s_dep, s_maxc + first, false);
 	}
-	// out-of-range counts: the 16 stagers of a probe are 16 consecutive lanes of one wave
-	for(int off = 8; off > 0; off >>= 1) my_oor += __shfl_xor(my_oor, off, 16);
-	if((t & 15) == 0) s_oor[sp] = my_oor;
-	if(t < kProbesPerBlock) {
-		s_maxc[t] = 0;
-		s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
-	}
-	__syncthreads();
-	blend_epilogue(a, s_idx, np, s_out_irr, kIrrCols, s_out_dep, kDepCols, s_maxc, false);
 	__syncthreads();
 	if(uint32_t(t) < np) update_state(a.states, s_idx[t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
-template <uint32_t MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_blend_mfma(BlendArgs a, uint32_t n_pad) {
-	blend_mfma_tile<MODE>(a, blockIdx.x * kProbesPerBlock, n_pad);
+// one wave per SIMD (a workgroup per CU): up to 512 registers per lane for the accumulators and
+// two K steps of prefetched ray data
+#define DDGI_WIDE_LDS                                                                             \
+	__shared__ __attribute__((aligned(16))) unsigned char wide_w0[kWideWBytes];                   \
+	__shared__ __attribute__((aligned(16))) unsigned char wide_w1[kWideWBytes];                   \
+	__shared__ __attribute__((aligned(16))) float wide_sums[kWideHalf * (kIrrStride + kDepStride)]; \
+	__shared__ uint32_t wide_small[3 * kWideProbes];                                              \
+	const WideLds lds{{wide_w0, wide_w1}, wide_sums, wide_small, wide_small + kWideProbes, wide_small + 2 * kWideProbes}
+
+template <uint32_t MODE, int STEPS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_blend_wide(BlendArgs a) {
+	DDGI_WIDE_LDS;
+	blend_mfma_wide<MODE, STEPS>(a, blockIdx.x * kWideProbes, lds);
 }
 
 // ---- batched update ----
@@ -532,15 +644,57 @@ __global__ __launch_bounds__(256) void k_batch_blend(BatchLaunch b) {
 	const uint32_t vol = __builtin_amdgcn_readfirstlane(b.blend_items[blockIdx.x].volume);
 	const uint32_t first = __builtin_amdgcn_readfirstlane(b.blend_items[blockIdx.x].first);
 	const BatchVolume& v = b.vols[vol];
-	if constexpr(MODE == kBlendFP32) blend_fp32_tile<NP>(v.blend, first, v.n_pad);
-	else blend_mfma_tile<MODE>(v.blend, first, v.n_pad);
+	blend_fp32_tile<NP>(v.blend, first, v.n_pad);
 	__syncthreads();
 	if(threadIdx.x == 0) atomicMax(&b.ctl->done[vol], (unsigned long long)wall_clock64());
 }
 
+// the MFMA blends of a batch: one workgroup per 64-slot item, as k_blend_wide
+template <uint32_t MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_batch_blend_wide(BatchLaunch b) {
+	const uint32_t vol = __builtin_amdgcn_readfirstlane(b.blend_items[blockIdx.x].volume);
+	const uint32_t first = __builtin_amdgcn_readfirstlane(b.blend_items[blockIdx.x].first);
+	const BatchVolume& v = b.vols[vol];
+	DDGI_WIDE_LDS;
+	switch(__builtin_amdgcn_readfirstlane(v.n_pad) / kKStep) {
+		case 2: blend_mfma_wide<MODE, 2>(v.blend, first, lds); break;
+		case 4: blend_mfma_wide<MODE, 4>(v.blend, first, lds); break;
+		case 6: blend_mfma_wide<MODE, 6>(v.blend, first, lds); break;
+		case 8: blend_mfma_wide<MODE, 8>(v.blend, first, lds); break;
+		case 10: blend_mfma_wide<MODE, 10>(v.blend, first, lds); break;
+		case 12: blend_mfma_wide<MODE, 12>(v.blend, first, lds); break;
+		case 14: blend_mfma_wide<MODE, 14>(v.blend, first, lds); break;
+		case 16: blend_mfma_wide<MODE, 16>(v.blend, first, lds); break;
+	}
+	__syncthreads();
+	if(threadIdx.x == 0) atomicMax(&b.ctl->done[vol], (unsigned long long)wall_clock64());
+}
+
+template <uint32_t MODE>
+const void* wide_blend_kernel_of(uint32_t steps) {
+	switch(steps) {
+		case 2: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 2>);
+		case 4: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 4>);
+		case 6: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 6>);
+		case 8: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 8>);
+		case 10: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 10>);
+		case 12: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 12>);
+		case 14: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 14>);
+		case 16: return reinterpret_cast<const void*>(&k_blend_wide<MODE, 16>);
+	}
+	return nullptr;
+}
+
+// The wide MFMA blend for `steps` K steps of 32 rays (n_pad = 32 steps, a multiple of 64 up to 512).
+const void* wide_blend_kernel(uint32_t mode, uint32_t steps) {
+	if(mode == kBlendBF16) return wide_blend_kernel_of<kBlendBF16>(steps);
+	if(mode == kBlendFP16) return wide_blend_kernel_of<kBlendFP16>(steps);
+	return nullptr;
+}
+
 const void* batch_blend_kernel(uint32_t mode, uint32_t probes) {
-	if(mode == kBlendBF16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend<kBlendBF16, 16>) : nullptr;
-	if(mode == kBlendFP16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend<kBlendFP16, 16>) : nullptr;
+	if(mode == kBlendBF16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend_wide<kBlendBF16>) : nullptr;
+	if(mode == kBlendFP16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend_wide<kBlendFP16>) : nullptr;
 	switch(probes) {
 		case 4: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 4>);
 		case 8: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 8>);
@@ -562,12 +716,16 @@ hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, const ui
 
 hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s) {
 	if(a.max_slots == 0) return hipSuccess;
-	const uint32_t blocks = (a.max_slots + kProbesPerBlock - 1) / kProbesPerBlock;
 	const uint32_t n_pad = (a.grid.n + kMfmaChunk - 1) & ~uint32_t(kMfmaChunk - 1);
-	if(mode == kBlendBF16) hipLaunchKernelGGL(k_blend_mfma<kBlendBF16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
-	else if(mode == kBlendFP16) hipLaunchKernelGGL(k_blend_mfma<kBlendFP16>, dim3(blocks), dim3(256), 0, s, a, n_pad);
-	else hipLaunchKernelGGL(k_blend_fp32, dim3(blocks), dim3(256), 0, s, a, n_pad);
-	return hipGetLastError();
+	if(mode == kBlendFP32) {
+		hipLaunchKernelGGL(k_blend_fp32, dim3((a.max_slots + kProbesPerBlock - 1) / kProbesPerBlock), dim3(256), 0, s, a, n_pad);
+		return hipGetLastError();
+	}
+	const void* f = wide_blend_kernel(mode, n_pad / kKStep);
+	if(!f) return hipErrorInvalidValue;
+	BlendArgs arg = a;
+	void* args[] = {&arg};
+	return hipLaunchKernel(f, dim3((a.max_slots + kWideProbes - 1) / kWideProbes), dim3(256), args, 0, s);
 }
 
 hipError_t launch_batch_weights(const BatchLaunch& b, hipStream_t s) {

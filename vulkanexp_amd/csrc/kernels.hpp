@@ -270,7 +270,7 @@ constexpr uint32_t kBlendFP16 = 2;
 
 // ---- batched update of many probe volumes (one graph launch per update) ----
 constexpr uint32_t kMaxBatchVolumes = 256;
-constexpr uint32_t kMfmaProbes = 16;   // probes per MFMA blend tile
+constexpr uint32_t kMfmaProbes = 64;   // probes per MFMA blend workgroup (BF16 / FP16)
 
 // Everything the kernels of one volume read during a batched update.
 struct BatchVolume {
