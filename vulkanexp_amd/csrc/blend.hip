@@ -22,8 +22,14 @@ namespace {
 
 constexpr int kProbesPerBlock = 16;
 
+// Column of ray i in the 16-bit weight rows, which only the wide MFMA blend reads: inside each
+// block of 32 rays, ray 4 j + g is stored at 8 g + j, so that the 8 consecutive weights a lane
+// of K block g takes are the rays 4 j + g that its A fragment holds (loaded as 4 consecutive rays
+// of each of 16 probes per instruction).
+__device__ __forceinline__ uint32_t wide_column(uint32_t i) { return (i & ~31u) | ((i & 3u) << 3) | ((i & 31u) >> 2); }
+
 // One block per padded weight row: the weight of every ray and their sum, stored in [row][n_pad]
-// layout as FP32, or as 16-bit copies (BF16: hi/lo split; FP16: one copy).
+// layout as FP32, or as 16-bit copies (BF16: hi/lo split; FP16: one copy) in wide_column order.
 __device__ __forceinline__ void blend_weight_row(const GridConst& g, const float4* __restrict__ dirs,
 												 const uint16_t* __restrict__ order, float* __restrict__ wt,
 												 float* __restrict__ inv_sum, void* __restrict__ w_hi_v,
@@ -53,12 +59,12 @@ __device__ __forceinline__ void blend_weight_row(const GridConst& g, const float
 			// multiplies, so the rounding of the weights cancels in the normalisation
 			const __bf16 hi = __bf16(w);
 			const __bf16 lo = __bf16(w - float(hi));
-			w_hi[size_t(row) * n_pad + i] = hi;
-			w_lo[size_t(row) * n_pad + i] = lo;
+			w_hi[size_t(row) * n_pad + wide_column(i)] = hi;
+			w_lo[size_t(row) * n_pad + wide_column(i)] = lo;
 			w = float(hi) + float(lo);
 		} else if(mode == kBlendFP16) {
 			const _Float16 h = _Float16(w);
-			w_h[size_t(row) * n_pad + i] = h;
+			w_h[size_t(row) * n_pad + wide_column(i)] = h;
 			w = float(h);
 		}
 		sum += w;
@@ -425,7 +431,8 @@ __device__ __forceinline__ void lds_barrier() {
 	__asm__ volatile("" ::: "memory");
 }
 
-constexpr int kWideProbes = 64;  // probe slots per workgroup (16 per wave)
+constexpr int kWideProbes = 64;   // probe slots per workgroup (16 per probe group)
+constexpr int kWideThreads = 512; // eight waves: four probe groups x two halves of the texel tiles
 constexpr int kKStep = 32;       // rays per MFMA K step
 
 constexpr int kWideWBytes = 2 * 16 * 64 * 16;  // weights of one K step: 2 parts x 16 tiles x 64 lanes x 16 B
@@ -443,67 +450,57 @@ struct WideLds {
 	uint32_t* s_idx;
 };
 
-template <uint32_t MODE, int STEPS>
-__device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slot0, const WideLds& lds) {
+// K loop of one wave of the wide blend: its 16 probes (group q) times the texel tiles of half H
+// (0: the 3 irradiance tiles and depth tiles 3..6; 1: depth tiles 7..15), summed in `acc`; then
+// the sums go to LDS when the epilogue takes this probe group.
+template <uint32_t MODE, int STEPS, int H>
+__device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, uint32_t np, const WideLds& lds) {
 	typedef MfmaTypes<MODE> M;
 	typedef typename M::T T;
 	typedef typename M::V V;
 	constexpr int P = M::kParts;
-	uint32_t* s_oor = lds.s_oor;
-	uint32_t* s_maxc = lds.s_maxc;
-	uint32_t* s_idx = lds.s_idx;
-
-	const uint32_t count = __builtin_amdgcn_readfirstlane(*a.count);  // uniform: see blend_fp32_tile
-	if(slot0 >= count) return;
-	const uint32_t np = min(uint32_t(kWideProbes), count - slot0);
+	constexpr int kTile0 = H == 0 ? 0 : 7, kTile1 = H == 0 ? 7 : 16;  // texel tiles [kTile0, kTile1)
+	constexpr int kCh0 = H == 0 ? 0 : 3;                                 // first data channel used
+	constexpr int kAcc = H == 0 ? 3 * 3 + 2 * 4 : 2 * 9;                 // accumulator tiles
 	const uint32_t n = __builtin_amdgcn_readfirstlane(a.grid.n);
 	constexpr uint32_t steps = STEPS;
 	const uint32_t n_pad = STEPS * kKStep;
 	const float cell_len = a.grid.cell_len;
-	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+	const int t = threadIdx.x, lane = t & 63, wave = t >> 6, q = wave & 3;
 	const int lrow = lane & 15, lgrp = lane >> 4;
-	const uint32_t my_slot = uint32_t(16 * wave + lrow);  // this lane's A row: a probe slot of the workgroup
+	const uint32_t my_slot = uint32_t(16 * q + lrow);  // this lane's A row: a probe slot of the workgroup
 	const bool my_valid = my_slot < np;
 	const float4* __restrict__ my_rays = a.rays + size_t(slot0 + (my_valid ? my_slot : 0u)) * n;
 	const T* __restrict__ wsrc[2] = {static_cast<const T*>(a.w_hi16), static_cast<const T*>(a.w_lo16)};
 
-	// weights of K step s into LDS buffer b: (part, tile) j by wave j % 4, lane l copies texel row
-	// 16 tile + l % 16, rays 32 s + 8 (l / 16) .. +7
+	// weights of K step s into LDS buffer b: (part, tile) j by wave j % 8, lane l copies texel row
+	// 16 tile + l % 16, K block l / 16 (8 weights in wide_column order)
 	auto stage_weights = [&](uint32_t s, int b) {
 #pragma unroll
-		for(int i = 0; i < P * 4; ++i) {
-			const int j = wave + 4 * i, part = j >> 4, tile = j & 15;
+		for(int i = 0; i < P * 2; ++i) {
+			const int j = wave + 8 * i, part = j >> 4, tile = j & 15;
 			const T* src = wsrc[part] + size_t(16 * tile + lrow) * n_pad + s * kKStep + lgrp * 8;
 			unsigned char* dst = lds.w[b] + (part * 16 + tile) * 1024;
 			lds_copy16(src, dst);
 		}
 	};
-	// ray data of K step s: rays 32 s + 8 (l / 16) .. +7 of this lane's probe.  The loads are
+	// ray data of K step s: lane (probe l % 16, K block g = l / 16) takes rays 32 s + 4 j + g,
+	// j = 0..7, so that load j of the wave reads 4 consecutive rays (64 bytes) of each of its 16
+	// probes (the weights are stored in the matching wide_column order).  The loads are
 	// unconditional (clamped to the probe's last ray, or to slot0's rays for an absent probe) so
-	// that they issue back to back without branches; the rays that do not exist (beyond n, absent
-	// probes) are replaced by zero where they are used, ray j of the step being valid when
-	// j < valid_rays(s).
+	// that they issue back to back without branches; rays that do not exist (beyond n, absent
+	// probes) are replaced by zero where they are used.
 	auto load_rays = [&](uint32_t s, float4* r) {
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
-			const uint32_t k = s * kKStep + uint32_t(lgrp * 8 + j);
+			const uint32_t k = s * kKStep + uint32_t(4 * j + lgrp);
 			r[j] = my_rays[min(k, n - 1u)];
 		}
 	};
-	auto valid_rays = [&](uint32_t s) -> int {
-		const int first = int(s * kKStep) + lgrp * 8;
-		return my_valid ? max(0, min(8, int(n) - first)) : 0;
-	};
 
-	floatx4_t acc_i[3][3], acc_d[2][13];
+	floatx4_t acc[kAcc];
 #pragma unroll
-	for(int c = 0; c < 3; ++c)
-#pragma unroll
-		for(int tt = 0; tt < 3; ++tt) acc_i[c][tt] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-	for(int c = 0; c < 2; ++c)
-#pragma unroll
-		for(int tt = 0; tt < 13; ++tt) acc_d[c][tt] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+	for(int k = 0; k < kAcc; ++k) acc[k] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
 	uint32_t my_oor = 0;
 
 	// one K step on the data in r (rays of step s) and LDS buffer b; then r receives step s + 2
@@ -514,19 +511,19 @@ __device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slo
 		else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
 		lds_barrier();
 		if(s + 1 < steps) stage_weights(s + 1, b ^ 1);
-		// A fragments (hi / lo) of the five data channels
+		// A fragments (hi / lo) of the data channels this half uses
 		V A[5][2];
-		const int nvalid = valid_rays(s);
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
 			// rays that do not exist are zero: in range, and nothing in any column
-			const float4 x = j < nvalid ? r[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-			if(x.w < 0.0f || x.w > cell_len) ++my_oor;
+			const bool valid = my_valid && s * kKStep + uint32_t(4 * j + lgrp) < n;
+			const float4 x = valid ? r[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			if(H == 0 && (x.w < 0.0f || x.w > cell_len)) ++my_oor;
 			float dc = fminf(cell_len, x.w);
 			if(dc < 0.0f) dc = cell_len;
 			const float v[5] = {x.x, x.y, x.z, dc, dc * dc};
 #pragma unroll
-			for(int c = 0; c < 5; ++c) {
+			for(int c = kCh0; c < 5; ++c) {
 				const T hi = T(v[c]);
 				A[c][0][j] = hi;
 				if(P == 2) A[c][1][j] = T(v[c] - float(hi));
@@ -534,29 +531,23 @@ __device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slo
 		}
 		if(s + 2 < steps) load_rays(s + 2, r);
 		const unsigned char* wb = lds.w[b] + lane * 16;
+		// per texel tile, the products in the order hi x hi, lo x hi, hi x lo for every channel, so
+		// that consecutive MFMAs update different accumulators
 #pragma unroll
-		for(int tt = 0; tt < 16; ++tt) {
+		for(int tt = kTile0; tt < kTile1; ++tt) {
 			V B[2];
 #pragma unroll
 			for(int p = 0; p < P; ++p) B[p] = *reinterpret_cast<const V*>(wb + (p * 16 + tt) * 1024);
-			if(tt < 3) {
+			const int c0 = tt < 3 ? 0 : 3, nc = tt < 3 ? 3 : 2;
+			// accumulator of (channel c, tile tt): irradiance 3 tt + c; depth base + 2 (tt - first) + c
+			const int k0 = tt < 3 ? 3 * tt : (H == 0 ? 9 + 2 * (tt - 3) : 2 * (tt - 7));
 #pragma unroll
-				for(int c = 0; c < 3; ++c) {
-					acc_i[c][tt] = M::mma(A[c][0], B[0], acc_i[c][tt]);
-					if(P == 2) {
-						acc_i[c][tt] = M::mma(A[c][1], B[0], acc_i[c][tt]);
-						acc_i[c][tt] = M::mma(A[c][0], B[1], acc_i[c][tt]);
-					}
-				}
-			} else {
+			for(int c = 0; c < nc; ++c) acc[k0 + c] = M::mma(A[c0 + c][0], B[0], acc[k0 + c]);
+			if(P == 2) {
 #pragma unroll
-				for(int c = 0; c < 2; ++c) {
-					acc_d[c][tt - 3] = M::mma(A[3 + c][0], B[0], acc_d[c][tt - 3]);
-					if(P == 2) {
-						acc_d[c][tt - 3] = M::mma(A[3 + c][1], B[0], acc_d[c][tt - 3]);
-						acc_d[c][tt - 3] = M::mma(A[3 + c][0], B[1], acc_d[c][tt - 3]);
-					}
-				}
+				for(int c = 0; c < nc; ++c) acc[k0 + c] = M::mma(A[c0 + c][1], B[0], acc[k0 + c]);
+#pragma unroll
+				for(int c = 0; c < nc; ++c) acc[k0 + c] = M::mma(A[c0 + c][0], B[1], acc[k0 + c]);
 			}
 		}
 	};
@@ -572,12 +563,13 @@ __device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slo
 	}
 
 	// out-of-range counts: the four lanes of a probe row (l % 16 equal)
-	my_oor += __shfl_xor(my_oor, 16);
-	my_oor += __shfl_xor(my_oor, 32);
-	if(lgrp == 0) s_oor[my_slot] = my_oor;
-	if(t < kWideProbes) s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
-	// epilogue, half of the probes at a time: waves 2h, 2h + 1 store their sums (D row 4 (l / 16) + r
-	// = probe, column l % 16 = texel), then all four waves blend, store and reduce those tiles
+	if(H == 0) {
+		my_oor += __shfl_xor(my_oor, 16);
+		my_oor += __shfl_xor(my_oor, 32);
+		if(lgrp == 0) lds.s_oor[my_slot] = my_oor;
+	}
+	// epilogue, half of the probes at a time: groups 2h, 2h + 1 store their sums (D row 4 (l / 16) +
+	// r = probe, column l % 16 = texel), then all eight waves blend, store and reduce those tiles
 	float* s_irr = lds.sums;
 	float* s_dep = s_irr + kWideHalf * kIrrStride;
 #pragma unroll
@@ -585,37 +577,51 @@ __device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slo
 		const uint32_t first = uint32_t(half * kWideHalf);
 		if(first >= np) break;
 		__syncthreads();
-		if((wave >> 1) == half) {
+		if((q >> 1) == half) {
 #pragma unroll
 			for(int r = 0; r < 4; ++r) {
-				const int slot = 16 * (wave & 1) + 4 * lgrp + r;
+				const int slot = 16 * (q & 1) + 4 * lgrp + r;
 #pragma unroll
-				for(int tt = 0; tt < 3; ++tt) {
-					const int tex = 16 * tt + lrow;
-					if(tex < int(kIrrTexels)) {
+				for(int tt = kTile0; tt < kTile1; ++tt) {
+					const int tex = 16 * (tt < 3 ? tt : tt - 3) + lrow;
+					const int k0 = tt < 3 ? 3 * tt : (H == 0 ? 9 + 2 * (tt - 3) : 2 * (tt - 7));
+					if(tt < 3) {
+						if(tex < int(kIrrTexels)) {
 #pragma unroll
-						for(int c = 0; c < 3; ++c) s_irr[slot * kIrrStride + tex * 4 + c] = acc_i[c][tt][r];
-					}
-				}
-#pragma unroll
-				for(int tt = 0; tt < 13; ++tt) {
-					const int tex = 16 * tt + lrow;
-					if(tex < int(kDepTexels)) {
-						s_dep[slot * kDepStride + tex * 2 + 0] = acc_d[0][tt][r];
-						s_dep[slot * kDepStride + tex * 2 + 1] = acc_d[1][tt][r];
+							for(int c = 0; c < 3; ++c) s_irr[slot * kIrrStride + tex * 4 + c] = acc[k0 + c][r];
+						}
+					} else if(tex < int(kDepTexels)) {
+						s_dep[slot * kDepStride + tex * 2 + 0] = acc[k0][r];
+						s_dep[slot * kDepStride + tex * 2 + 1] = acc[k0 + 1][r];
 					}
 				}
 			}
 		}
 		__syncthreads();
-		blend_epilogue<16>(a, s_idx + first, min(uint32_t(kWideHalf), np - first), s_irr, s_dep, s_maxc + first, false);
+		blend_epilogue<4>(a, lds.s_idx + first, min(uint32_t(kWideHalf), np - first), s_irr, s_dep, lds.s_maxc + first,
+						  false);
 	}
-	__syncthreads();
-	if(uint32_t(t) < np) update_state(a.states, s_idx[t], __uint_as_float(s_maxc[t]), s_oor[t], n);
 }
 
-// one wave per SIMD (a workgroup per CU): up to 512 registers per lane for the accumulators and
-// two K steps of prefetched ray data
+template <uint32_t MODE, int STEPS>
+__device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slot0, const WideLds& lds) {
+	const uint32_t count = __builtin_amdgcn_readfirstlane(*a.count);  // uniform: see blend_fp32_tile
+	if(slot0 >= count) return;
+	const uint32_t np = min(uint32_t(kWideProbes), count - slot0);
+	const int t = threadIdx.x;
+	if(t < kWideProbes) lds.s_idx[t] = uint32_t(t) < np ? a.indices[slot0 + t] : 0u;
+	// waves 0..3 take the irradiance tiles and the first depth tiles of probe groups 0..3, waves
+	// 4..7 the other depth tiles of the same groups (wave-uniform branch: every wave meets the same
+	// barriers)
+	if(__builtin_amdgcn_readfirstlane(t >> 6) < 4) wide_wave<MODE, STEPS, 0>(a, slot0, np, lds);
+	else wide_wave<MODE, STEPS, 1>(a, slot0, np, lds);
+	__syncthreads();
+	if(uint32_t(t) < np)
+		update_state(a.states, lds.s_idx[t], __uint_as_float(lds.s_maxc[t]), lds.s_oor[t], __builtin_amdgcn_readfirstlane(a.grid.n));
+}
+
+// eight waves, two per SIMD (a workgroup per CU): up to 256 registers per lane for a wave's
+// accumulators and two K steps of prefetched ray data
 #define DDGI_WIDE_LDS                                                                             \
 	__shared__ __attribute__((aligned(16))) unsigned char wide_w0[kWideWBytes];                   \
 	__shared__ __attribute__((aligned(16))) unsigned char wide_w1[kWideWBytes];                   \
@@ -624,7 +630,7 @@ __device__ __forceinline__ void blend_mfma_wide(const BlendArgs& a, uint32_t slo
 	const WideLds lds{{wide_w0, wide_w1}, wide_sums, wide_small, wide_small + kWideProbes, wide_small + 2 * kWideProbes}
 
 template <uint32_t MODE, int STEPS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_blend_wide(BlendArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blend_wide(BlendArgs a) {
 	DDGI_WIDE_LDS;
 	blend_mfma_wide<MODE, STEPS>(a, blockIdx.x * kWideProbes, lds);
 }
@@ -651,7 +657,7 @@ __global__ __launch_bounds__(256) void k_batch_blend(BatchLaunch b) {
 
 // the MFMA blends of a batch: one workgroup per 64-slot item, as k_blend_wide
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_batch_blend_wide(BatchLaunch b) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_batch_blend_wide(BatchLaunch b) {
 	const uint32_t vol = __builtin_amdgcn_readfirstlane(b.blend_items[blockIdx.x].volume);
 	const uint32_t first = __builtin_amdgcn_readfirstlane(b.blend_items[blockIdx.x].first);
 	const BatchVolume& v = b.vols[vol];
@@ -725,7 +731,7 @@ hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s) {
 	if(!f) return hipErrorInvalidValue;
 	BlendArgs arg = a;
 	void* args[] = {&arg};
-	return hipLaunchKernel(f, dim3((a.max_slots + kWideProbes - 1) / kWideProbes), dim3(256), args, 0, s);
+	return hipLaunchKernel(f, dim3((a.max_slots + kWideProbes - 1) / kWideProbes), dim3(kWideThreads), args, 0, s);
 }
 
 hipError_t launch_batch_weights(const BatchLaunch& b, hipStream_t s) {
@@ -738,7 +744,7 @@ hipError_t launch_batch_blend(const BatchLaunch& b, hipStream_t s) {
 	if(!f || b.blend_item_count == 0) return hipErrorInvalidValue;
 	BatchLaunch arg = b;
 	void* args[] = {&arg};
-	return hipLaunchKernel(f, dim3(b.blend_item_count), dim3(256), args, 0, s);
+	return hipLaunchKernel(f, dim3(b.blend_item_count), dim3(b.blend_mode == kBlendFP32 ? 256 : kWideThreads), args, 0, s);
 }
 
 }  // namespace ddgi
