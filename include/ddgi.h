@@ -358,6 +358,16 @@ ddgi_status ddgi_direct_light(void* handle, const ddgi_gbuffer_texel* gbuffer, u
  * than it holds. */
 ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
 							const float* noise, float* out_rgba, void* stream);
+/* ddgi_reflection without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as ddgi_reflection, for at most
+ * 2,097,152 texels per call (a 1920 x 1080 view fits; larger counts are refused with
+ * DDGI_ERROR_INVALID_ARGUMENT: split the G-buffer into calls).  Its scratch is one 64.5 MiB array
+ * per GPU, part of the library's code object and held until the process ends; every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams, other handles, other
+ * threads) race and must be serialised by the caller. */
+ddgi_status ddgi_reflection_noalloc(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count,
+									const float view_origin[3], const float* noise, float* out_rgba, void* stream);
 
 /* ---- differentiable probes (no counterpart in the reference, which is not differentiable) ----
  * Gradients of a shaded-G-buffer loss with respect to per-material albedo and the sun radiance,

@@ -124,6 +124,7 @@ _SIGNATURES = {
     "ddgi_gather": (C.c_int, [_P, _P, _U64, _P, _P, _P, _P, _P]),
     "ddgi_direct_light": (C.c_int, [_P, _P, _U64, _P, _P, _P, _P]),
     "ddgi_reflection": (C.c_int, [_P, _P, _U64, _P, _P, _P, _P]),
+    "ddgi_reflection_noalloc": (C.c_int, [_P, _P, _U64, _P, _P, _P, _P]),
     "ddgi_batch_create": (C.c_int, [_P, _U32, _P, C.POINTER(C.c_void_p)]),
     "ddgi_batch_destroy": (C.c_int, [_P]),
     "ddgi_batch_set_config": (C.c_int, [_P, _P]),

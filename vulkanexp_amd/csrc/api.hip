@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -316,6 +317,79 @@ void advance_hysteresis(Handle* h, uint32_t updated) {
 	h->updated_probes += updated;
 }
 
+// ---- device scratch of the allocation-free reflection pass (ddgi_reflection_noalloc) ----
+// One array per GPU the library runs on, held until the process ends: 32 bytes per texel for up to
+// kNoAllocTexels texels (hit record, queue entry, key, the bucketed key and entry of the sort),
+// then the queue length and the sort counters.  Every call on a GPU shares that GPU's copy.
+constexpr uint64_t kNoAllocTexels = uint64_t(1) << 21;  // 2,097,152 texels (a 1920 x 1080 view fits)
+constexpr size_t kNoAllocCountOffset = size_t(kNoAllocTexels) * 32;
+constexpr size_t kNoAllocCountersOffset = kNoAllocCountOffset + 256;
+constexpr size_t kNoAllocBytes = kNoAllocCountersOffset + size_t(kShadeSortCounts) * 4;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxNoAllocDevices] = {};
+
+// The device address of `scratch` on the handle's device (current), looked up once per device.
+// In host code the array's name is only a stand-in: kernels and copies take this address.
+ddgi_status noalloc_scratch(Handle* h, unsigned char** out) {
+	if(h->device < 0 || h->device >= kMaxNoAllocDevices)
+		return fail(h, DDGI_ERROR_UNSUPPORTED, "device ordinal above the allocation-free scratch table");
+	std::lock_guard<std::mutex> lk(g_noalloc_mu);
+	if(!g_noalloc_ptr[h->device]) {
+		void* ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || !ptr)
+			return fail(h, DDGI_ERROR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+		g_noalloc_ptr[h->device] = ptr;
+	}
+	*out = static_cast<unsigned char*>(g_noalloc_ptr[h->device]);
+	return DDGI_OK;
+}
+
+// Scratch of one reflection chunk of up to `cap` texels.
+struct ReflScratch {
+	float4* hits;
+	uint32_t *list, *keys, *bkeys, *blist, *count, *counts;
+};
+
+// reflection.rgen:101-175 over `count` G-buffer texels in chunks of at most `chunk_cap`, with the
+// scratch `sc` (see ddgi_reflection).
+ddgi_status enqueue_reflection(Handle* h, const ddgi_gbuffer_texel* gbuffer, uint64_t count, const float view_origin[3],
+							   const float* noise, float* out_rgba, const ReflScratch& sc, uint64_t chunk_cap,
+							   hipStream_t s) {
+	ReflectionArgs a{};
+	a.scene = h->sd;
+	a.grid = make_grid_const(*h);
+	a.light = h->light;
+	a.states = h->states;
+	a.irr = h->irr[h->cur];
+	a.dep = h->dep[h->cur];
+	a.gbuffer = gbuffer;
+	a.noise = reinterpret_cast<const float2*>(noise);
+	a.out = reinterpret_cast<float4*>(out_rgba);
+	a.count = count;
+	for(int k = 0; k < 3; ++k) a.origin[k] = view_origin[k];
+	uint32_t key_shift = 0, key_bits = 0;
+	shade_key_layout(a.grid, &key_shift, &key_bits);
+	a.hits = sc.hits;
+	a.list = sc.list;
+	a.keys = sc.keys;
+	a.list_count = sc.count;
+	a.key_shift = key_shift;
+	for(uint64_t first = 0; first < count; first += chunk_cap) {
+		a.first = first;
+		a.chunk = uint32_t(std::min<uint64_t>(chunk_cap, count - first));
+		HIP_TRY(h, launch_clear(sc.count, 1, s));
+		HIP_TRY(h, launch_reflection_trace(a, s));
+		HIP_TRY(h, launch_key_sort(SortJob{sc.keys, sc.list, sc.count, sc.counts, sc.bkeys, sc.blist, nullptr, 0u, 1u, 0u, 0u},
+								   a.chunk, 3 * key_bits, s));
+		HIP_TRY(h, launch_reflection_shade(a, s));
+	}
+	return DDGI_OK;
+}
+
 }  // namespace host
 }  // namespace ddgi
 
@@ -383,6 +457,10 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 		return fail(nullptr, s, m);
 	};
 	if((st = set_device(h)) != DDGI_OK) return cleanup_fail(st);
+	{  // the device address of the allocation-free scratch, once per device
+		unsigned char* ptr = nullptr;
+		if((st = noalloc_scratch(h, &ptr)) != DDGI_OK) return cleanup_fail(st);
+	}
 	h->P = uint32_t(params->resolution[0]) * params->resolution[1] * params->resolution[2];
 	h->max_rays = opts.max_rays_per_probe;
 	const size_t P = h->P, R = h->max_rays;
@@ -1136,18 +1214,6 @@ ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uin
 	if(!view_origin || !finite_all(view_origin, 3)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid view origin");
 	ddgi_status st = set_device(h);
 	if(st != DDGI_OK) return st;
-	ReflectionArgs a{};
-	a.scene = h->sd;
-	a.grid = make_grid_const(*h);
-	a.light = h->light;
-	a.states = h->states;
-	a.irr = h->irr[h->cur];
-	a.dep = h->dep[h->cur];
-	a.gbuffer = gbuffer;
-	a.noise = reinterpret_cast<const float2*>(noise);
-	a.out = reinterpret_cast<float4*>(out_rgba);
-	a.count = count;
-	for(int k = 0; k < 3; ++k) a.origin[k] = view_origin[k];
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	// texels go in chunks of at most 4 Mi; the scratch (32 B per texel of a chunk) grows on demand
 	constexpr uint64_t kChunk = uint64_t(1) << 22;
@@ -1168,23 +1234,36 @@ ddgi_status ddgi_reflection(void* handle, const ddgi_gbuffer_texel* gbuffer, uin
 		}
 		h->refl_cap = cap;
 	}
-	uint32_t key_shift = 0, key_bits = 0;
-	shade_key_layout(a.grid, &key_shift, &key_bits);
-	a.hits = h->refl_hits;
-	a.list = h->refl_list;
-	a.keys = h->refl_keys;
-	a.list_count = h->refl_count;
-	a.key_shift = key_shift;
-	for(uint64_t first = 0; first < count; first += kChunk) {
-		a.first = first;
-		a.chunk = uint32_t(std::min<uint64_t>(kChunk, count - first));
-		HIP_TRY(h, launch_clear(h->refl_count, 1, s));
-		HIP_TRY(h, launch_reflection_trace(a, s));
-		HIP_TRY(h, launch_key_sort(SortJob{h->refl_keys, h->refl_list, h->refl_count, h->refl_counts, h->refl_bkeys,
-										   h->refl_blist, nullptr, 0u, 1u, 0u, 0u},
-								   a.chunk, 3 * key_bits, s));
-		HIP_TRY(h, launch_reflection_shade(a, s));
-	}
+	const ReflScratch sc{h->refl_hits, h->refl_list, h->refl_keys, h->refl_bkeys, h->refl_blist, h->refl_count, h->refl_counts};
+	if((st = enqueue_reflection(h, gbuffer, count, view_origin, noise, out_rgba, sc, kChunk, s)) != DDGI_OK) return st;
+	h->last_stream = s;
+	return DDGI_OK;
+}
+
+ddgi_status ddgi_reflection_noalloc(void* handle, const ddgi_gbuffer_texel* gbuffer, uint64_t count,
+									const float view_origin[3], const float* noise, float* out_rgba, void* stream) {
+	if(!handle) return fail(nullptr, DDGI_ERROR_INVALID_ARGUMENT, "null handle");
+	Handle* h = H(handle);
+	if(!h->has_scene) return fail(h, DDGI_ERROR_NO_SCENE, "ddgi_set_scene must precede ddgi_reflection_noalloc");
+	if(count && (!gbuffer || !out_rgba)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "null G-buffer or output");
+	if(!view_origin || !finite_all(view_origin, 3)) return fail(h, DDGI_ERROR_INVALID_ARGUMENT, "invalid view origin");
+	if(count > kNoAllocTexels)
+		return fail(h, DDGI_ERROR_INVALID_ARGUMENT,
+					"count exceeds the 2,097,152 texels of the allocation-free scratch (split the G-buffer into calls)");
+	ddgi_status st = set_device(h);
+	if(st != DDGI_OK) return st;
+	unsigned char* ptr = nullptr;
+	if((st = noalloc_scratch(h, &ptr)) != DDGI_OK) return st;
+	const size_t N = size_t(kNoAllocTexels);
+	const ReflScratch sc{reinterpret_cast<float4*>(ptr),
+						 reinterpret_cast<uint32_t*>(ptr + N * 16),
+						 reinterpret_cast<uint32_t*>(ptr + N * 20),
+						 reinterpret_cast<uint32_t*>(ptr + N * 24),
+						 reinterpret_cast<uint32_t*>(ptr + N * 28),
+						 reinterpret_cast<uint32_t*>(ptr + kNoAllocCountOffset),
+						 reinterpret_cast<uint32_t*>(ptr + kNoAllocCountersOffset)};
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	if((st = enqueue_reflection(h, gbuffer, count, view_origin, noise, out_rgba, sc, kNoAllocTexels, s)) != DDGI_OK) return st;
 	h->last_stream = s;
 	return DDGI_OK;
 }

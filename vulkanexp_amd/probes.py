@@ -180,6 +180,18 @@ class IrradianceProbes:
                                         _stream_ptr(stream)), self.handle)
         return out
 
+    def reflection_noalloc(self, gbuffer, view_origin, out, noise=None, stream=None):
+        """reflection() without device allocations (ddgi_reflection_noalloc): the caller passes the
+        (N, 4) float32 CUDA output tensor; N <= 2,097,152 texels per call."""
+        gbuffer = gbuffer.contiguous()
+        n = int(gbuffer.shape[0])
+        origin = np.ascontiguousarray(view_origin, np.float32).reshape(3)
+        nz = noise.contiguous() if noise is not None else None
+        check(self._lib.ddgi_reflection_noalloc(self.handle, _tensor_ptr(gbuffer), n, _p(origin),
+                                                _tensor_ptr(nz) if nz is not None else None, _tensor_ptr(out),
+                                                _stream_ptr(stream)), self.handle)
+        return out
+
     def trace_rays(self, rays, mask: int = 0xFF, any_hit: bool = False, stream=None, cull_back: bool = False):
         """rays: CUDA float32 tensor (N, 8) = (origin, tmin, direction, tmax).  Returns a CUDA int32
         tensor (N, 6): t bits, u bits, v bits, instance, primitive, flags."""
