@@ -267,11 +267,12 @@ void destroy_handle(Handle* h) {
 
 // `order`: the ray order of the `rays` records (the traversal's thread order for ddgi_update,
 // nullptr = natural order for caller-provided buffers).
-ddgi_status enqueue_blend_and_publish(Handle* h, const Rot& R, const float4* rays, const uint16_t* order,
+// `R` is null when h->dirs already holds this frame's directions (the update wrote them for its trace).
+ddgi_status enqueue_blend_and_publish(Handle* h, const Rot* R, const float4* rays, const uint16_t* order,
 									  uint32_t max_slots, bool all, uint32_t flags, hipStream_t s) {
 	const GridConst g = make_grid_const(*h);
 	const uint32_t mode = (flags & DDGI_UPDATE_BLEND_BF16) ? kBlendBF16 : ((flags & DDGI_UPDATE_BLEND_FP16) ? kBlendFP16 : kBlendFP32);
-	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s));
+	if(R) HIP_TRY(h, launch_frame_dirs(*R, h->fib, h->dirs, g.n, s));
 	HIP_TRY(h, launch_blend_weights(g, h->dirs, order, h->wt, h->inv_sum, h->w_hi, h->w_lo, mode, s));
 	const int nxt = h->cur ^ 1;
 	if(!all) {
@@ -871,7 +872,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	HIP_TRY(h, launch_shade(a, s));
 	if(h->light.sky_model == DDGI_SKY_ATMOSPHERE) HIP_TRY(h, launch_sky(a, s));
 	if(timing) HIP_TRY(h, hipEventRecord(h->ev[3], s));
-	if((st = enqueue_blend_and_publish(h, R, h->rays, h->perm, max_slots, all, flags, s)) != DDGI_OK) return st;
+	if((st = enqueue_blend_and_publish(h, nullptr, h->rays, h->perm, max_slots, all, flags, s)) != DDGI_OK) return st;
 	if(timing) {
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));
 		HIP_TRY(h, hipEventRecord(h->ev[5], s));
@@ -902,7 +903,7 @@ ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_in
 	HIP_TRY(h, launch_set_count(h->sel, count, s));
 	h->indices_are_iota = false;
 	// the listed probes may not cover the grid: the next atlas starts as a copy of the exposed one
-	if((st = enqueue_blend_and_publish(h, R, static_cast<const float4*>(rays), nullptr, count, false, flags, s)) != DDGI_OK)
+	if((st = enqueue_blend_and_publish(h, &R, static_cast<const float4*>(rays), nullptr, count, false, flags, s)) != DDGI_OK)
 		return st;
 	if(timing) {
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));

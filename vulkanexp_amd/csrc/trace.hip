@@ -166,6 +166,19 @@ __device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const 
 	SurfaceShade r;
 	const float b0 = 1.0f - b1 - b2;
 	const Vec3 position = v3(d.x * t + o[0], d.y * t + o[1], d.z * t + o[2]);
+	// shadow ray toward the light: any hit, mask 0xFF, tmin 0.1, tmax 10000.  It goes first: it needs
+	// only the hit point, so few values stay live across the traversal (the shading terms, live
+	// across it when it came last, spilled 48 bytes per lane at 6 waves per SIMD).
+	RayPrep sr;
+	ray_prepare(position.x, position.y, position.z, light.dir[0], light.dir[1], light.dir[2], sr);
+	uint32_t nn = 0, nt = 0;
+	const HitResult sh = traverse<true, STACK, COUNT>(sc.nodes, sc.tris, sr, 0.1f, 10000.0f, 0xFFu, stack, stride, nn, nt);
+	if(COUNT) {
+		atomicAdd(&visit_counters[2], (unsigned long long)nn);
+		atomicAdd(&visit_counters[3], (unsigned long long)nt);
+		atomicAdd(&visit_counters[4], 1ull);
+	}
+	const bool lit = sh.slot == 0xFFFFFFFFu;
 	// fetch geometry and material records (closesthit.glsl:147-159)
 	const float4* tr = sc.tris + size_t(tslot) * 3;
 	const uint32_t gid = __float_as_uint(tr[0].w), inst = __float_as_uint(tr[1].w);
@@ -199,17 +212,6 @@ __device__ __forceinline__ SurfaceShade shade_surface(const SceneDev& sc, const 
 	r.em_spec = color;
 	const Vec3 indirect = sample_probes(grid, states, irr, dep, position, normal, to_cam);
 	color = color + indirect * diffuse;
-	// shadow ray toward the light: any hit, mask 0xFF, tmin 0.1, tmax 10000
-	RayPrep sr;
-	ray_prepare(position.x, position.y, position.z, light.dir[0], light.dir[1], light.dir[2], sr);
-	uint32_t nn = 0, nt = 0;
-	const HitResult sh = traverse<true, STACK, COUNT>(sc.nodes, sc.tris, sr, 0.1f, 10000.0f, 0xFFu, stack, stride, nn, nt);
-	if(COUNT) {
-		atomicAdd(&visit_counters[2], (unsigned long long)nn);
-		atomicAdd(&visit_counters[3], (unsigned long long)nt);
-		atomicAdd(&visit_counters[4], 1ull);
-	}
-	const bool lit = sh.slot == 0xFFFFFFFFu;
 	float fade = 1.0f;
 	if(lit) {
 		const Vec3 L = v3(light.dir[0], light.dir[1], light.dir[2]);
@@ -492,9 +494,8 @@ __device__ __forceinline__ void shade_ray(const TraceFrameArgs& a, uint32_t k, u
 // One thread per entry of the ordered shade list, over a grid sized for the list's capacity (its
 // length is known only on the device; workgroups past it end at once).  The hardware refills a CU
 // as soon as one of its workgroups ends, so the CUs stay busy to the end of the list, which a
-// resident grid striding over the list does not do.  At least 6 waves per SIMD: the shadow
-// traversal is latency-bound, and the sixth wave hides more latency than the 48 bytes per lane its
-// register budget spills cost (0.461 against 0.469 ms at 5 waves).
+// resident grid striding over the list does not do.  At least 6 waves per SIMD (80 registers): the
+// shadow traversal is latency-bound.
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kShadeBlock, 6) void k_shade(TraceFrameArgs a) {
 	__shared__ uint32_t stack[STACK * kShadeBlock];
@@ -570,7 +571,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_trace(BatchLaunch b) {
 // k_shade): workgroup i takes the kShadeBlock entries of shade item i; entries past a list's length
 // return at once.
 template <int STACK>
-__global__ __launch_bounds__(kShadeBlock, 5) void k_batch_shade(BatchLaunch b) {
+__global__ __launch_bounds__(kShadeBlock, 6) void k_batch_shade(BatchLaunch b) {
 	__shared__ uint32_t stack[STACK * kShadeBlock];
 	const uint32_t vol = __builtin_amdgcn_readfirstlane(b.shade_items[blockIdx.x].volume);
 	const uint32_t first = __builtin_amdgcn_readfirstlane(b.shade_items[blockIdx.x].first);
