@@ -556,10 +556,20 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 	stage_weights(0, 0);
 	load_rays(0, ra);
 	if(steps > 1) load_rays(1, rb);
+	// unrolled up to 8 steps (256 rays); beyond that the compiler's scheduling across the unrolled
+	// steps spills (216 bytes per lane at 16 steps), and a loop over step pairs runs faster
+	if constexpr(STEPS <= 8) {
 #pragma unroll
-	for(uint32_t s = 0; s < steps; s += 2) {
-		step(s, 0, ra);
-		if(s + 1 < steps) step(s + 1, 1, rb);
+		for(uint32_t s = 0; s < steps; s += 2) {
+			step(s, 0, ra);
+			if(s + 1 < steps) step(s + 1, 1, rb);
+		}
+	} else {
+#pragma unroll 1
+		for(uint32_t s = 0; s < steps; s += 2) {
+			step(s, 0, ra);
+			if(s + 1 < steps) step(s + 1, 1, rb);
+		}
 	}
 
 	// out-of-range counts: the four lanes of a probe row (l % 16 equal)
