@@ -470,7 +470,13 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 	const int lrow = lane & 15, lgrp = lane >> 4;
 	const uint32_t my_slot = uint32_t(16 * q + lrow);  // this lane's A row: a probe slot of the workgroup
 	const bool my_valid = my_slot < np;
-	const float4* __restrict__ my_rays = a.rays + size_t(slot0 + (my_valid ? my_slot : 0u)) * n;
+	// the workgroup's rays as a buffer resource (np probes x n rays): loads past its end return
+	// zero, so no address needs a clamp, and the lane's offset stays in one register for the whole
+	// K loop (per-load clamped addresses were reused as MFMA results right after the loads, which
+	// made the compiler wait for every load in flight)
+	const __amdgpu_buffer_rsrc_t rays_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+		const_cast<float4*>(a.rays + size_t(slot0) * n), short(0), int(np * n * 16u), 0x00020000);
+	const uint32_t my_off = (my_slot * n + uint32_t(lgrp)) * 16u;
 	const T* __restrict__ wsrc[2] = {static_cast<const T*>(a.w_hi16), static_cast<const T*>(a.w_lo16)};
 
 	// weights of K step s into LDS buffer b: (part, tile) j by wave j % 8, lane l copies texel row
@@ -487,14 +493,16 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 	// ray data of K step s: lane (probe l % 16, K block g = l / 16) takes rays 32 s + 4 j + g,
 	// j = 0..7, so that load j of the wave reads 4 consecutive rays (64 bytes) of each of its 16
 	// probes (the weights are stored in the matching wide_column order).  The loads are
-	// unconditional (clamped to the probe's last ray, or to slot0's rays for an absent probe) so
-	// that they issue back to back without branches; rays that do not exist (beyond n, absent
-	// probes) are replaced by zero where they are used.
+	// unconditional, so that they issue back to back without branches: rays beyond n read the next
+	// probe's (or nothing, past the workgroup's last probe), and rays that do not exist (beyond n,
+	// absent probes) are replaced by zero where they are used.
 	auto load_rays = [&](uint32_t s, float4* r) {
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
-			const uint32_t k = s * kKStep + uint32_t(4 * j + lgrp);
-			r[j] = my_rays[min(k, n - 1u)];
+			typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+			const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rays_rsrc, my_off + uint32_t(64 * j),
+																   int(s * kKStep * 16u), 0);
+			r[j] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
 		}
 	};
 
@@ -510,6 +518,18 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 		if(s + 1 < steps) __builtin_amdgcn_s_waitcnt(0x0F78);  // vmcnt(8): the step s + 1 rays may stay in flight
 		else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
 		lds_barrier();
+		// the depth half's ray data of step s (distances only) counts as produced here, after the
+		// wait: otherwise the compiler hoists its conversion into an earlier step, where the
+		// in-order load counter (with the weight copies in flight as well) can only be waited on
+		// down to zero
+		if constexpr(H == 1) {
+#pragma unroll
+			for(int j = 0; j < 8; ++j) {
+				float w = r[j].w;
+				__asm__ volatile("" : "+v"(w));
+				r[j].w = w;
+			}
+		}
 		if(s + 1 < steps) stage_weights(s + 1, b ^ 1);
 		// A fragments (hi / lo) of the data channels this half uses
 		V A[5][2];
