@@ -400,21 +400,32 @@ struct MfmaTypes<kBlendFP16> {
 };
 
 // ---------------------------------------------------------------------------------------------
-// MFMA blend, transposed and pipelined (BF16 / FP16 modes).  Per channel the blend of a wave's 16
+// MFMA blend, transposed and pipelined (BF16 / FP16 modes).  Per channel the blend of a group of 16
 // probes is C^T [16 probes x texels] = X^T [16 probes x n rays] W^T [n x texels]: the ray data is
-// the A operand, read by each lane straight from HBM (row = one of the wave's probes, 8
-// consecutive rays per lane and K step = 128 contiguous bytes), and the weights are the B operand,
-// shared by the workgroup's four waves through LDS.  A workgroup owns 64 probe slots (16 per wave)
-// and keeps 140 accumulator registers per lane (irradiance: 3 channels x 3 texel tiles, depth: 2
-// channels x 13 texel tiles), so it runs one wave per SIMD and hides latency by prefetching: the
-// ray data of K step s + 2 is in flight in registers and the weights of step s + 1 are being
-// copied from L2 into LDS (global_load_lds, no registers) while the matrix cores work on step s.
+// the A operand, read by each lane straight from HBM (row = one of the group's probes, 8 rays per
+// lane and K step), and the weights are the B operand, shared by the workgroup's eight waves
+// through LDS.  A workgroup owns 64 probe slots (four groups of 16); each group's texel tiles are
+// split between two waves (irradiance: 3 channels x 3 texel tiles plus depth tiles 3..6; depth
+// tiles 7..15), whose accumulators take up to 72 registers per lane, so it runs two waves per SIMD
+// and hides latency by prefetching: the ray data of K step s + 2 is in flight in registers and
+// the weights of step s + 1 are being copied from L2 into LDS (global_load_lds, no registers)
+// while the matrix cores work on step s.
 // Products and their order are those of the blend above (hi x hi, lo x hi, hi x lo into FP32).
 // LDS weight layout per K step: [part][texel tile][K block (4)][row (16)] x 8 elements, so that a
 // wave's B fragment (lane l: texel row l % 16, K block l / 16) is one contiguous 1 KB read and one
 // 64-lane global_load_lds instruction fills one (part, tile).
 // One 16-byte global -> LDS copy per lane (global_load_lds_dwordx4): the lanes' data lands at
 // lds + 16 * lane (lds must be wave-uniform), without passing through registers.
+// x = hi + lo for two floats at once: hi = bf16(x) (round to nearest even, one packed conversion),
+// lo = bf16(x - hi), where hi's float value is its bits shifted (or masked) into the high half.
+__device__ __forceinline__ void split_bf16x2(float a, float b, uint32_t& hi, uint32_t& lo) {
+	typedef float f32x2 __attribute__((ext_vector_type(2)));
+	typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+	hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+	const float ha = __uint_as_float(hi << 16), hb = __uint_as_float(hi & 0xFFFF0000u);
+	lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a - ha, b - hb}, bf16x2));
+}
+
 __device__ __forceinline__ void lds_copy16(const void* src, void* lds) {
 	__builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
 }
@@ -533,6 +544,7 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 		if(s + 1 < steps) stage_weights(s + 1, b ^ 1);
 		// A fragments (hi / lo) of the data channels this half uses
 		V A[5][2];
+		float v[8][5];
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
 			// rays that do not exist are zero: in range, and nothing in any column
@@ -541,12 +553,25 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 			if(H == 0 && (x.w < 0.0f || x.w > cell_len)) ++my_oor;
 			float dc = fminf(cell_len, x.w);
 			if(dc < 0.0f) dc = cell_len;
-			const float v[5] = {x.x, x.y, x.z, dc, dc * dc};
+			v[j][0] = x.x;
+			v[j][1] = x.y;
+			v[j][2] = x.z;
+			v[j][3] = dc;
+			v[j][4] = dc * dc;
+		}
 #pragma unroll
-			for(int c = kCh0; c < 5; ++c) {
-				const T hi = T(v[c]);
-				A[c][0][j] = hi;
-				if(P == 2) A[c][1][j] = T(v[c] - float(hi));
+		for(int c = kCh0; c < 5; ++c) {
+			if constexpr(P == 2) {
+				// two rays at a time: one packed rounding for the high parts, their exact float
+				// values by a shift and a mask, one packed rounding for the residuals
+				uint32_t hi[4], lo[4];
+#pragma unroll
+				for(int i = 0; i < 4; ++i) split_bf16x2(v[2 * i][c], v[2 * i + 1][c], hi[i], lo[i]);
+				A[c][0] = __builtin_bit_cast(V, hi);
+				A[c][1] = __builtin_bit_cast(V, lo);
+			} else {
+#pragma unroll
+				for(int j = 0; j < 8; ++j) A[c][0][j] = T(v[j][c]);
 			}
 		}
 		if(s + 2 < steps) load_rays(s + 2, r);
