@@ -488,7 +488,7 @@ def achievable_peaks(torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build targets one GPU; multi-GPU sharding is designed but not built "
+        print(json.dumps({"error": "the library and this benchmark run on one GPU; multi-GPU sharding is designed but not built "
                           "(probe z-slabs per GPU with an atlas all-gather), so --gpus must be 1"}))
         sys.exit(2)
     import torch

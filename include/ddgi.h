@@ -34,7 +34,7 @@ typedef enum ddgi_status {
 	DDGI_ERROR_INVALID_ARGUMENT = 1, /* null handle/pointer, bad size, bad grid, NaN/Inf input */
 	DDGI_ERROR_OUT_OF_MEMORY = 2,    /* device or host allocation failed */
 	DDGI_ERROR_HIP = 3,              /* a HIP runtime call failed; see ddgi_last_error */
-	DDGI_ERROR_UNSUPPORTED = 4,      /* valid request this build does not implement */
+	DDGI_ERROR_UNSUPPORTED = 4,      /* valid request the library does not implement */
 	DDGI_ERROR_NO_SCENE = 5,         /* update/init/trace before ddgi_set_scene */
 	DDGI_ERROR_INVALID_STATE = 6,    /* call order violated (e.g. saved state of another grid) */
 	DDGI_ERROR_NO_DEVICE = 7         /* no usable gfx950 device */
