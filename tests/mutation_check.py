@@ -1,0 +1,65 @@
+"""Mutation check of the oracle's pins (CPU only; run by hand: python tests/mutation_check.py).
+
+Each entry below is a plausible slip in the oracle's restatement of the reference shaders.  The
+script applies one slip at a time to a scratch copy of the repository's oracle, rebuilds it, and
+runs every tests/test_oracle_*.py file against it.  A slip that no test catches ("SURVIVED") marks
+a part of the oracle that no CPU check pins.  Nothing here touches the repository itself."""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# (name, text in oracle/ddgi_oracle.cpp, replacement) -- the shader lines each follows:
+# sampleProbes src/shaders/irradiance.glsl:181-234, closest hit src/shaders/closesthit.glsl:137-285
+MUTATIONS = [
+    ("Chebyshev factor squared", "cheb = std::max(cheb * cheb * cheb, T(0));", "cheb = std::max(cheb * cheb, T(0));"),
+    ("trilinear weights swapped",
+     "const V3<T> tri(off[0] ? alpha.x : T(1) - alpha.x, off[1] ? alpha.y : T(1) - alpha.y,\n\t\t\t\t\t\toff[2] ? alpha.z : T(1) - alpha.z);",
+     "const V3<T> tri(off[0] ? T(1) - alpha.x : alpha.x, off[1] ? T(1) - alpha.y : alpha.y,\n\t\t\t\t\t\toff[2] ? T(1) - alpha.z : alpha.z);"),
+    ("back-face weight not squared", "weight *= bfw * bfw + T(0.2);", "weight *= bfw + T(0.2);"),
+    ("crush threshold 0.1", "const T crush = T(0.2);", "const T crush = T(0.1);"),
+    ("crush cubed -> squared", "if(weight < crush) weight *= weight * weight * (T(1) / (crush * crush));",
+     "if(weight < crush) weight *= weight * (T(1) / crush);"),
+    ("trilinear floor 0.001 dropped", "const T tw = tri.x * tri.y * tri.z + T(0.001);", "const T tw = tri.x * tri.y * tri.z;"),
+    ("fallback mix x4 instead of x8", "T(8) * clampT(ctx.total_w, T(0), T(1) / T(8))", "T(4) * clampT(ctx.total_w, T(0), T(1) / T(4))"),
+    ("back-face depth x1 instead of x0.8", "out[3] = t * T(0.8);", "out[3] = t;"),
+    ("diffuse without (1 - f0)", "const V3<T> diffuse = albedo * (V3<T>(1, 1, 1) - f0) * (T(1) - metal);",
+     "const V3<T> diffuse = albedo * (T(1) - metal);"),
+    ("specular looked up along the normal",
+     "const V3<T> reflection = sample_probes<T>(g, states, irr, dep, position, refl_dir, -d);",
+     "const V3<T> reflection = sample_probes<T>(g, states, irr, dep, position, normal, -d);"),
+    ("sun fade on the direct term only", "\t\tcolor = color * fade;\n\t}\n\tif(aux) {", "\t}\n\tif(aux) {"),
+]
+
+
+def main(only=()):
+    tests = sorted(f for f in os.listdir(os.path.join(REPO, "tests")) if f.startswith("test_oracle_"))
+    survived = []
+    for name, old, new in MUTATIONS:
+        if only and name not in only:
+            continue
+        with tempfile.TemporaryDirectory() as d:
+            for sub in ("oracle", "tests", "vulkanexp_amd", "include"):
+                shutil.copytree(os.path.join(REPO, sub), os.path.join(d, sub),
+                                ignore=shutil.ignore_patterns("_build", "_ref", "__pycache__", "*.so"))
+            shutil.copy(os.path.join(REPO, "pytest.ini"), d)
+            path = os.path.join(d, "oracle", "ddgi_oracle.cpp")
+            src = open(path).read()
+            assert src.count(old) >= 1, f"{name}: the original text is missing"
+            open(path, "w").write(src.replace(old, new, 1))  # the first occurrence: the forward code
+            subprocess.run(["make", "-C", os.path.join(d, "oracle")], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+            r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider"] +
+                               [os.path.join("tests", t) for t in tests], cwd=d, capture_output=True, text=True)
+        caught = [l.split("::")[1].split(" ")[0] for l in r.stdout.splitlines() if l.startswith("FAILED")]
+        print(f"{name:38s} {'caught by ' + ', '.join(caught) if r.returncode else 'SURVIVED'}", flush=True)
+        if not r.returncode:
+            survived.append(name)
+    return survived
+
+
+if __name__ == "__main__":
+    sys.exit(1 if main(tuple(sys.argv[1:])) else 0)
