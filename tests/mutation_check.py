@@ -32,6 +32,19 @@ MUTATIONS = [
      "const V3<T> reflection = sample_probes<T>(g, states, irr, dep, position, refl_dir, -d);",
      "const V3<T> reflection = sample_probes<T>(g, states, irr, dep, position, normal, -d);"),
     ("sun fade on the direct term only", "\t\tcolor = color * fade;\n\t}\n\tif(aux) {", "\t}\n\tif(aux) {"),
+    # blend and state machine, src/shaders/probesUpdate.glsl:49-119
+    ("hysteresis applied to the new value", "px[c] = double(mixT(acc[c], prev, h));", "px[c] = double(mixT(prev, acc[c], h));"),
+    ("depth lobe without its exponent", "const T w = std::pow(std::max(T(0), dot(td, rd)), sharp);",
+     "const T w = std::max(T(0), dot(td, rd));"),
+    ("depth of a miss 0 instead of the cell diagonal", "\t\t\t\t\tif(depth < T(0)) depth = cell_len;\n\t\t\t\t\tconst T w = std::pow",
+     "\t\t\t\t\tif(depth < T(0)) depth = T(0);\n\t\t\t\t\tconst T w = std::pow"),
+    ("state step-up threshold 0.03", "if(mc < T(0.02) / T(st)) st = std::min(st + 1u, 8u);",
+     "if(mc < T(0.03) / T(st)) st = std::min(st + 1u, 8u);"),
+    ("all-out-of-range test at half the rays", "if(out_of_range >= n) {", "if(out_of_range >= n / 2) {"),
+    # intersection, the watertight test's interval and the closest-hit tie break
+    ("tmax exclusive", "return t >= r.tmin && t <= r.tmax;", "return t >= r.tmin && t < r.tmax;"),
+    ("ties to the larger triangle id", "(t == best.t && id < best.tri)", "(t == best.t && id > best.tri)"),
+    ("edge functions without the FP64 fallback", "\tif(U == 0.0f || V == 0.0f || W == 0.0f) {", "\tif(false) {"),
 ]
 
 

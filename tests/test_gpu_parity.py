@@ -90,6 +90,20 @@ def test_shadow_rays_any_hit(small):
     probes.destroy()
 
 
+def test_edge_fallback_and_ties_bit_exact():
+    """The two decisions of the intersection that the random scenes never reach: a ray whose FP32
+    edge function is exactly 0 but whose exact one is negative (the FP64 recomputation makes it a
+    miss), and two coincident triangles met at the same t (the lower triangle id wins)."""
+    from test_oracle_pins import edge_and_tie_scene
+    scene, rays = edge_and_tie_scene()
+    p = _make(scene, scenes.grid_params((-1, -1, -1), (11, 1, 5), (2, 2, 2), 64))
+    got = _gpu_trace(p, rays, 0xFF)
+    p.destroy()
+    want, _ = orc.OracleScene(scene).trace(rays)
+    _compare_hits(got, want, "edge / tie rays")
+    assert got[0, 1] == 0xFFFFFFFF and (got[1, 1], got[1, 2]) == (0, 0) and (got[2, 1], got[2, 2]) == (1, 0)
+
+
 def test_init_probes_states_identical(small):
     scene, grid, osc = small
     og = orc.Grid(grid)

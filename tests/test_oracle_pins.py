@@ -205,3 +205,89 @@ def test_sun_below_the_horizon_fades_the_whole_colour():
     front_floor = (hits[..., 1] == 0) & (hits[..., 3] == 0)
     assert front_floor.sum() > 50
     assert np.all(res["rays"][front_floor][:, :3] == 0.0)
+
+
+# ---- intersection: the FP64 edge-function fallback and the closest-hit tie break ----
+_EPS = float(np.finfo(np.float32).eps)
+# Seen along +z from (0, 0, -1), the edge B C of this triangle passes through the ray's projected
+# origin in FP32 arithmetic (Cx By and Cy Bx round to the same float) but not exactly: the exact
+# edge function is -2^-46, so the ray misses, just outside the edge, while A's side is inside.
+EDGE_TRI = np.array([[-1.0, 1.0, 0.0], [1.0, 1.0 + _EPS, 0.0], [-(1.0 + _EPS), -(1.0 + 2 * _EPS), 0.0]], np.float32)
+
+
+def edge_and_tie_scene():
+    """Instance 0: EDGE_TRI.  Instance 1: two coincident triangles (the same three vertices) at
+    z = 5 around x = 10, so that a ray meets both at the same t.  Returns the scene and three rays:
+    through the edge (a miss), just inside the edge triangle, and through the coincident pair."""
+    sb = scenes.SceneBuilder()
+    m = sb.add_material((0.5, 0.5, 0.5))
+    up = np.tile(np.array([0, 0, -1], np.float32), (3, 1))
+    sb.add_instance(sb.add_mesh(EDGE_TRI, up, np.array([[0, 1, 2]]), m))
+    pair = np.array([[9.0, -1.0, 5.0], [11.0, -1.0, 5.0], [10.0, 1.0, 5.0]], np.float32)
+    sb.add_instance(sb.add_mesh(pair, up, np.array([[0, 2, 1], [0, 2, 1]]), m))
+    scene = sb.build((-2, -2, -2), (12, 2, 6))
+    rays = np.array([[0.0, 0.0, -1.0, 0.0, 0.0, 0.0, 1.0, 100.0],
+                     [-1e-3, 1e-3, -1.0, 0.0, 0.0, 0.0, 1.0, 100.0],
+                     [10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 100.0]], np.float32)
+    return scene, rays
+
+
+def test_edge_function_fp64_fallback():
+    a, b, c = EDGE_TRI.astype(np.float32)
+    u32 = np.float32(c[0] * b[1]) - np.float32(c[1] * b[0])        # FP32 edge function of B C
+    u64 = float(c[0]) * float(b[1]) - float(c[1]) * float(b[0])
+    assert u32 == 0.0 and u64 < 0.0
+    ok, _ = orc.tri_intersect32(np.array([0, 0, -1, 0, 0, 0, 1, 100], np.float32), EDGE_TRI)
+    assert not ok, "the FP64 recomputation puts the ray outside the edge"
+    ok, out = orc.tri_intersect32(np.array([-1e-3, 1e-3, -1, 0, 0, 0, 1, 100], np.float32), EDGE_TRI)
+    assert ok and out[0] == 1.0
+
+
+def test_closest_hit_ties_go_to_the_lower_triangle():
+    scene, rays = edge_and_tie_scene()
+    for brute in (False, True):
+        hits, _ = orc.OracleScene(scene).trace(rays, brute=brute)
+        assert hits[0, 1] == 0xFFFFFFFF                               # through the edge: miss
+        assert hits[1, 1] == 0 and hits[1, 2] == 0                    # inside the edge triangle
+        assert hits[2, 1] == 1 and hits[2, 2] == 0                    # the coincident pair: the lower id
+        assert hits[2, 0].view(np.float32) == np.float32(5.0)
+
+
+@pytest.mark.parametrize("state,change,want", [(2, 0.0125, 2), (4, 0.009, 4), (4, 0.0045, 5)])
+def test_state_thresholds(state, change, want):
+    """probesUpdate.glsl:110-118 between the thresholds: for s = 2 a change of 0.0125 lies between
+    0.02 / s and 0.04 / s (no step); for s = 4, 0.009 lies between 0.005 and 0.01 (no step) and
+    0.0045 below 0.005 (one step up)."""
+    n = 64
+    g = orc.Grid(scenes.grid_params((0, 0, 0), (4, 4, 4), (3, 3, 3), n, hysteresis=0.0))
+    dirs = orc.frame_dirs(scenes.orientations(5, 1)[0], n)
+    rays = np.zeros((1, n, 4))
+    rays[0, :, :3] = 1.0 + change
+    rays[0, :, 3] = 0.1
+    irr, dep = g.zero_atlases()
+    irr[:] = 1.0
+    states = np.ones(g.count, np.uint32)
+    states[13] = state
+    mc, _ = orc.blend(g, dirs, rays, [13], states, irr, dep)
+    assert mc[0] == pytest.approx(change, abs=1e-12)
+    assert states[13] == want
+
+
+@pytest.mark.parametrize("out_of_range,want", [(63, 5), (64, 8)])
+def test_state_far_probe_needs_every_ray_out_of_range(out_of_range, want):
+    """A probe goes to state 8 only when all of its rays are out of range (a miss, or a hit beyond
+    the cell diagonal); 63 of 64 keep the normal state machine (probesUpdate.glsl:110-118)."""
+    n = 64
+    g = orc.Grid(scenes.grid_params((0, 0, 0), (4, 4, 4), (3, 3, 3), n, hysteresis=0.0))
+    dirs = orc.frame_dirs(scenes.orientations(5, 1)[0], n)
+    rays = np.zeros((1, n, 4))
+    rays[0, :, :3] = 1.0
+    rays[0, :, 3] = 0.1
+    rays[0, :out_of_range, 3] = -1.0
+    irr, dep = g.zero_atlases()
+    irr[:] = 1.0
+    states = np.ones(g.count, np.uint32)
+    states[13] = 4
+    _, oor = orc.blend(g, dirs, rays, [13], states, irr, dep)
+    assert oor[0] == out_of_range
+    assert states[13] == want
