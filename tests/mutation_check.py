@@ -12,7 +12,7 @@ import tempfile
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
-# (name, text in oracle/ddgi_oracle.cpp, replacement) -- the shader lines each follows:
+# (name, text in oracle/ddgi_oracle.cpp (or the file named fourth), replacement) -- the shader lines each follows:
 # sampleProbes src/shaders/irradiance.glsl:181-234, closest hit src/shaders/closesthit.glsl:137-285
 MUTATIONS = [
     ("Chebyshev factor squared", "cheb = std::max(cheb * cheb * cheb, T(0));", "cheb = std::max(cheb * cheb, T(0));"),
@@ -45,13 +45,31 @@ MUTATIONS = [
     ("tmax exclusive", "return t >= r.tmin && t <= r.tmax;", "return t >= r.tmin && t < r.tmax;"),
     ("ties to the larger triangle id", "(t == best.t && id < best.tri)", "(t == best.t && id > best.tri)"),
     ("edge functions without the FP64 fallback", "\tif(U == 0.0f || V == 0.0f || W == 0.0f) {", "\tif(false) {"),
+    # ray directions, sphericalFibonacci (src/shaders/irradiance.glsl:78-87), in oracle.hpp
+    ("Fibonacci z without the half step", "T cos_t = T(1) - (T(2) * i + T(1)) * (T(1) / n);",
+     "T cos_t = T(1) - (T(2) * i) * (T(1) / n);", "oracle.hpp"),
+    # pbrMetallicRoughness (src/shaders/pbrMetallicRoughness.glsl:8-84)
+    ("roughness not squared", "const T alpha = rough * rough;", "const T alpha = rough;"),
+    ("reflectance at 90 degrees x50", "const T r90 = clampT(reflectance * T(25), T(0), T(1));",
+     "const T r90 = clampT(reflectance * T(50), T(0), T(1));"),
+    ("microfacet distribution clamp 8", "const T D = clampT(k * k * (T(1) / T(kPi)), T(0), T(4));",
+     "const T D = clampT(k * k * (T(1) / T(kPi)), T(0), T(8));"),
+    # border copy (src/shaders/probesCopyBorders.comp)
+    ("left border not mirrored", "cp(0, j, 1, R - 1 - j);", "cp(0, j, 1, j);"),
+    # probe init (src/shaders/probesInit.rgen:29-64) and selection (src/IrradianceProbes.cpp:396-424)
+    ("inside-the-probe test threshold", "if(float(back) > 0.5f * float(n)) st = 0;", "if(float(back) > 0.25f * float(n)) st = 0;"),
+    ("affecting geometry within two cells", "inside = inside && (dist < cell[k]);", "inside = inside && (dist < 2.0f * cell[k]);"),
+    ("selection ignores the loop index", "((idx + *loop_index) % states[idx]) == 0", "(idx % states[idx]) == 0"),
+    # atmospheric sky (src/shaders/sky.glsl:60-127)
+    ("Mie asymmetry -0.98", "kSkyG = double(-0.990f)", "kSkyG = double(-0.980f)"),
+    ("Rayleigh constant 0.0030", "kSkyKr = double(0.0025f)", "kSkyKr = double(0.0030f)"),
 ]
 
 
 def main(only=()):
     tests = sorted(f for f in os.listdir(os.path.join(REPO, "tests")) if f.startswith("test_oracle_"))
     survived = []
-    for name, old, new in MUTATIONS:
+    for name, old, new, *where in MUTATIONS:
         if only and name not in only:
             continue
         with tempfile.TemporaryDirectory() as d:
@@ -59,7 +77,7 @@ def main(only=()):
                 shutil.copytree(os.path.join(REPO, sub), os.path.join(d, sub),
                                 ignore=shutil.ignore_patterns("_build", "_ref", "__pycache__", "*.so"))
             shutil.copy(os.path.join(REPO, "pytest.ini"), d)
-            path = os.path.join(d, "oracle", "ddgi_oracle.cpp")
+            path = os.path.join(d, "oracle", where[0] if where else "ddgi_oracle.cpp")
             src = open(path).read()
             assert src.count(old) >= 1, f"{name}: the original text is missing"
             open(path, "w").write(src.replace(old, new, 1))  # the first occurrence: the forward code

@@ -104,6 +104,23 @@ def test_edge_fallback_and_ties_bit_exact():
     assert got[0, 1] == 0xFFFFFFFF and (got[1, 1], got[1, 2]) == (0, 0) and (got[2, 1], got[2, 2]) == (1, 0)
 
 
+@pytest.mark.parametrize("height,front_up", [(2.0 - 0.5 * np.sqrt(3.0), False), (-0.4, True)])
+def test_init_probes_threshold_scenes(height, front_up):
+    """Probe init on the two scenes that sit between its thresholds (a back face seen by 3/8 of
+    the rays; geometry within reach but beyond one cell): states and back-face counts identical."""
+    from test_oracle_probes import _floor_quad_scene
+    scene = _floor_quad_scene(height, 8.0, front_up)
+    grid = scenes.grid_params((0, 0, 0), (6, 6, 6), (4, 4, 4), 512)
+    R = scenes.orientations(1, 1)[0]
+    p = _make(scene, grid)
+    p.init_probes(R)
+    got = p.states()
+    p.destroy()
+    want, _ = orc.OracleScene(scene).init_probes(orc.Grid(grid), R)
+    assert np.array_equal(got, want), f"{np.sum(got != want)} probe states differ"
+    assert got[1 + 4 + 16] == (1 if not front_up else 8)
+
+
 def test_init_probes_states_identical(small):
     scene, grid, osc = small
     og = orc.Grid(grid)

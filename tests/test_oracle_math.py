@@ -144,6 +144,27 @@ def test_pbr_closed_forms():
     assert np.all(got > 0) and np.all(got < 0.01)
 
 
+def test_pbr_fresnel_off_axis():
+    """Light and view 60 degrees either side of the normal: h = n, so NdotH = 1, VdotH = NdotL =
+    NdotV = 1/2, the Schlick factor (1 - 1/2)^5 = 1/32, and with roughness 1 (alpha = 1) the
+    Smith terms 2 (1/2) / (1/2 + 1) = 2/3 each and D = 1 / pi.  A dark metal (albedo 0.01, so the
+    90-degree reflectance is clamp(25 * 0.01) = 0.25) and a dielectric (f0 = 0.04, reflectance at
+    90 degrees 1) give the values written out below."""
+    n = np.array([0.0, 1.0, 0.0])
+    s3 = np.sqrt(3.0) / 2
+    light, view = np.array([s3, 0.5, 0.0]), np.array([-s3, 0.5, 0.0])
+    lc = np.array([2.0, 2.0, 2.0])
+    spec_scale = (4.0 / 9.0) * (1.0 / REF_PI) / (4.0 * 0.5 * 0.5)      # G D / (4 NdotL NdotV)
+    F = 0.01 + (0.25 - 0.01) / 32.0
+    got = orc.pbr(n, view, lc, light, np.full(3, 0.01), 1.0, 1.0)
+    assert np.allclose(got, lc * 0.5 * F * spec_scale, rtol=1e-12)
+    F = 0.04 + (1.0 - 0.04) / 32.0
+    albedo = np.array([0.5, 0.25, 0.125])
+    got = orc.pbr(n, view, lc, light, albedo, 0.0, 1.0)
+    want = lc * 0.5 * ((1.0 - F) * albedo * 0.96 / REF_PI + F * spec_scale)
+    assert np.allclose(got, want, rtol=1e-12)
+
+
 def test_selection_reference_semantics():
     # src/IrradianceProbes.cpp:396-424 by hand: pick idx if state != 0 and (idx + loop) % state == 0
     states = np.array([1, 2, 0, 3, 8, 1], np.uint32)

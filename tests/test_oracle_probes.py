@@ -187,6 +187,38 @@ def test_init_probe_states():
     assert states[lin(2, 1, 1)] == 1                                  # 1 m from the box face
 
 
+def _floor_quad_scene(y, half, front_up):
+    """One square of half-size `half` around (2, y, 2), facing up (+y) or down."""
+    sb = scenes.SceneBuilder()
+    m = sb.add_material((0.5, 0.5, 0.5))
+    pos = np.array([[2 - half, y, 2 - half], [2 + half, y, 2 - half], [2 + half, y, 2 + half], [2 - half, y, 2 + half]],
+                   np.float32)
+    up = 1.0 if front_up else -1.0
+    nrm = np.tile(np.array([0, up, 0], np.float32), (4, 1))
+    # front face <=> dot(d, cross(v1 - v0, v2 - v0)) < 0: cross points up for (0, 2, 1), down for (0, 1, 2)
+    tris = np.array([[0, 2, 1], [0, 3, 2]]) if front_up else np.array([[0, 1, 2], [0, 2, 3]])
+    sb.add_instance(sb.add_mesh(pos, nrm, tris, m))
+    return sb.build((-10, y - 1, -10), (10, 10, 10))
+
+
+def test_init_probe_back_face_share_and_reach():
+    """The two thresholds of probesInit.rgen:31-64 away from the easy cases, for the probe at
+    (2, 2, 2) of a grid of 2 m cells (reach = the cell diagonal 2 sqrt 3):
+      * the back of a large square 0.25 reach below it: 3/8 of the 512 init rays hit it within reach
+        (the cap 1 - d / reach over the sphere, halved), above n / 4 but below n / 2 = 256 -> the
+        probe is not inside geometry, and hits within one cell in every axis make it state 1;
+      * the front of a square 1.2 cells below it: hits within reach but none within one cell along
+        y -> nothing affects the probe, state 8."""
+    g = _grid(res=(4, 4, 4), n=512, lo=(0, 0, 0), hi=(6, 6, 6))
+    reach = 2.0 * np.sqrt(3.0)
+    lin = 1 + 4 * 1 + 16 * 1
+    R = scenes.orientations(1, 1)[0]
+    states, back = orc.OracleScene(_floor_quad_scene(2.0 - 0.25 * reach, 8.0, front_up=False)).init_probes(g, R)
+    assert states[lin] == 1 and 128 < back[lin] < 256, (states[lin], back[lin])
+    states, back = orc.OracleScene(_floor_quad_scene(2.0 - 2.4, 8.0, front_up=True)).init_probes(g, R)
+    assert states[lin] == 8 and back[lin] == 0
+
+
 def _empty_scene():
     # a single small triangle far outside the grid: every probe ray misses
     sb = scenes.SceneBuilder()
