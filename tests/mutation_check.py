@@ -63,6 +63,13 @@ MUTATIONS = [
     # atmospheric sky (src/shaders/sky.glsl:60-127)
     ("Mie asymmetry -0.98", "kSkyG = double(-0.990f)", "kSkyG = double(-0.980f)"),
     ("Rayleigh constant 0.0030", "kSkyKr = double(0.0025f)", "kSkyKr = double(0.0030f)"),
+    # final gather (src/shaders/FinalGather.frag:38-77) and the ray-traced shadow pass
+    # (src/shaders/directLight.rgen), in py.py and ddgi_oracle.cpp
+    ("gather f0 0.04 instead of 0.004", "    f0 = 0.004\n", "    f0 = 0.04\n", "py.py"),
+    ("gather without the emissive term", "    color += g[:, 12:15].astype(np.float64)\n", "\n", "py.py"),
+    ("gather view direction reversed", "view = np.asarray(origin, np.float64)[None, :] - pos",
+     "view = pos - np.asarray(origin, np.float64)[None, :]", "py.py"),
+    ("shadow cone 0.04 rad", "2.0f * (nx - 0.5f) * 0.02f", "2.0f * (nx - 0.5f) * 0.04f"),
 ]
 
 
