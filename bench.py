@@ -366,8 +366,15 @@ def diff_bench(lib):
     fwd = _event_ms(lambda: dctx.forward(A, S, target))
     bwd = _event_ms(lambda: dctx.backward(A, S, target, one))
     p.destroy()
+    # algorithmic bytes per texel: forward 640 B of probe taps (8 cage probes x 4 irradiance and 4
+    # depth texels, both atlases) + 32 B of probe states + 28 B of G-buffer + 12 B out = 712 B;
+    # backward the same reads plus a 12-byte gradient per irradiance tap (8 x 4 x 12 = 384 B)
+    fwd_b, bwd_b = 712 * int(dctx.N), (712 + 384) * int(dctx.N)
     return {"gbuffer_texels": int(dctx.N), "params": 256 * 3 + 3, "forward_ms": round(fwd, 4),
-            "backward_ms": round(bwd, 4), "bwd_over_fwd": round(bwd / fwd, 3)}
+            "backward_ms": round(bwd, 4), "bwd_over_fwd": round(bwd / fwd, 3),
+            "forward_algorithmic_GB/s": round(fwd_b / (fwd * 1e-3) / 1e9, 1),
+            "backward_algorithmic_GB/s": round(bwd_b / (bwd * 1e-3) / 1e9, 1),
+            "bytes_per_texel": {"forward": 712, "backward": 712 + 384}}
 
 
 def batch_bench(lib, frames=5):
