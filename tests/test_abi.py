@@ -129,3 +129,21 @@ def test_scene_generators():
     light = scenes.sun_light()
     d = light["direction"][:3]
     assert np.allclose(d, -np.array([0.3, -1, 0.2]) / np.linalg.norm([0.3, -1, 0.2]), atol=1e-7)
+
+
+def test_cpp_mirror_header_compiles():
+    """include/ddgi_probes.hpp (the C++ mirror of IrradianceProbes over the C-ABI) compiles as a
+    plain C++17 translation unit, with no HIP or torch headers."""
+    import os
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "use.cpp")
+        with open(src, "w") as fh:
+            fh.write('#include "ddgi_probes.hpp"\n'
+                     "int main() { ddgi::IrradianceProbes p; p.seed(7); auto R = p.randomOrientation();\n"
+                     "  return p.getProbeCount() == 32u * 16u * 32u && R[8] * R[8] <= 1.0f ? 0 : 1; }\n")
+        r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                            "-I" + os.path.join(root, "include"), src], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
