@@ -354,7 +354,6 @@ done:
 
 // ---------------------------------------------------------------------------------------------
 // Octahedral maps
-__device__ __forceinline__ float sign_not_zero(float v) { return v >= 0.0f ? 1.0f : -1.0f; }
 __device__ __forceinline__ float gsign(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 
 // spherePointToOctohedralUV (src/shaders/irradiance.glsl:119-138), GLSL sign() semantics.
