@@ -630,6 +630,7 @@ def main():
                    "probes": P, "rays_per_probe": n, "triangles": scene.triangle_count,
                    "bvh": probes.bvh_info(), "bvh_build_s": round(bvh_s, 3)},
         "phases_ms": {k: round(v, 4) for k, v in avg.items()},
+        "phases_ms_median": {k: round(float(np.median(v)), 4) for k, v in phase.items()},
         "kernels": kernels, "trace_stats": trace_stats,
         "roofline": roofline, "cpu_baseline": cpu,
         "full_frame": frame, "blend_precision": blend, "blend_config3": config3, "differentiable": diff, "batched_volumes": batched,
