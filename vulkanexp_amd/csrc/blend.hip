@@ -137,79 +137,104 @@ __device__ __forceinline__ float wave_max(float v) {
 // value and stores into the next atlas; a wavefront writes one tile row-block at a time, so the
 // stores are contiguous.  Irradiance: one wavefront per 8x8 tile (lane = texel), maxChange per
 // probe over the interior texels by a wave reduction.  Depth: a wavefront per 16x16 tile, four
-// texels per lane.  The exposed-atlas loads of kTilesPerPass tiles per wavefront are issued before
-// their stores.
+// texels per lane.  A pass takes kTilesPerPass tiles per wavefront (tiles p0 + waves u); its
+// exposed-atlas loads, irradiance and depth together, depend on the probe indices only, so they
+// are issued before the stores (epilogue_load), and a kernel may issue them before its sums are
+// ready.
+
+// The lane's texels: the interior texel its irradiance texel and its four depth texels hold after
+// the border copy, and those texels' weight normalisers.
+struct EpiLane {
+	uint32_t irr_src, irr_stex, dep_src[4], dep_stex[4];
+	float irr_inv, dep_inv[4];
+	bool interior;
+};
+
+__device__ __forceinline__ EpiLane epilogue_lane(const BlendArgs& a) {
+	const uint32_t lane = threadIdx.x & 63u;
+	EpiLane L;
+	int sx, sy;
+	border_source<8>(int(lane & 7u), int(lane >> 3), sx, sy);
+	L.interior = sx == int(lane & 7u) && sy == int(lane >> 3);
+	L.irr_stex = uint32_t(sy - 1) * 6u + uint32_t(sx - 1);
+	L.irr_src = uint32_t(sy) * 8u + uint32_t(sx);
+	L.irr_inv = a.inv_sum[L.irr_stex];
+#pragma unroll
+	for(uint32_t i = 0; i < 4; ++i) {
+		border_source<16>(int(lane & 15u), int(lane >> 4) + 4 * int(i), sx, sy);
+		L.dep_stex[i] = uint32_t(sy - 1) * 14u + uint32_t(sx - 1);
+		L.dep_src[i] = uint32_t(sy) * 16u + uint32_t(sx);
+		L.dep_inv[i] = a.inv_sum[kIrrTexels + L.dep_stex[i]];
+	}
+	return L;
+}
+
+// exposed-atlas values of one pass
+template <uint32_t kTilesPerPass>
+struct EpiPrev {
+	float4 irr[kTilesPerPass];
+	float2 dep[kTilesPerPass][4];
+};
+
+template <uint32_t kTilesPerPass>
+__device__ __forceinline__ void epilogue_load(const BlendArgs& a, const EpiLane& L, const uint32_t* s_idx, uint32_t np,
+											  uint32_t p0, bool irr_only, EpiPrev<kTilesPerPass>& e) {
+	const uint32_t waves = blockDim.x >> 6;
+#pragma unroll
+	for(uint32_t u = 0; u < kTilesPerPass; ++u) {
+		const uint32_t p = p0 + waves * u;
+		if(p < np) {
+			const size_t probe = s_idx[p];
+			e.irr[u] = a.irr_prev[probe * 64 + L.irr_src];
+			if(!irr_only) {
+#pragma unroll
+				for(uint32_t i = 0; i < 4; ++i) e.dep[u][i] = a.dep_prev[probe * 256 + L.dep_src[i]];
+			}
+		}
+	}
+}
+
+template <uint32_t kTilesPerPass>
+__device__ __forceinline__ void epilogue_store(const BlendArgs& a, const EpiLane& L, const uint32_t* s_idx, uint32_t np,
+											   uint32_t p0, const float* s_irr, const float* s_dep, uint32_t* s_maxc,
+											   bool irr_only, const EpiPrev<kTilesPerPass>& e) {
+	const float h = a.grid.hysteresis;
+	float4* __restrict__ irr_out = a.irr_out;
+	float2* __restrict__ dep_out = a.dep_out;
+	const uint32_t lane = threadIdx.x & 63u, waves = blockDim.x >> 6;
+#pragma unroll
+	for(uint32_t u = 0; u < kTilesPerPass; ++u) {
+		const uint32_t p = p0 + waves * u;
+		if(p >= np) break;
+		const size_t probe = s_idx[p];
+		const float4 sum = *reinterpret_cast<const float4*>(s_irr + p * kIrrStride + L.irr_stex * 4);
+		const float r0 = sum.x * L.irr_inv, r1 = sum.y * L.irr_inv, r2 = sum.z * L.irr_inv;
+		const float4 pv = e.irr[u];
+		const float mc = L.interior ? fmaxf(fmaxf(fabsf(r0 - pv.x), fabsf(r1 - pv.y)), fabsf(r2 - pv.z)) : 0.0f;
+		irr_out[probe * 64 + lane] = make_float4(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h,
+												 r2 * (1.0f - h) + pv.z * h, 1.0f);
+		const float m = wave_max(mc);
+		if(lane == 0) s_maxc[p] = __float_as_uint(m);
+		if(irr_only) continue;
+#pragma unroll
+		for(uint32_t i = 0; i < 4; ++i) {
+			const float2 dsum = *reinterpret_cast<const float2*>(s_dep + p * kDepStride + L.dep_stex[i] * 2);
+			const float d0 = dsum.x * L.dep_inv[i], d1 = dsum.y * L.dep_inv[i];
+			const float2 dv = e.dep[u][i];
+			dep_out[probe * 256 + lane + 64 * i] = make_float2(d0 * (1.0f - h) + dv.x * h, d1 * (1.0f - h) + dv.y * h);
+		}
+	}
+}
+
 template <uint32_t kTilesPerPass = 4>
 __device__ __forceinline__ void blend_epilogue(const BlendArgs& a, const uint32_t* s_idx, uint32_t np,
 											   const float* s_irr, const float* s_dep, uint32_t* s_maxc, bool irr_only) {
-	const float h = a.grid.hysteresis;
-	const float4* __restrict__ irr_prev = a.irr_prev;
-	const float2* __restrict__ dep_prev = a.dep_prev;
-	float4* __restrict__ irr_out = a.irr_out;
-	float2* __restrict__ dep_out = a.dep_out;
-	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-	{
-		int sx, sy;
-		border_source<8>(int(lane & 7u), int(lane >> 3), sx, sy);
-		const bool interior = sx == int(lane & 7u) && sy == int(lane >> 3);
-		const uint32_t stex = uint32_t(sy - 1) * 6u + uint32_t(sx - 1);
-		const uint32_t src = uint32_t(sy) * 8u + uint32_t(sx);
-		const float inv = a.inv_sum[stex];
-		for(uint32_t p0 = wave; p0 < np; p0 += waves * kTilesPerPass) {
-			float4 prev[kTilesPerPass];
-#pragma unroll
-			for(uint32_t u = 0; u < kTilesPerPass; ++u) {
-				const uint32_t p = p0 + waves * u;
-				if(p < np) prev[u] = irr_prev[size_t(s_idx[p]) * 64 + src];
-			}
-#pragma unroll
-			for(uint32_t u = 0; u < kTilesPerPass; ++u) {
-				const uint32_t p = p0 + waves * u;
-				if(p >= np) break;
-				const float4 sum = *reinterpret_cast<const float4*>(s_irr + p * kIrrStride + stex * 4);
-				const float r0 = sum.x * inv, r1 = sum.y * inv, r2 = sum.z * inv;
-				const float4 pv = prev[u];
-				const float mc = interior ? fmaxf(fmaxf(fabsf(r0 - pv.x), fabsf(r1 - pv.y)), fabsf(r2 - pv.z)) : 0.0f;
-				irr_out[size_t(s_idx[p]) * 64 + lane] = make_float4(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h,
-																	  r2 * (1.0f - h) + pv.z * h, 1.0f);
-				const float m = wave_max(mc);
-				if(lane == 0) s_maxc[p] = __float_as_uint(m);
-			}
-		}
-	}
-	if(irr_only) return;
-	uint32_t src[4], stex[4];
-	float inv[4];
-#pragma unroll
-	for(uint32_t i = 0; i < 4; ++i) {
-		int sx, sy;
-		border_source<16>(int(lane & 15u), int(lane >> 4) + 4 * int(i), sx, sy);
-		stex[i] = uint32_t(sy - 1) * 14u + uint32_t(sx - 1);
-		src[i] = uint32_t(sy) * 16u + uint32_t(sx);
-		inv[i] = a.inv_sum[kIrrTexels + stex[i]];
-	}
+	const EpiLane L = epilogue_lane(a);
+	const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
 	for(uint32_t p0 = wave; p0 < np; p0 += waves * kTilesPerPass) {
-		float2 prev[kTilesPerPass][4];
-#pragma unroll
-		for(uint32_t u = 0; u < kTilesPerPass; ++u) {
-			const uint32_t p = p0 + waves * u;
-			if(p < np) {
-#pragma unroll
-				for(uint32_t i = 0; i < 4; ++i) prev[u][i] = dep_prev[size_t(s_idx[p]) * 256 + src[i]];
-			}
-		}
-#pragma unroll
-		for(uint32_t u = 0; u < kTilesPerPass; ++u) {
-			const uint32_t p = p0 + waves * u;
-			if(p >= np) break;
-#pragma unroll
-			for(uint32_t i = 0; i < 4; ++i) {
-				const float2 sum = *reinterpret_cast<const float2*>(s_dep + p * kDepStride + stex[i] * 2);
-				const float r0 = sum.x * inv[i], r1 = sum.y * inv[i];
-				const float2 pv = prev[u][i];
-				dep_out[size_t(s_idx[p]) * 256 + lane + 64 * i] = make_float2(r0 * (1.0f - h) + pv.x * h, r1 * (1.0f - h) + pv.y * h);
-			}
-		}
+		EpiPrev<kTilesPerPass> e;
+		epilogue_load(a, L, s_idx, np, p0, irr_only, e);
+		epilogue_store(a, L, s_idx, np, p0, s_irr, s_dep, s_maxc, irr_only, e);
 	}
 }
 
@@ -480,7 +505,6 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 	const int t = threadIdx.x, lane = t & 63, wave = t >> 6, q = wave & 3;
 	const int lrow = lane & 15, lgrp = lane >> 4;
 	const uint32_t my_slot = uint32_t(16 * q + lrow);  // this lane's A row: a probe slot of the workgroup
-	const bool my_valid = my_slot < np;
 	// the workgroup's rays as a buffer resource (np probes x n rays): loads past its end return
 	// zero, so no address needs a clamp, and the lane's offset stays in one register for the whole
 	// K loop (per-load clamped addresses were reused as MFMA results right after the loads, which
@@ -503,16 +527,21 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 	};
 	// ray data of K step s: lane (probe l % 16, K block g = l / 16) takes rays 32 s + 4 j + g,
 	// j = 0..7, so that load j of the wave reads 4 consecutive rays (64 bytes) of each of its 16
-	// probes (the weights are stored in the matching wide_column order).  The loads are
-	// unconditional, so that they issue back to back without branches: rays beyond n read the next
-	// probe's (or nothing, past the workgroup's last probe), and rays that do not exist (beyond n,
-	// absent probes) are replaced by zero where they are used.
+	// probes (the weights are stored in the matching wide_column order).  Rays that do not exist
+	// load as zero, which contributes nothing to any column: absent probes lie past the buffer's
+	// records, and in the last, partial K step of a ray count that is not a multiple of 32 the
+	// rays beyond n get an offset past the records as well.  The whole offset is in the range-
+	// checked vector operand (the scalar offset is not checked against the records).
 	auto load_rays = [&](uint32_t s, float4* r) {
+		typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+		const uint32_t base = my_off + s * (kKStep * 16u);
+		// rays of step s that exist (uniform): all 32 before the last two steps (the steps cover n
+		// rounded up to 64), so that the selects below fold away there
+		const uint32_t lim = s + 2 < steps ? uint32_t(kKStep) : (n > s * kKStep ? min(n - s * kKStep, uint32_t(kKStep)) : 0u);
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
-			typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-			const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rays_rsrc, my_off + uint32_t(64 * j),
-																   int(s * kKStep * 16u), 0);
+			const uint32_t off = uint32_t(4 * j + lgrp) < lim ? base + uint32_t(64 * j) : 0x80000000u;
+			const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rays_rsrc, off, 0, 0);
 			r[j] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
 		}
 	};
@@ -547,9 +576,7 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 		float v[8][5];
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
-			// rays that do not exist are zero: in range, and nothing in any column
-			const bool valid = my_valid && s * kKStep + uint32_t(4 * j + lgrp) < n;
-			const float4 x = valid ? r[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			const float4 x = r[j];  // zero where the ray does not exist: in range, nothing in any column
 			if(H == 0 && (x.w < 0.0f || x.w > cell_len)) ++my_oor;
 			float dc = fminf(cell_len, x.w);
 			if(dc < 0.0f) dc = cell_len;
@@ -559,6 +586,9 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 			v[j][3] = dc;
 			v[j][4] = dc * dc;
 		}
+		// the count is taken in its step: otherwise the compiler sinks the comparisons to the end
+		// of the K loop and keeps every step's distances live until then
+		if constexpr(H == 0) __asm__ volatile("" : "+v"(my_oor));
 #pragma unroll
 		for(int c = kCh0; c < 5; ++c) {
 			if constexpr(P == 2) {
@@ -617,6 +647,17 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 		}
 	}
 
+	// the epilogue's exposed-atlas loads for both halves of the probes, issued now: they need only
+	// the probe indices, and their latency then passes while the sums go through LDS (one pass
+	// per half: 32 probes over eight waves, four tiles each)
+	static_assert(kWideHalf == (kWideThreads / 64) * 4, "one epilogue pass per half");
+	const EpiLane el = epilogue_lane(a);
+	EpiPrev<4> pre[2];
+#pragma unroll
+	for(int half = 0; half < 2; ++half) {
+		const uint32_t first = uint32_t(half * kWideHalf);
+		if(first < np) epilogue_load(a, el, lds.s_idx + first, min(uint32_t(kWideHalf), np - first), uint32_t(wave), false, pre[half]);
+	}
 	// out-of-range counts: the four lanes of a probe row (l % 16 equal)
 	if(H == 0) {
 		my_oor += __shfl_xor(my_oor, 16);
@@ -653,8 +694,8 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 			}
 		}
 		__syncthreads();
-		blend_epilogue<4>(a, lds.s_idx + first, min(uint32_t(kWideHalf), np - first), s_irr, s_dep, lds.s_maxc + first,
-						  false);
+		epilogue_store(a, el, lds.s_idx + first, min(uint32_t(kWideHalf), np - first), uint32_t(wave), s_irr, s_dep,
+					   lds.s_maxc + first, false, pre[half]);
 	}
 }
 
