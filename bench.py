@@ -154,17 +154,23 @@ def _event_ms(fn, reps=10):
 
 
 def blend_compare(probes, R, lib):
-    """FP32 / BF16 / FP16 blends of one ray buffer (16,384 probes x 256 rays): the blend phase of
-    the update (weights, blend) in each mode, and the error of BF16/FP16 against FP32.  Each mode
-    runs the same update (same state, same orientation, so the same rays) 10 times."""
+    """FP32 / BF16 / FP16 blends of one ray buffer (16,384 probes x 256 rays), 10 times each: the
+    blend phase (frame directions, weights, blend) and the error of BF16/FP16 against FP32.  The
+    probe list covers the grid (UPDATE_ALL_PROBES), so no atlas is copied forward."""
+    import torch
     state = probes.save_state()
+    probes.update(R, lib.UPDATE_ALL_PROBES)
+    probes.synchronize()
+    rays, idx = probes.last_update_rays()
+    t_rays = torch.from_numpy(rays).cuda()
+    t_idx = torch.from_numpy(idx.view(np.int32).copy()).cuda()
     out = {}
     ref = None
     for mode, flag in (("fp32", 0), ("bf16", lib.UPDATE_BLEND_BF16), ("fp16", lib.UPDATE_BLEND_FP16)):
         ms = []
         for _ in range(10):
             probes.load_state(state)
-            probes.update(R, lib.UPDATE_ALL_PROBES | flag)
+            probes.blend_rays(t_rays, t_idx, R, lib.UPDATE_ALL_PROBES | flag)
             ms.append(probes.timings()["blend_ms"])
         probes.synchronize()
         irr = probes.irradiance(lib.LAYOUT_TILED)[..., :3].astype(np.float64)
@@ -182,9 +188,11 @@ def blend_compare(probes, R, lib):
 def blend_config3(lib):
     """The mixed-precision blend workload: the closed room + 64 boxes (780 triangles), 8x4x8 probes
     x 512 rays, cosine weights [36 x 512] for irradiance and cos^50 weights [196 x 512] for depth.
-    The same update (same state and orientation, so the same ray buffer) runs in each mode with
-    hysteresis 0 (the new-frame estimate), its blend phase timed over 10 runs; the atlases are kept
-    for the error check against the FP64 blend of that ray buffer (done in the CPU leg)."""
+    One ray buffer from the update is blended by each mode with hysteresis 0 (the new-frame
+    estimate), its blend phase (frame directions, weights, blend; the probe list covers the grid)
+    timed over 10 blends; the atlases are kept for the error check against the FP64 blend (done in
+    the CPU leg)."""
+    import torch
     from vulkanexp_amd import IrradianceProbes, scenes
     scene, grid = scenes.config_small(seed=1, rays=512)
     grid = np.array(grid)
@@ -205,12 +213,14 @@ def blend_config3(lib):
     p.load_state(state)
     check = {"grid": np.array(p.GridParameters), "R": Rs[3], "rays": rays, "idx": idx, "states": p.states(),
              "irr": p.irradiance()[..., :3].astype(np.float64), "dep": p.depth().astype(np.float64), "out": {}}
+    t_rays = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    t_idx = torch.from_numpy(idx.view(np.int32).copy()).cuda()
     out = {"workload": "closed room + 64 boxes (780 tris), 8x4x8 probes x 512 rays, depthSharpness 50, hysteresis 0"}
     for mode, flag in (("fp32", 0), ("bf16", lib.UPDATE_BLEND_BF16), ("fp16", lib.UPDATE_BLEND_FP16)):
         ms = []
         for _ in range(10):
             p.load_state(state)
-            p.update(Rs[3], lib.UPDATE_ALL_PROBES | flag)
+            p.blend_rays(t_rays, t_idx, Rs[3], lib.UPDATE_ALL_PROBES | flag)
             ms.append(p.timings()["blend_ms"])
         p.synchronize()
         check["out"][mode] = (p.irradiance()[..., :3].astype(np.float64), p.depth().astype(np.float64))

@@ -310,7 +310,10 @@ ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags
 
 /* Blend only: feed an externally produced ray buffer `rays` (device float4[count*n], slot-major
  * (rgb, depth), n = rays_per_probe) for probes `probe_indices` (device uint32[count]) and the
- * frame orientation, then run blend + border + state update + publish like ddgi_update does. */
+ * frame orientation, then run blend + border + state update + publish like ddgi_update does.
+ * Probes not listed keep their tiles: the next atlas starts as a copy of the exposed one.  With
+ * DDGI_UPDATE_ALL_PROBES in `flags` and count = the probe count, the caller states that
+ * probe_indices lists every probe once, and that copy is skipped. */
 ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_indices, uint32_t count,
 							const float orientation[9], uint32_t flags, void* stream);
 

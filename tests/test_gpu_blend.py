@@ -93,3 +93,24 @@ def test_bf16_and_fp32_select_same_states(ray_buffer):
         p.synchronize()
         out.append(p.states())
     assert np.mean(out[0] == out[1]) >= 0.99
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16"])
+def test_full_probe_list_skips_the_copy_forward(ray_buffer, mode):
+    """ddgi_blend_rays with UPDATE_ALL_PROBES and every probe listed skips copying the exposed
+    atlas forward: atlases and states identical to the call without the flag.  With a partial list
+    the flag changes nothing (the copy still runs: unlisted probes keep their tiles)."""
+    import torch
+    p, grid, rays, idx, R, state = ray_buffer
+    half = len(idx) // 2
+    for count in (len(idx), half):
+        t_rays = torch.from_numpy(np.ascontiguousarray(rays[:count])).cuda()
+        t_idx = torch.from_numpy(idx[:count].view(np.int32).copy()).cuda()
+        out = []
+        for extra in (0, _lib.UPDATE_ALL_PROBES):
+            p.load_state(state)
+            p.blend_rays(t_rays, t_idx, R, FLAGS[mode] | extra)
+            p.synchronize()
+            out.append((p.irradiance(_lib.LAYOUT_TILED), p.depth(_lib.LAYOUT_TILED), p.states()))
+        for a, b in zip(*out):
+            assert np.array_equal(a, b), f"{mode}, {count} probes listed"

@@ -902,8 +902,10 @@ ddgi_status ddgi_blend_rays(void* handle, const void* rays, const void* probe_in
 	HIP_TRY(h, hipMemcpyAsync(h->indices, probe_indices, size_t(count) * 4, hipMemcpyDeviceToDevice, s));
 	HIP_TRY(h, launch_set_count(h->sel, count, s));
 	h->indices_are_iota = false;
-	// the listed probes may not cover the grid: the next atlas starts as a copy of the exposed one
-	if((st = enqueue_blend_and_publish(h, &R, static_cast<const float4*>(rays), nullptr, count, false, flags, s)) != DDGI_OK)
+	// the listed probes may not cover the grid: the next atlas starts as a copy of the exposed one,
+	// unless the caller states that they are every probe
+	const bool all = (flags & DDGI_UPDATE_ALL_PROBES) && count == h->P;
+	if((st = enqueue_blend_and_publish(h, &R, static_cast<const float4*>(rays), nullptr, count, all, flags, s)) != DDGI_OK)
 		return st;
 	if(timing) {
 		HIP_TRY(h, hipEventRecord(h->ev[4], s));
