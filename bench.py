@@ -124,7 +124,7 @@ def cpu_baseline(scene, grid, light, sky, R, n_probes, max_threads):
 
 PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "ddgi_update_large_rocprof.md")
 PROFILE_KERNELS = {"trace_primary": "k_trace_primary<16, false>", "shade_shadow": "k_shade<16, false>",
-                   "blend": "k_blend_fp32"}
+                   "blend": "k_blend_wide<0u, 8>"}
 
 
 def profiled_traffic(kernel: str):

@@ -114,3 +114,41 @@ def test_full_probe_list_skips_the_copy_forward(ray_buffer, mode):
             out.append((p.irradiance(_lib.LAYOUT_TILED), p.depth(_lib.LAYOUT_TILED), p.states()))
         for a, b in zip(*out):
             assert np.array_equal(a, b), f"{mode}, {count} probes listed"
+
+
+@pytest.mark.parametrize("rays", [33, 64, 256, 512])
+def test_fp32_workgroup_shapes_give_the_same_sums(rays):
+    """The FP32 blend runs in 64-probe workgroups (k_blend_wide) once they fill the device and in
+    16-probe blocks (k_blend_fp32) below that; both issue the same MFMAs per output element in the
+    same order, so their atlases and states are bit-identical.  A grid of 64 probes per compute unit
+    is blended whole (64-probe workgroups) and as its first half (16-probe blocks); the first half's
+    tiles and states must agree exactly (ray counts with a partial K step, 2, 8 and 16 K steps)."""
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    scene, _ = scenes.config_small(seed=3, rays=rays, open_top=True)
+    lo, hi = scenes.inset_extent(scene)
+    grid = np.array(scenes.grid_params(lo, hi, (32, 2 * cus // 32, 32), rays))
+    p = IrradianceProbes(grid, max_rays_per_probe=rays)
+    p.set_scene(scene)
+    p.set_light(scenes.sun_light())
+    p.set_sky(scenes.SKY_RADIANCE)
+    P = p.get_probe_count()
+    assert P >= 64 * cus
+    for R in scenes.orientations(70 + rays, 2):
+        p.update(R, _lib.UPDATE_ALL_PROBES)
+    R = scenes.orientations(80 + rays, 1)[0]
+    state = p.save_state()
+    p.update(R, _lib.UPDATE_ALL_PROBES)
+    p.synchronize()
+    ray_buf, idx = p.last_update_rays()
+    out = []
+    for count, flags in ((P // 2, 0), (P, _lib.UPDATE_ALL_PROBES)):
+        p.load_state(state)
+        p.blend_rays(torch.from_numpy(np.ascontiguousarray(ray_buf[:count])).cuda(),
+                     torch.from_numpy(idx[:count].view(np.int32).copy()).cuda(), R, flags)
+        p.synchronize()
+        sel = idx[:P // 2].astype(np.int64)
+        out.append((p.irradiance(_lib.LAYOUT_TILED)[sel], p.depth(_lib.LAYOUT_TILED)[sel], p.states()[sel]))
+    p.destroy()
+    for a, b, what in zip(*out, ("irradiance", "depth", "states")):
+        assert np.array_equal(a, b), f"{rays} rays: {what} differs"

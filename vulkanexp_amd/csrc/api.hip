@@ -295,7 +295,8 @@ ddgi_status enqueue_blend_and_publish(Handle* h, const Rot* R, const float4* ray
 	b.irr_out = h->irr[nxt];
 	b.dep_out = h->dep[nxt];
 	b.max_slots = max_slots;
-	HIP_TRY(h, launch_blend(b, mode, s));
+	// the FP32 blend in 64-probe workgroups once they fill the device, else in 16-probe blocks
+	HIP_TRY(h, launch_blend(b, mode, s, h->cus > 0 && max_slots >= 64u * h->cus));
 	h->cur = nxt;  // publish (reference: vkCmdCopyImage work -> exposed, src/IrradianceProbes.cpp:530-576)
 	return DDGI_OK;
 }
@@ -452,6 +453,7 @@ ddgi_status ddgi_create(const ddgi_grid_params* params, const ddgi_options* opti
 		delete h;
 		return fail(nullptr, DDGI_ERROR_NO_DEVICE, "device is not gfx950");
 	}
+	h->cus = uint32_t(prop.multiProcessorCount);
 	auto cleanup_fail = [&](ddgi_status s) {
 		std::string m = h->err;
 		destroy_handle(h);

@@ -423,9 +423,20 @@ struct MfmaTypes<kBlendFP16> {
 	static constexpr int kParts = 1;
 	__device__ static floatx4_t mma(V a, V b, floatx4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 };
+// FP32 on the wide structure: v_mfma_f32_16x16x4_f32, 8 per accumulator and 32-ray K step; the
+// step's weights are staged as two 16-ray halves (the LDS geometry of BF16's two parts)
+template <>
+struct MfmaTypes<kBlendFP32> {
+	typedef float T;
+	typedef float V;
+	static constexpr int kParts = 2;
+	__device__ static floatx4_t mma(float a, float b, floatx4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
 
 // ---------------------------------------------------------------------------------------------
-// MFMA blend, transposed and pipelined (BF16 / FP16 modes).  Per channel the blend of a group of 16
+// MFMA blend, transposed and pipelined (BF16 / FP16 modes, and FP32 on grids whose 64-probe
+// workgroups fill the device: there v_mfma_f32_16x16x4_f32 runs 8 per accumulator and K step on
+// the unconverted ray data, pairing rays as blend_fp32_tile does).  Per channel the blend of a group of 16
 // probes is C^T [16 probes x texels] = X^T [16 probes x n rays] W^T [n x texels]: the ray data is
 // the A operand, read by each lane straight from HBM (row = one of the group's probes, 8 rays per
 // lane and K step), and the weights are the B operand, shared by the workgroup's eight waves
@@ -511,23 +522,31 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 	// made the compiler wait for every load in flight)
 	const __amdgpu_buffer_rsrc_t rays_rsrc = __builtin_amdgcn_make_buffer_rsrc(
 		const_cast<float4*>(a.rays + size_t(slot0) * n), short(0), int(np * n * 16u), 0x00020000);
-	const uint32_t my_off = (my_slot * n + uint32_t(lgrp)) * 16u;
-	const T* __restrict__ wsrc[2] = {static_cast<const T*>(a.w_hi16), static_cast<const T*>(a.w_lo16)};
+	constexpr bool kF32 = MODE == kBlendFP32;
+	// ray of K step s that load j of lane group g = l / 16 takes: 16-bit, 4 j + g (load j reads 4
+	// consecutive rays, 64 bytes, of each of the wave's 16 probes; the weights are stored in the
+	// matching wide_column order); FP32, 16 (j / 4) + 4 g + j % 4, so that MFMA j pairs the rays the
+	// 16-ray groups of blend_fp32_tile pair, in the same order (the same sums)
+	auto ray_of = [&](int j) { return kF32 ? uint32_t(16 * (j >> 2) + 4 * lgrp + (j & 3)) : uint32_t(4 * j + lgrp); };
+	const uint32_t my_off = (my_slot * n + (kF32 ? 4u * uint32_t(lgrp) : uint32_t(lgrp))) * 16u;
+	const T* __restrict__ wsrc[2] = {static_cast<const T*>(kF32 ? static_cast<const void*>(a.wt) : a.w_hi16),
+									  static_cast<const T*>(a.w_lo16)};
 
 	// weights of K step s into LDS buffer b: (part, tile) j by wave j % 8, lane l copies texel row
-	// 16 tile + l % 16, K block l / 16 (8 weights in wide_column order)
+	// 16 tile + l % 16, K block l / 16: 16-bit, 8 weights in wide_column order; FP32, the 4 weights
+	// of rays 16 part + 4 (l / 16) + 0..3 (the natural order of the FP32 weight rows)
 	auto stage_weights = [&](uint32_t s, int b) {
 #pragma unroll
 		for(int i = 0; i < P * 2; ++i) {
 			const int j = wave + 8 * i, part = j >> 4, tile = j & 15;
-			const T* src = wsrc[part] + size_t(16 * tile + lrow) * n_pad + s * kKStep + lgrp * 8;
+			const T* src = kF32 ? wsrc[0] + size_t(16 * tile + lrow) * n_pad + s * kKStep + part * 16 + lgrp * 4
+								: wsrc[part] + size_t(16 * tile + lrow) * n_pad + s * kKStep + lgrp * 8;
 			unsigned char* dst = lds.w[b] + (part * 16 + tile) * 1024;
 			lds_copy16(src, dst);
 		}
 	};
-	// ray data of K step s: lane (probe l % 16, K block g = l / 16) takes rays 32 s + 4 j + g,
-	// j = 0..7, so that load j of the wave reads 4 consecutive rays (64 bytes) of each of its 16
-	// probes (the weights are stored in the matching wide_column order).  Rays that do not exist
+	// ray data of K step s: lane (probe l % 16, K block g = l / 16) takes rays 32 s + ray_of(j),
+	// j = 0..7.  Rays that do not exist
 	// load as zero, which contributes nothing to any column: absent probes lie past the buffer's
 	// records, and in the last, partial K step of a ray count that is not a multiple of 32 the
 	// rays beyond n get an offset past the records as well.  The whole offset is in the range-
@@ -540,7 +559,8 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 		const uint32_t lim = s + 2 < steps ? uint32_t(kKStep) : (n > s * kKStep ? min(n - s * kKStep, uint32_t(kKStep)) : 0u);
 #pragma unroll
 		for(int j = 0; j < 8; ++j) {
-			const uint32_t off = uint32_t(4 * j + lgrp) < lim ? base + uint32_t(64 * j) : 0x80000000u;
+			const uint32_t rel = kF32 ? uint32_t(16 * (16 * (j >> 2) + (j & 3))) : uint32_t(64 * j);  // ray_of(j) - ray_of(0), bytes
+			const uint32_t off = ray_of(j) < lim ? base + rel : 0x80000000u;
 			const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rays_rsrc, off, 0, 0);
 			r[j] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
 		}
@@ -569,9 +589,17 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 				__asm__ volatile("" : "+v"(w));
 				r[j].w = w;
 			}
+		} else if constexpr(kF32) {
+			// the same for the FP32 irradiance half, whose MFMAs take the ray data unconverted
+#pragma unroll
+			for(int j = 0; j < 8; ++j) {
+				floatx4_t x = floatx4_t{r[j].x, r[j].y, r[j].z, r[j].w};
+				__asm__ volatile("" : "+v"(x));
+				r[j] = make_float4(x[0], x[1], x[2], x[3]);
+			}
 		}
 		if(s + 1 < steps) stage_weights(s + 1, b ^ 1);
-		// A fragments (hi / lo) of the data channels this half uses
+		// A fragments (hi / lo) of the data channels this half uses (16-bit)
 		V A[5][2];
 		float v[8][5];
 #pragma unroll
@@ -591,7 +619,9 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 		if constexpr(H == 0) __asm__ volatile("" : "+v"(my_oor));
 #pragma unroll
 		for(int c = kCh0; c < 5; ++c) {
-			if constexpr(P == 2) {
+			if constexpr(kF32) {
+				// FP32: the operands are the channel values themselves
+			} else if constexpr(P == 2) {
 				// two rays at a time: one packed rounding for the high parts, their exact float
 				// values by a shift and a mask, one packed rounding for the residuals
 				uint32_t hi[4], lo[4];
@@ -604,18 +634,41 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 				for(int j = 0; j < 8; ++j) A[c][0][j] = T(v[j][c]);
 			}
 		}
-		if(s + 2 < steps) load_rays(s + 2, r);
+		if constexpr(!kF32) {
+			if(s + 2 < steps) load_rays(s + 2, r);
+		}
 		const unsigned char* wb = lds.w[b] + lane * 16;
 		// per texel tile, the products in the order hi x hi, lo x hi, hi x lo for every channel, so
 		// that consecutive MFMAs update different accumulators
+		// FP32: the next tile's weights are read while this tile's MFMAs issue
+		float4 nb0, nb1;
+		if constexpr(kF32) {
+			nb0 = *reinterpret_cast<const float4*>(wb + kTile0 * 1024);
+			nb1 = *reinterpret_cast<const float4*>(wb + (16 + kTile0) * 1024);
+		}
 #pragma unroll
 		for(int tt = kTile0; tt < kTile1; ++tt) {
-			V B[2];
-#pragma unroll
-			for(int p = 0; p < P; ++p) B[p] = *reinterpret_cast<const V*>(wb + (p * 16 + tt) * 1024);
 			const int c0 = tt < 3 ? 0 : 3, nc = tt < 3 ? 3 : 2;
 			// accumulator of (channel c, tile tt): irradiance 3 tt + c; depth base + 2 (tt - first) + c
 			const int k0 = tt < 3 ? 3 * tt : (H == 0 ? 9 + 2 * (tt - 3) : 2 * (tt - 7));
+			if constexpr(kF32) {
+				// 8 MFMAs per accumulator: MFMA j multiplies ray ray_of(j) (K index l / 16) of each
+				// probe row with its weights, the channels interleaved
+				const float4 B0 = nb0, B1 = nb1;
+				if(tt + 1 < kTile1) {
+					nb0 = *reinterpret_cast<const float4*>(wb + (tt + 1) * 1024);
+					nb1 = *reinterpret_cast<const float4*>(wb + (16 + tt + 1) * 1024);
+				}
+				const float bw[8] = {B0.x, B0.y, B0.z, B0.w, B1.x, B1.y, B1.z, B1.w};
+#pragma unroll
+				for(int j = 0; j < 8; ++j)
+#pragma unroll
+					for(int c = 0; c < nc; ++c) acc[k0 + c] = M::mma(v[j][c0 + c], bw[j], acc[k0 + c]);
+				continue;
+			}
+			V B[2];
+#pragma unroll
+			for(int p = 0; p < P; ++p) B[p] = *reinterpret_cast<const V*>(wb + (p * 16 + tt) * 1024);
 #pragma unroll
 			for(int c = 0; c < nc; ++c) acc[k0 + c] = M::mma(A[c0 + c][0], B[0], acc[k0 + c]);
 			if(P == 2) {
@@ -624,6 +677,10 @@ __device__ __forceinline__ void wide_wave(const BlendArgs& a, uint32_t slot0, ui
 #pragma unroll
 				for(int c = 0; c < nc; ++c) acc[k0 + c] = M::mma(A[c0 + c][0], B[1], acc[k0 + c]);
 			}
+		}
+		// FP32: the MFMAs read the ray registers themselves, so the next loads into them follow
+		if constexpr(kF32) {
+			if(s + 2 < steps) load_rays(s + 2, r);
 		}
 	};
 
@@ -791,6 +848,7 @@ const void* wide_blend_kernel_of(uint32_t steps) {
 const void* wide_blend_kernel(uint32_t mode, uint32_t steps) {
 	if(mode == kBlendBF16) return wide_blend_kernel_of<kBlendBF16>(steps);
 	if(mode == kBlendFP16) return wide_blend_kernel_of<kBlendFP16>(steps);
+	if(mode == kBlendFP32) return wide_blend_kernel_of<kBlendFP32>(steps);
 	return nullptr;
 }
 
@@ -816,10 +874,10 @@ hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, const ui
 	return hipGetLastError();
 }
 
-hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s) {
+hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s, bool fp32_wide) {
 	if(a.max_slots == 0) return hipSuccess;
 	const uint32_t n_pad = (a.grid.n + kMfmaChunk - 1) & ~uint32_t(kMfmaChunk - 1);
-	if(mode == kBlendFP32) {
+	if(mode == kBlendFP32 && (!fp32_wide || (a.flags & kBlendIrradianceOnly))) {
 		hipLaunchKernelGGL(k_blend_fp32, dim3((a.max_slots + kProbesPerBlock - 1) / kProbesPerBlock), dim3(256), 0, s, a, n_pad);
 		return hipGetLastError();
 	}

@@ -24,6 +24,7 @@ struct Handle {
 	ddgi_grid_params params{};
 	ddgi_options opts{};
 	int device = 0;
+	uint32_t cus = 0;  // compute units of the device
 	uint32_t P = 0;
 	uint32_t max_rays = 0;
 	// atlases: [cur] is exposed, [cur ^ 1] receives the next update

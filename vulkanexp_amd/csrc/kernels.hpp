@@ -260,7 +260,10 @@ hipError_t launch_sample(const GridConst& g, const uint32_t* states, const float
 // consume ray records stored in the traversal's thread order.
 hipError_t launch_blend_weights(const GridConst& g, const float4* dirs, const uint16_t* order, float* wt, float* inv_sum,
 								void* w_hi16, void* w_lo16, uint32_t mode, hipStream_t s);
-hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s);
+// FP32 (mode kBlendFP32): fp32_wide selects the 64-probe workgroups of the 16-bit modes
+// (k_blend_wide) over the 16-probe blocks (k_blend_fp32); both give the same sums (the same MFMAs
+// per output element, in the same order).  The irradiance-only blend always takes the blocks.
+hipError_t launch_blend(const BlendArgs& a, uint32_t mode, hipStream_t s, bool fp32_wide = false);
 
 // blend modes
 constexpr uint32_t kBlendFP32 = 0;
