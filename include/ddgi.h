@@ -425,7 +425,8 @@ ddgi_status ddgi_synchronize(void* handle);
 typedef struct ddgi_batch_config {
 	uint32_t trace_block;       /* primary traversal workgroup size: 64, 128 or 256 threads */
 	uint32_t trace_item_blocks; /* work item = trace_block * trace_item_blocks consecutive rays of one volume (1..64) */
-	uint32_t blend_probes;      /* FP32 blend tile: probes per workgroup, 4, 8, 16 or 32 (MFMA blends use 64) */
+	uint32_t blend_probes;      /* FP32 blend tile: probes per workgroup, 4, 8, 16, 32 or 64 (64: the
+	                               512-thread workgroups of the 16-bit blends, which always use 64) */
 	uint32_t reserved[5];
 } ddgi_batch_config;
 

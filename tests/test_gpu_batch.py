@@ -85,8 +85,9 @@ def test_batch_matches_per_volume_updates(scene_list, flags):
 
 @pytest.mark.parametrize("cfg", [dict(trace_block=64, trace_item_blocks=1, blend_probes=4),
                                  dict(trace_block=256, trace_item_blocks=16, blend_probes=32),
-                                 dict(trace_block=128, trace_item_blocks=64, blend_probes=8)],
-                         ids=["b64-i1-p4", "b256-i16-p32", "b128-i64-p8"])
+                                 dict(trace_block=128, trace_item_blocks=64, blend_probes=8),
+                                 dict(trace_block=256, trace_item_blocks=8, blend_probes=64)],
+                         ids=["b64-i1-p4", "b256-i16-p32", "b128-i64-p8", "b256-i8-p64"])
 def test_configurations_give_identical_results(scene_list, cfg):
     a, b = make_volumes(scene_list), make_volumes(scene_list)
     ba, bb = ProbeBatch(a), ProbeBatch(b, config=cfg)
@@ -112,7 +113,7 @@ def test_autotune_restores_state_and_picks_a_measured_config(scene_list):
         before = state_of(vols)
         hyst = [float(p.params()["hysteresis"]) for p in vols]
         best, records = batch.autotune(frame_orientations(1, len(SPECS)), _lib.UPDATE_HYSTERESIS_RAMP, iterations=2)
-        assert len(records) == 9 + 4
+        assert len(records) == 9 + 5
         assert all(r["ms"] > 0 for r in records)
         assert best in [{k: r[k] for k in ("trace_block", "trace_item_blocks", "blend_probes")} for r in records]
         assert min(r["ms"] for r in records) == pytest.approx(

@@ -77,8 +77,8 @@ ddgi_status check_config(const ddgi_batch_config& c) {
 		return bfail(DDGI_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
 	if(c.trace_item_blocks < 1 || c.trace_item_blocks > 64)
 		return bfail(DDGI_ERROR_INVALID_ARGUMENT, "trace_item_blocks must be in [1, 64]");
-	if(c.blend_probes != 4 && c.blend_probes != 8 && c.blend_probes != 16 && c.blend_probes != 32)
-		return bfail(DDGI_ERROR_INVALID_ARGUMENT, "blend_probes must be 4, 8, 16 or 32");
+	if(c.blend_probes != 4 && c.blend_probes != 8 && c.blend_probes != 16 && c.blend_probes != 32 && c.blend_probes != 64)
+		return bfail(DDGI_ERROR_INVALID_ARGUMENT, "blend_probes must be 4, 8, 16, 32 or 64");
 	return DDGI_OK;
 }
 
@@ -540,10 +540,11 @@ ddgi_status ddgi_batch_autotune(void* batch, const float* orientations, uint32_t
 				win = c;
 			}
 		}
-	// phase 1: FP32 blend tile (the MFMA blends have one tile shape)
+	// phase 1: FP32 blend tile (the 16-bit blends have one tile shape); 64 = the 16-bit blends'
+	// 64-probe workgroups, which give the same sums as the smaller tiles
 	if(st == DDGI_OK && mode_of(flags) == kBlendFP32) {
 		const ddgi_batch_config base = win;
-		const uint32_t tiles[] = {4, 8, 16, 32};
+		const uint32_t tiles[] = {4, 8, 16, 32, 64};
 		for(uint32_t p : tiles) {
 			ddgi_batch_config c = base;
 			c.blend_probes = p;

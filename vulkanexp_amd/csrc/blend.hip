@@ -856,6 +856,7 @@ const void* batch_blend_kernel(uint32_t mode, uint32_t probes) {
 	if(mode == kBlendBF16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend_wide<kBlendBF16>) : nullptr;
 	if(mode == kBlendFP16) return probes == kMfmaProbes ? reinterpret_cast<const void*>(&k_batch_blend_wide<kBlendFP16>) : nullptr;
 	switch(probes) {
+		case kMfmaProbes: return reinterpret_cast<const void*>(&k_batch_blend_wide<kBlendFP32>);
 		case 4: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 4>);
 		case 8: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 8>);
 		case 16: return reinterpret_cast<const void*>(&k_batch_blend<kBlendFP32, 16>);
@@ -898,7 +899,7 @@ hipError_t launch_batch_blend(const BatchLaunch& b, hipStream_t s) {
 	if(!f || b.blend_item_count == 0) return hipErrorInvalidValue;
 	BatchLaunch arg = b;
 	void* args[] = {&arg};
-	return hipLaunchKernel(f, dim3(b.blend_item_count), dim3(b.blend_mode == kBlendFP32 ? 256 : kWideThreads), args, 0, s);
+	return hipLaunchKernel(f, dim3(b.blend_item_count), dim3(b.blend_probes == kMfmaProbes ? kWideThreads : 256), args, 0, s);
 }
 
 }  // namespace ddgi
