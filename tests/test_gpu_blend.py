@@ -121,8 +121,9 @@ def test_fp32_workgroup_shapes_give_the_same_sums(rays):
     """The FP32 blend runs in 64-probe workgroups (k_blend_wide) once they fill the device and in
     16-probe blocks (k_blend_fp32) below that; both issue the same MFMAs per output element in the
     same order, so their atlases and states are bit-identical.  A grid of 64 probes per compute unit
-    is blended whole (64-probe workgroups) and as its first half (16-probe blocks); the first half's
-    tiles and states must agree exactly (ray counts with a partial K step, 2, 8 and 16 K steps)."""
+    is blended whole (64-probe workgroups) and as its first half (16-probe blocks), the probes
+    listed in a shuffled order; the first half's tiles and states must agree exactly (ray counts
+    with a partial K step, 2, 8 and 16 K steps)."""
     import torch
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     scene, _ = scenes.config_small(seed=3, rays=rays, open_top=True)
@@ -141,6 +142,8 @@ def test_fp32_workgroup_shapes_give_the_same_sums(rays):
     p.update(R, _lib.UPDATE_ALL_PROBES)
     p.synchronize()
     ray_buf, idx = p.last_update_rays()
+    perm = np.random.Generator(np.random.PCG64(rays)).permutation(P)  # slots in any order
+    ray_buf, idx = ray_buf[perm], idx[perm]
     out = []
     for count, flags in ((P // 2, 0), (P, _lib.UPDATE_ALL_PROBES)):
         p.load_state(state)
