@@ -166,6 +166,9 @@ void free_scene(Handle* h) {
 	h->move_xf = nullptr;
 	h->move_meta = nullptr;
 	h->obj_tris = nullptr;
+	h->slot_box = nullptr;
+	h->rel_xf = nullptr;
+	h->build_xf.clear();
 	h->level_nodes = nullptr;
 	h->instances.clear();
 	for(auto& b : h->scene_bufs) (void)hipFree(b.p);
@@ -184,6 +187,47 @@ ddgi_status upload(Handle* h, const T* src, size_t count, const T** dst) {
 	if(count && src) HIP_TRY(h, hipMemcpy(b.p, src, count * sizeof(T), hipMemcpyHostToDevice));
 	if(dst) *dst = static_cast<const T*>(b.p);
 	return DDGI_OK;
+}
+
+// Row-major 3x4 identity transforms.
+std::vector<float> identity_xf(size_t count) {
+	std::vector<float> v(std::max<size_t>(count, 1) * 12, 0.0f);
+	for(size_t k = 0; k < count; ++k) v[k * 12 + 0] = v[k * 12 + 5] = v[k * 12 + 10] = 1.0f;
+	return v;
+}
+
+// out = M * B^-1 (row-major 3x4 affine transforms, in double): the motion of an instance since the
+// BVH was built with transform B.  Within 1e-12 of the identity it is the identity exactly, so an
+// instance moved back bounds its pieces as built.
+void relative_transform(const float* M, const float* B, float* out) {
+	const double a[9] = {B[0], B[1], B[2], B[4], B[5], B[6], B[8], B[9], B[10]};
+	const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+	const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+	double inv[9] = {c00 / det,
+					 (a[2] * a[7] - a[1] * a[8]) / det,
+					 (a[1] * a[5] - a[2] * a[4]) / det,
+					 c01 / det,
+					 (a[0] * a[8] - a[2] * a[6]) / det,
+					 (a[2] * a[3] - a[0] * a[5]) / det,
+					 c02 / det,
+					 (a[1] * a[6] - a[0] * a[7]) / det,
+					 (a[0] * a[4] - a[1] * a[3]) / det};
+	const double tb[3] = {B[3], B[7], B[11]};
+	double r[12];
+	bool ident = true;
+	for(int i = 0; i < 3; ++i) {
+		for(int j = 0; j < 3; ++j) {
+			r[i * 4 + j] = double(M[i * 4 + 0]) * inv[0 * 3 + j] + double(M[i * 4 + 1]) * inv[1 * 3 + j] +
+						   double(M[i * 4 + 2]) * inv[2 * 3 + j];
+			ident = ident && std::fabs(r[i * 4 + j] - (i == j ? 1.0 : 0.0)) < 1e-12;
+		}
+		// translation: M's offset minus M's linear part applied to B^-1's offset (B^-1 x = inv (x - tb))
+		double t = M[i * 4 + 3];
+		for(int j = 0; j < 3; ++j) t -= r[i * 4 + j] * tb[j];
+		r[i * 4 + 3] = t;
+		ident = ident && std::fabs(t) < 1e-12 * (1.0 + std::fabs(tb[0]) + std::fabs(tb[1]) + std::fabs(tb[2]));
+	}
+	for(int q = 0; q < 12; ++q) out[q] = ident ? ((q == 0 || q == 5 || q == 10) ? 1.0f : 0.0f) : float(r[q]);
 }
 
 // Device record of an instance with row-major 3x4 transform M: the normal matrix (inverse
@@ -647,6 +691,8 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	   (st = upload(h, reinterpret_cast<const float4*>(vertices), vertex_count * 4, &vp)) != DDGI_OK ||
 	   (st = upload(h, mats.data(), mats.size(), &mp)) != DDGI_OK ||
 	   (st = upload(h, bvh.obj_tris.data(), bvh.obj_tris.size(), &h->obj_tris)) != DDGI_OK ||
+	   (st = upload(h, bvh.slot_box.data(), bvh.slot_box.size(), &h->slot_box)) != DDGI_OK ||
+	   (st = upload(h, identity_xf(instance_count).data(), instance_count * 12, const_cast<const float**>(&h->rel_xf))) != DDGI_OK ||
 	   (st = upload(h, bvh.level_nodes.data(), bvh.level_nodes.size(), &h->level_nodes)) != DDGI_OK ||
 	   (st = upload<float>(h, nullptr, instance_count * 13, nullptr)) != DDGI_OK) {
 		free_scene(h);
@@ -656,7 +702,7 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	h->move_meta = reinterpret_cast<uint32_t*>(h->move_xf + instance_count * 12);
 	if(!h->move_done) HIP_TRY(h, hipEventCreateWithFlags(&h->move_done, hipEventDisableTiming));
 	{
-		const hipError_t e = hipHostMalloc(&h->move_host, std::max<size_t>(instance_count * (13 * 4 + sizeof(InstanceDev)), 16),
+		const hipError_t e = hipHostMalloc(&h->move_host, std::max<size_t>(instance_count * (25 * 4 + sizeof(InstanceDev)), 16),
 										   hipHostMallocDefault);
 		if(e != hipSuccess) {
 			h->move_host = nullptr;
@@ -665,6 +711,8 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 		}
 	}
 	h->instances.assign(instances, instances + instance_count);
+	h->build_xf.resize(instance_count * 12);
+	for(size_t k = 0; k < instance_count; ++k) std::memcpy(&h->build_xf[k * 12], instances[k].transform, 48);
 	sd.nodes = nodes;
 	sd.tris = tris;
 	sd.inst = ip;
@@ -682,6 +730,8 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
 	bvh.tris.clear();
 	bvh.obj_tris.clear();
 	bvh.obj_tris.shrink_to_fit();
+	bvh.slot_box.clear();
+	bvh.slot_box.shrink_to_fit();
 	h->bvh_meta = std::move(bvh);
 	h->has_scene = true;
 	h->has_metallic = metallic;
@@ -708,6 +758,7 @@ ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, 
 	float* xf = static_cast<float*>(h->move_host);
 	uint32_t* meta = reinterpret_cast<uint32_t*>(xf + n * 12);
 	InstanceDev* inst = reinterpret_cast<InstanceDev*>(meta + n);
+	float* rel = reinterpret_cast<float*>(inst + n);  // n * 12 floats
 	for(uint32_t k = 0; k < count; ++k) {
 		const ddgi_instance& in = h->instances[first + k];
 		const float* M = transforms + size_t(k) * 12;
@@ -718,6 +769,7 @@ ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, 
 						   double(M[2]) * (double(M[4]) * M[9] - double(M[5]) * M[8]);
 		const bool flip = (det < 0.0) != ((in.flags & DDGI_INSTANCE_FLIP_FACING) != 0);
 		meta[k] = (in.mask & 0xFFu) | (flip ? 0x100u : 0u);
+		relative_transform(M, &h->build_xf[size_t(first + k) * 12], rel + size_t(k) * 12);
 	}
 	std::memcpy(xf, transforms, size_t(count) * 48);
 	hipStream_t s = static_cast<hipStream_t>(stream);
@@ -725,12 +777,13 @@ ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, 
 	HIP_TRY(h, hipMemcpyAsync(h->move_meta, meta, size_t(count) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 	HIP_TRY(h, hipMemcpyAsync(const_cast<InstanceDev*>(h->sd.inst) + first, inst, size_t(count) * sizeof(InstanceDev),
 							  hipMemcpyHostToDevice, s));
+	HIP_TRY(h, hipMemcpyAsync(h->rel_xf + size_t(first) * 12, rel, size_t(count) * 48, hipMemcpyHostToDevice, s));
 	float4* tris = const_cast<float4*>(h->sd.tris);
 	float4* nodes = const_cast<float4*>(h->sd.nodes);
 	HIP_TRY(h, launch_retransform(tris, h->obj_tris, h->sd.tri_count, h->move_xf, h->move_meta, first, count, s));
 	const std::vector<uint32_t>& lv = h->bvh_meta.level_offsets;
 	for(size_t d = lv.size() - 1; d >= 1; --d)
-		HIP_TRY(h, launch_refit_level(nodes, tris, h->level_nodes + lv[d - 1], lv[d] - lv[d - 1], s));
+		HIP_TRY(h, launch_refit_level(nodes, tris, h->slot_box, h->rel_xf, h->level_nodes + lv[d - 1], lv[d] - lv[d - 1], s));
 	// the staging area and move_xf / move_meta are reused once this has run
 	HIP_TRY(h, hipEventRecord(h->move_done, s));
 	h->move_pending = true;

@@ -1,4 +1,4 @@
-// Binned-SAH BVH builder (host).  See bvh.hpp for the device layout.
+// Binned-SAH BVH builder (host) with spatial splits.  See bvh.hpp for the device layout.
 #include "bvh.hpp"
 
 #include <algorithm>
@@ -13,6 +13,13 @@ namespace {
 constexpr int kBins = 32;
 constexpr double kCostTraverse = 1.0;  // one 64-byte node fetch + 2 slab tests
 constexpr double kCostTriangle = 1.0;  // one 48-byte triangle fetch + watertight test
+// Spatial splits (Stich, Friedrich and Dietrich, "Spatial Splits in Bounding Volume Hierarchies",
+// HPG 2009) are tried where the best object split's children overlap by more than kSplitAlpha of
+// the root's surface, while the references stay within kSplitBudget times the triangles; below
+// kSplitMaxDepth binary levels the builder uses object splits only.
+constexpr double kSplitAlpha = 1e-3;
+constexpr double kSplitBudget = 1.5;
+constexpr uint32_t kSplitMaxDepth = 48;
 
 struct Aabb {
 	float lo[3], hi[3];
@@ -49,64 +56,146 @@ struct BuildNode {
 	bool leaf() const { return left < 0; }
 };
 
+// A reference to triangle `tri`: the bounds of the part of it this subtree holds (the whole
+// triangle, or the piece a spatial split left on one side).
+struct Ref {
+	uint32_t tri;
+	Aabb box;
+	float centroid(int a) const { return 0.5f * (box.lo[a] + box.hi[a]); }
+};
+
+// Convex polygon (a triangle cut by axis-aligned planes: at most 3 + 2 vertices), in double.
+struct Poly {
+	double v[8][3];
+	int n = 0;
+};
+
+// The part of `in` with x[axis] <= plane (keep_low) or >= plane.
+void clip_poly(const Poly& in, int axis, double plane, bool keep_low, Poly& out) {
+	out.n = 0;
+	for(int i = 0; i < in.n; ++i) {
+		const double* a = in.v[i];
+		const double* b = in.v[(i + 1) % in.n];
+		const double da = keep_low ? plane - a[axis] : a[axis] - plane;
+		const double db = keep_low ? plane - b[axis] : b[axis] - plane;
+		if(da >= 0.0) {
+			for(int k = 0; k < 3; ++k) out.v[out.n][k] = a[k];
+			++out.n;
+		}
+		if((da >= 0.0) != (db >= 0.0)) {
+			const double u = da / (da - db);
+			for(int k = 0; k < 3; ++k) out.v[out.n][k] = a[k] + (b[k] - a[k]) * u;
+			out.v[out.n][axis] = plane;
+			++out.n;
+		}
+	}
+}
+
+// Float bounds containing the polygon (rounded outward), intersected with `within`; empty if the
+// polygon is empty or misses `within`.
+Aabb poly_bounds(const Poly& p, const Aabb& within) {
+	Aabb out;
+	if(p.n == 0) return out;
+	for(int i = 0; i < p.n; ++i)
+		for(int k = 0; k < 3; ++k) {
+			float lo = float(p.v[i][k]), hi = lo;
+			if(double(lo) > p.v[i][k]) lo = std::nextafter(lo, -std::numeric_limits<float>::infinity());
+			if(double(hi) < p.v[i][k]) hi = std::nextafter(hi, std::numeric_limits<float>::infinity());
+			out.lo[k] = std::min(out.lo[k], lo);
+			out.hi[k] = std::max(out.hi[k], hi);
+		}
+	for(int k = 0; k < 3; ++k) {
+		out.lo[k] = std::max(out.lo[k], within.lo[k]);
+		out.hi[k] = std::min(out.hi[k], within.hi[k]);
+		if(out.lo[k] > out.hi[k]) return Aabb();
+	}
+	return out;
+}
+
 struct Builder {
-	const std::vector<Aabb>& tri_box;
-	const std::vector<float>& centroid;  // 3 per triangle
-	std::vector<uint32_t> order;
+	const std::vector<float>& world;  // 9 floats per triangle
+	std::vector<uint32_t> order;      // triangle of each slot, leaf order (a split triangle repeats)
+	std::vector<Aabb> order_box;      // the bounds of the piece each slot holds
 	std::vector<BuildNode> nodes;
 	uint32_t max_depth = 0;
+	bool spatial = true;
 
-	Builder(const std::vector<Aabb>& tb, const std::vector<float>& c) : tri_box(tb), centroid(c) {}
+	explicit Builder(const std::vector<float>& w) : world(w) {}
 
-	int bin_of(uint32_t tri, int axis, float cmin, double scale) const {
-		int b = int((double(centroid[tri * 3 + axis]) - cmin) * scale);
-		return std::min(std::max(b, 0), kBins - 1);
+	Poly triangle(uint32_t t) const {
+		Poly p;
+		p.n = 3;
+		for(int v = 0; v < 3; ++v)
+			for(int k = 0; k < 3; ++k) p.v[v][k] = world[size_t(t) * 9 + v * 3 + k];
+		return p;
 	}
 
-	void build(uint32_t n) {
-		order.resize(n);
-		for(uint32_t i = 0; i < n; ++i) order[i] = i;
-		nodes.reserve(2 * size_t(n) / 2 + 8);
+	// Bounds of the pieces of reference r below and above `plane` on `axis`.
+	void split_ref(const Ref& r, int axis, double plane, Aabb& lo_box, Aabb& hi_box) const {
+		const Poly tri = triangle(r.tri);
+		Poly part;
+		clip_poly(tri, axis, plane, true, part);
+		lo_box = poly_bounds(part, r.box);
+		clip_poly(tri, axis, plane, false, part);
+		hi_box = poly_bounds(part, r.box);
+	}
+
+	void build(std::vector<Ref> all) {
+		const size_t n = all.size();
+		const size_t budget = std::min(size_t(double(n) * kSplitBudget), (size_t(1) << 28) - 1);
+		size_t refs = n;
+		nodes.reserve(2 * n + 8);
 		nodes.emplace_back();
 		struct Job {
-			uint32_t node, begin, end, depth;
+			uint32_t node, depth;
+			std::vector<Ref> refs;
 		};
-		std::vector<Job> stack{{0, 0, n, 0}};
+		std::vector<Job> stack;
+		stack.push_back(Job{0, 0, std::move(all)});
+		double root_area = 0.0;
 		while(!stack.empty()) {
-			Job j = stack.back();
+			Job j = std::move(stack.back());
 			stack.pop_back();
 			max_depth = std::max(max_depth, j.depth);
+			std::vector<Ref>& R = j.refs;
 			Aabb box, cbox;
-			for(uint32_t i = j.begin; i < j.end; ++i) {
-				box.grow(tri_box[order[i]]);
-				cbox.grow(&centroid[order[i] * 3]);
+			for(const Ref& r : R) {
+				box.grow(r.box);
+				const float c[3] = {r.centroid(0), r.centroid(1), r.centroid(2)};
+				cbox.grow(c);
 			}
 			nodes[j.node].box = box;
-			const uint32_t cnt = j.end - j.begin;
+			if(j.node == 0) root_area = std::max(box.half_area(), 1e-30);
+			const uint32_t cnt = uint32_t(R.size());
 			const double leaf_cost = kCostTriangle * cnt;
+			const double parent_area = std::max(box.half_area(), 1e-30);
+			// object split: references binned by the centroids of their bounds
 			double best = std::numeric_limits<double>::max();
 			int best_axis = -1, best_bin = -1;
-			const double parent_area = std::max(box.half_area(), 1e-30);
+			Aabb best_l, best_r;
+			auto bin_of = [&](const Ref& r, int ax, double scale) {
+				const int b = int((double(r.centroid(ax)) - cbox.lo[ax]) * scale);
+				return std::min(std::max(b, 0), kBins - 1);
+			};
 			for(int ax = 0; ax < 3 && cnt > 1; ++ax) {
 				const double ext = double(cbox.hi[ax]) - cbox.lo[ax];
 				if(!(ext > 0.0)) continue;
 				const double scale = kBins / ext;
 				Aabb bb[kBins];
 				uint32_t bc[kBins] = {};
-				for(uint32_t i = j.begin; i < j.end; ++i) {
-					const uint32_t t = order[i];
-					const int b = bin_of(t, ax, cbox.lo[ax], scale);
-					bb[b].grow(tri_box[t]);
+				for(const Ref& r : R) {
+					const int b = bin_of(r, ax, scale);
+					bb[b].grow(r.box);
 					++bc[b];
 				}
-				double ra[kBins];
+				Aabb racc[kBins];
 				uint32_t rc[kBins];
 				Aabb acc;
 				uint32_t c = 0;
 				for(int b = kBins - 1; b > 0; --b) {
 					acc.grow(bb[b]);
 					c += bc[b];
-					ra[b] = acc.half_area();
+					racc[b] = acc;
 					rc[b] = c;
 				}
 				Aabb lacc;
@@ -115,38 +204,137 @@ struct Builder {
 					lacc.grow(bb[b]);
 					lc += bc[b];
 					if(lc == 0 || rc[b + 1] == 0) continue;
-					const double cost = kCostTraverse + kCostTriangle * (lacc.half_area() * lc + ra[b + 1] * rc[b + 1]) / parent_area;
+					const double cost =
+						kCostTraverse + kCostTriangle * (lacc.half_area() * lc + racc[b + 1].half_area() * rc[b + 1]) / parent_area;
 					if(cost < best) {
 						best = cost;
 						best_axis = ax;
 						best_bin = b;
+						best_l = lacc;
+						best_r = racc[b + 1];
+					}
+				}
+			}
+			// spatial split: bins of equal width over the node's bounds; a reference enters the bin
+			// of its low end and leaves from the bin of its high end, and every bin it spans grows
+			// by the bounds of the piece of the triangle inside it (cut bin by bin)
+			double best_s = std::numeric_limits<double>::max();
+			int s_axis = -1;
+			double s_plane = 0.0;
+			bool try_spatial = spatial && best_axis >= 0 && refs < budget && j.depth < kSplitMaxDepth;
+			if(try_spatial) {
+				Aabb ov;
+				for(int k = 0; k < 3; ++k) {
+					ov.lo[k] = std::max(best_l.lo[k], best_r.lo[k]);
+					ov.hi[k] = std::min(best_l.hi[k], best_r.hi[k]);
+				}
+				const bool overlap = ov.lo[0] <= ov.hi[0] && ov.lo[1] <= ov.hi[1] && ov.lo[2] <= ov.hi[2];
+				try_spatial = overlap && ov.half_area() > kSplitAlpha * root_area;
+			}
+			for(int ax = 0; ax < 3 && try_spatial; ++ax) {
+				const double lo = box.lo[ax], ext = double(box.hi[ax]) - lo;
+				if(!(ext > 0.0)) continue;
+				const double w = ext / kBins;
+				auto plane_at = [&](int b) { return lo + w * b; };  // low plane of bin b
+				Aabb bb[kBins];
+				uint32_t entry[kBins] = {}, leave[kBins] = {};
+				for(const Ref& r : R) {
+					int b0 = int((double(r.box.lo[ax]) - lo) / w), b1 = int((double(r.box.hi[ax]) - lo) / w);
+					b0 = std::min(std::max(b0, 0), kBins - 1);
+					b1 = std::min(std::max(b1, b0), kBins - 1);
+					++entry[b0];
+					++leave[b1];
+					if(b0 == b1) {
+						bb[b0].grow(r.box);
+						continue;
+					}
+					Poly rest = triangle(r.tri), piece, next;
+					for(int b = b0; b < b1; ++b) {
+						clip_poly(rest, ax, plane_at(b + 1), true, piece);
+						const Aabb pb = poly_bounds(piece, r.box);
+						if(!pb.empty()) bb[b].grow(pb);
+						clip_poly(rest, ax, plane_at(b + 1), false, next);
+						rest = next;
+					}
+					const Aabb pb = poly_bounds(rest, r.box);
+					if(!pb.empty()) bb[b1].grow(pb);
+				}
+				Aabb racc[kBins];
+				uint32_t rc[kBins];
+				Aabb acc;
+				uint32_t c = 0;
+				for(int b = kBins - 1; b > 0; --b) {
+					acc.grow(bb[b]);
+					c += leave[b];
+					racc[b] = acc;
+					rc[b] = c;
+				}
+				Aabb lacc;
+				uint32_t lc = 0;
+				for(int b = 0; b < kBins - 1; ++b) {
+					lacc.grow(bb[b]);
+					lc += entry[b];
+					if(lc == 0 || rc[b + 1] == 0) continue;
+					const double cost =
+						kCostTraverse + kCostTriangle * (lacc.half_area() * lc + racc[b + 1].half_area() * rc[b + 1]) / parent_area;
+					if(cost < best_s) {
+						best_s = cost;
+						s_axis = ax;
+						s_plane = plane_at(b + 1);
 					}
 				}
 			}
 			const bool can_leaf = cnt <= kMaxLeafSize;
-			if(cnt == 1 || (can_leaf && leaf_cost <= best) || (best_axis < 0 && can_leaf)) {
-				nodes[j.node].first = j.begin;
+			if(cnt == 1 || (can_leaf && leaf_cost <= std::min(best, best_s)) || (best_axis < 0 && s_axis < 0 && can_leaf)) {
+				nodes[j.node].first = uint32_t(order.size());
 				nodes[j.node].count = cnt;
+				for(const Ref& r : R) {
+					order.push_back(r.tri);
+					order_box.push_back(r.box);
+				}
 				continue;
 			}
-			uint32_t mid;
-			if(best_axis < 0) {
-				mid = j.begin + cnt / 2;  // coincident centroids: split by order
-			} else {
-				const double scale = kBins / (double(cbox.hi[best_axis]) - cbox.lo[best_axis]);
-				const float cmin = cbox.lo[best_axis];
-				auto it = std::partition(order.begin() + j.begin, order.begin() + j.end,
-										 [&](uint32_t t) { return bin_of(t, best_axis, cmin, scale) <= best_bin; });
-				mid = uint32_t(it - order.begin());
-				if(mid == j.begin || mid == j.end) mid = j.begin + cnt / 2;
+			std::vector<Ref> L, H;
+			if(s_axis >= 0 && best_s < best) {
+				// a reference on one side goes there whole; one across the plane is cut in two
+				size_t cut = 0;
+				for(const Ref& r : R) {
+					if(double(r.box.hi[s_axis]) <= s_plane) {
+						L.push_back(r);
+					} else if(double(r.box.lo[s_axis]) >= s_plane) {
+						H.push_back(r);
+					} else {
+						Aabb lb, hb;
+						split_ref(r, s_axis, s_plane, lb, hb);
+						if(!lb.empty()) L.push_back(Ref{r.tri, lb});
+						if(!hb.empty()) H.push_back(Ref{r.tri, hb});
+						cut += !lb.empty() && !hb.empty();
+					}
+				}
+				refs += cut;
 			}
+			if(L.empty() || H.empty()) {  // object split (or halves when the centroids coincide)
+				L.clear();
+				H.clear();
+				if(best_axis >= 0) {
+					const double scale = kBins / (double(cbox.hi[best_axis]) - cbox.lo[best_axis]);
+					for(const Ref& r : R) (bin_of(r, best_axis, scale) <= best_bin ? L : H).push_back(r);
+				}
+				if(L.empty() || H.empty()) {
+					L.clear();
+					H.clear();
+					for(uint32_t i = 0; i < cnt; ++i) (i < cnt / 2 ? L : H).push_back(R[i]);
+				}
+			}
+			R.clear();
+			R.shrink_to_fit();
 			const int32_t l = int32_t(nodes.size());
 			nodes.emplace_back();
 			nodes.emplace_back();
 			nodes[j.node].left = l;
 			nodes[j.node].right = l + 1;
-			stack.push_back({uint32_t(l + 1), mid, j.end, j.depth + 1});
-			stack.push_back({uint32_t(l), j.begin, mid, j.depth + 1});
+			stack.push_back(Job{uint32_t(l + 1), j.depth + 1, std::move(H)});
+			stack.push_back(Job{uint32_t(l), j.depth + 1, std::move(L)});
 		}
 	}
 };
@@ -188,84 +376,33 @@ void transform_point(const float* M, const float* p, float out[3]) {
 	}
 }
 
-bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices, size_t index_count,
-			   const ddgi_offset_entry* offsets, size_t offset_count, const ddgi_instance* instances,
-			   size_t instance_count, BvhHost& out, const char** err) {
-	out = BvhHost{};
-	// ---- flatten instances (TLAS order) into world-space triangles ----
-	size_t total = 0;
-	for(size_t k = 0; k < instance_count; ++k) total += instances[k].triangle_count;
-	if(total >= (size_t(1) << 28)) {
-		*err = "scene exceeds 2^28 triangles";
-		return false;
-	}
-	std::vector<float> world(total * 9), object(total * 9);
-	std::vector<uint32_t> inst_of(total), meta(total);
-	out.inst_prim_base.resize(instance_count);
-	size_t g = 0;
-	for(size_t k = 0; k < instance_count; ++k) {
-		const ddgi_instance& in = instances[k];
-		if(in.custom_index >= offset_count) {
-			*err = "instance custom_index outside the offset table";
-			return false;
-		}
-		const ddgi_offset_entry& oe = offsets[in.custom_index];
-		if(size_t(oe.index_offset) + size_t(in.triangle_count) * 3 > index_count) {
-			*err = "instance triangles exceed the index buffer";
-			return false;
-		}
-		for(int q = 0; q < 12; ++q) {
-			if(!std::isfinite(in.transform[q])) {
-				*err = "non-finite instance transform";
-				return false;
-			}
-		}
-		const float* M = in.transform;
-		const double det = double(M[0]) * (double(M[5]) * M[10] - double(M[6]) * M[9]) -
-						   double(M[1]) * (double(M[4]) * M[10] - double(M[6]) * M[8]) +
-						   double(M[2]) * (double(M[4]) * M[9] - double(M[5]) * M[8]);
-		const bool flip = (det < 0.0) != ((in.flags & DDGI_INSTANCE_FLIP_FACING) != 0);
-		out.inst_prim_base[k] = uint32_t(g);
-		for(uint32_t p = 0; p < in.triangle_count; ++p, ++g) {
-			for(int v = 0; v < 3; ++v) {
-				const uint32_t idx = indices[oe.index_offset + 3 * p + v];
-				if(size_t(oe.vertex_offset) + idx >= vertex_count) {
-					*err = "index outside the vertex buffer";
-					return false;
-				}
-				const float* pos = vertices[oe.vertex_offset + idx].pos;
-				if(!std::isfinite(pos[0]) || !std::isfinite(pos[1]) || !std::isfinite(pos[2])) {
-					*err = "non-finite vertex position";
-					return false;
-				}
-				transform_point(M, pos, &world[g * 9 + v * 3]);
-				for(int q = 0; q < 3; ++q) object[g * 9 + v * 3 + q] = pos[q];
-			}
-			inst_of[g] = uint32_t(k);
-			meta[g] = (in.mask & 0xFFu) | (flip ? 0x100u : 0u);
-		}
-	}
-	out.tri_count = uint32_t(total);
-	if(total == 0) {
-		out.node_count = 0;
-		return true;
-	}
-	// ---- build ----
-	std::vector<Aabb> tb(total);
-	std::vector<float> cen(total * 3);
+// Build the tree over the flattened triangles and emit the device layout into `out` (its node,
+// triangle and level fields).
+void emit_tree(const std::vector<float>& world, const std::vector<float>& object, const std::vector<uint32_t>& inst_of,
+			   const std::vector<uint32_t>& meta, bool spatial, BvhHost& out) {
+	const size_t total = inst_of.size();
+	std::vector<Ref> refs(total);
 	for(size_t i = 0; i < total; ++i) {
-		for(int v = 0; v < 3; ++v) tb[i].grow(&world[i * 9 + v * 3]);
-		for(int k = 0; k < 3; ++k) cen[i * 3 + k] = 0.5f * (tb[i].lo[k] + tb[i].hi[k]);
+		refs[i].tri = uint32_t(i);
+		for(int v = 0; v < 3; ++v) refs[i].box.grow(&world[i * 9 + v * 3]);
 	}
-	Builder b(tb, cen);
-	b.build(uint32_t(total));
+	Builder b(world);
+	b.spatial = spatial;
+	b.build(std::move(refs));
 	out.max_depth = b.max_depth;
+	const size_t slots = b.order.size();
+	out.tri_count = uint32_t(slots);
 
 	// triangles in leaf order
-	out.tris.resize(total * 12);
-	out.obj_tris.resize(total * 9);
-	for(size_t i = 0; i < total; ++i) {
+	out.tris.resize(slots * 12);
+	out.obj_tris.resize(slots * 9);
+	out.slot_box.resize(slots * 6);
+	for(size_t i = 0; i < slots; ++i) {
 		const uint32_t t = b.order[i];
+		for(int a = 0; a < 3; ++a) {
+			out.slot_box[i * 6 + a] = b.order_box[i].lo[a];
+			out.slot_box[i * 6 + 3 + a] = b.order_box[i].hi[a];
+		}
 		std::memcpy(&out.obj_tris[i * 9], &object[t * 9], 9 * sizeof(float));
 		float* d = &out.tris[i * 12];
 		for(int v = 0; v < 3; ++v) {
@@ -381,6 +518,73 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		cost += n.leaf() ? a * kCostTriangle * n.count : a * kCostTraverse;
 	}
 	out.sah_cost = cost;
+}
+
+bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices, size_t index_count,
+			   const ddgi_offset_entry* offsets, size_t offset_count, const ddgi_instance* instances,
+			   size_t instance_count, BvhHost& out, const char** err) {
+	out = BvhHost{};
+	// ---- flatten instances (TLAS order) into world-space triangles ----
+	size_t total = 0;
+	for(size_t k = 0; k < instance_count; ++k) total += instances[k].triangle_count;
+	if(total >= (size_t(1) << 28)) {
+		*err = "scene exceeds 2^28 triangles";
+		return false;
+	}
+	std::vector<float> world(total * 9), object(total * 9);
+	std::vector<uint32_t> inst_of(total), meta(total);
+	out.inst_prim_base.resize(instance_count);
+	size_t g = 0;
+	for(size_t k = 0; k < instance_count; ++k) {
+		const ddgi_instance& in = instances[k];
+		if(in.custom_index >= offset_count) {
+			*err = "instance custom_index outside the offset table";
+			return false;
+		}
+		const ddgi_offset_entry& oe = offsets[in.custom_index];
+		if(size_t(oe.index_offset) + size_t(in.triangle_count) * 3 > index_count) {
+			*err = "instance triangles exceed the index buffer";
+			return false;
+		}
+		for(int q = 0; q < 12; ++q) {
+			if(!std::isfinite(in.transform[q])) {
+				*err = "non-finite instance transform";
+				return false;
+			}
+		}
+		const float* M = in.transform;
+		const double det = double(M[0]) * (double(M[5]) * M[10] - double(M[6]) * M[9]) -
+						   double(M[1]) * (double(M[4]) * M[10] - double(M[6]) * M[8]) +
+						   double(M[2]) * (double(M[4]) * M[9] - double(M[5]) * M[8]);
+		const bool flip = (det < 0.0) != ((in.flags & DDGI_INSTANCE_FLIP_FACING) != 0);
+		out.inst_prim_base[k] = uint32_t(g);
+		for(uint32_t p = 0; p < in.triangle_count; ++p, ++g) {
+			for(int v = 0; v < 3; ++v) {
+				const uint32_t idx = indices[oe.index_offset + 3 * p + v];
+				if(size_t(oe.vertex_offset) + idx >= vertex_count) {
+					*err = "index outside the vertex buffer";
+					return false;
+				}
+				const float* pos = vertices[oe.vertex_offset + idx].pos;
+				if(!std::isfinite(pos[0]) || !std::isfinite(pos[1]) || !std::isfinite(pos[2])) {
+					*err = "non-finite vertex position";
+					return false;
+				}
+				transform_point(M, pos, &world[g * 9 + v * 3]);
+				for(int q = 0; q < 3; ++q) object[g * 9 + v * 3 + q] = pos[q];
+			}
+			inst_of[g] = uint32_t(k);
+			meta[g] = (in.mask & 0xFFu) | (flip ? 0x100u : 0u);
+		}
+	}
+	out.tri_count = uint32_t(total);
+	if(total == 0) {
+		out.node_count = 0;
+		return true;
+	}
+	// ---- build: with spatial splits, unless that tree is deeper than the traversal stack allows ----
+	emit_tree(world, object, inst_of, meta, true, out);
+	if(out.wide_depth > kMaxWideDepth) emit_tree(world, object, inst_of, meta, false, out);
 	return true;
 }
 

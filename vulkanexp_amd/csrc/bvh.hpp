@@ -11,7 +11,8 @@
 //               row 6 = child codes as int bits: inner node index, or
 //                       leaf = 0x80000000 | first_triangle << 3 | (count - 1), or kEmptyChild
 //             then padding (4-wide: 128 bytes, 8-wide: 256 bytes)
-//   triangle: 3 float4 = 48 bytes, leaf order:
+//   triangle: 3 float4 = 48 bytes, leaf order (a slot; a triangle cut by spatial splits fills one
+//             slot per piece, each a full copy of the triangle whose leaf box bounds only its piece):
 //               [0] = (v0.xyz, global triangle id)   global id = instance-major flattened index
 //               [1] = (v1.xyz, instance index)
 //               [2] = (v2.xyz, mask | flip_facing << 8)
@@ -50,6 +51,10 @@ struct BvhHost {
 	// order) and the wide nodes grouped by depth (level d = nodes level_nodes[level_offsets[d-1] ..
 	// level_offsets[d]), root at level 1)
 	std::vector<float> obj_tris;
+	// per slot the bounds of the part of its triangle that the slot holds (the whole triangle, or
+	// the piece a spatial split left there; 6 floats: lo xyz, hi xyz), which a refit keeps for
+	// instances that have not moved
+	std::vector<float> slot_box;
 	std::vector<uint32_t> level_nodes;
 	std::vector<uint32_t> level_offsets;
 };

@@ -75,6 +75,9 @@ struct Handle {
 	// staging area (`move_host`, reused once `move_done` has fired) to `move_xf` / `move_meta`
 	std::vector<ddgi_instance> instances;
 	const float* obj_tris = nullptr;
+	const float* slot_box = nullptr;  // the builder's piece bounds per slot (BvhHost::slot_box)
+	float* rel_xf = nullptr;          // per instance: current transform after the inverse of the build's (3x4)
+	std::vector<float> build_xf;      // per instance: the transform the BVH was built with
 	const uint32_t* level_nodes = nullptr;
 	float* move_xf = nullptr;
 	uint32_t* move_meta = nullptr;

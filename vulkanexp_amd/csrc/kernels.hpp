@@ -237,11 +237,13 @@ hipError_t launch_sky(const TraceFrameArgs& a, hipStream_t s);
 hipError_t launch_trace_rays(const SceneDev& sc, const ddgi_ray* rays, ddgi_hit* hits, uint64_t count, uint32_t mask,
 							 bool any_hit, bool cull, hipStream_t s);
 // Instance moves: re-transform the triangles of instances [first, first + count) (xf: 12 floats,
-// meta: mask | flip << 8 per moved instance), then refit the wide tree level by level, deepest first.
+// meta: mask | flip << 8 per moved instance), then refit the wide tree level by level, deepest
+// first (slot_box: the builder's piece bounds per slot; rel_xf: per instance its motion since the
+// build, 3x4).
 hipError_t launch_retransform(float4* tris, const float* obj, uint32_t tri_count, const float* xf, const uint32_t* meta,
 							  uint32_t first, uint32_t count, hipStream_t s);
-hipError_t launch_refit_level(float4* nodes, const float4* tris, const uint32_t* level_nodes, uint32_t count,
-							  hipStream_t s);
+hipError_t launch_refit_level(float4* nodes, const float4* tris, const float* slot_box, const float* rel_xf,
+							  const uint32_t* level_nodes, uint32_t count, hipStream_t s);
 hipError_t launch_gbuffer(const SceneDev& sc, const ddgi_ray* rays, uint64_t count, ddgi_gbuffer_texel* out, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, hipStream_t s);
 hipError_t launch_direct_light(const DirectLightArgs& a, hipStream_t s);

@@ -932,6 +932,7 @@ __device__ __forceinline__ float pad_up_dev(float v) {
 // Refit of one level of the wide tree (deeper levels already refitted): every child box of the
 // level's nodes becomes the padded bounds of its triangles (leaf) or of its child node's boxes.
 __global__ void k_refit_level(float4* __restrict__ nodes, const float4* __restrict__ tris,
+							  const float* __restrict__ slot_box, const float* __restrict__ rel_xf,
 							  const uint32_t* __restrict__ level_nodes, uint32_t count) {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= count) return;
@@ -943,15 +944,47 @@ __global__ void k_refit_level(float4* __restrict__ nodes, const float4* __restri
 		float hi[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
 		if(code & kLeafFlag) {
 			const uint32_t firstt = (code & ~kLeafFlag) >> 3, cnt = (code & 7u) + 1u;
-			for(uint32_t q = 0; q < cnt; ++q)
+			for(uint32_t q = 0; q < cnt; ++q) {
+				const size_t slot = size_t(firstt + q);
+				float tlo[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+				float thi[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
 				for(int v = 0; v < 3; ++v) {
-					const float4 p = tris[size_t(firstt + q) * 3 + v];
+					const float4 p = tris[slot * 3 + v];
 					const float c[3] = {p.x, p.y, p.z};
 					for(int a = 0; a < 3; ++a) {
-						lo[a] = fminf(lo[a], c[a]);
-						hi[a] = fmaxf(hi[a], c[a]);
+						tlo[a] = fminf(tlo[a], c[a]);
+						thi[a] = fmaxf(thi[a], c[a]);
 					}
 				}
+				// the slot bounds only its piece of the triangle (a spatial split may have cut it): the
+				// piece's box at the build, carried by the instance's motion since (the box of its 8
+				// moved corners, in double, rounded outward), clips the triangle's box
+				{
+					const float* M = rel_xf + size_t(__float_as_uint(tris[slot * 3 + 1].w)) * 12;
+					const float* bx = slot_box + slot * 6;
+					double mlo[3] = {1e300, 1e300, 1e300}, mhi[3] = {-1e300, -1e300, -1e300};
+					for(int c = 0; c < 8; ++c) {
+						const double p[3] = {bx[(c & 1) ? 3 : 0], bx[(c & 2) ? 4 : 1], bx[(c & 4) ? 5 : 2]};
+						for(int r = 0; r < 3; ++r) {
+							const double q = double(M[r * 4 + 0]) * p[0] + double(M[r * 4 + 1]) * p[1] +
+											 double(M[r * 4 + 2]) * p[2] + double(M[r * 4 + 3]);
+							mlo[r] = fmin(mlo[r], q);
+							mhi[r] = fmax(mhi[r], q);
+						}
+					}
+					for(int a = 0; a < 3; ++a) {
+						float flo = float(mlo[a]), fhi = float(mhi[a]);
+						if(double(flo) > mlo[a]) flo = nextafterf(flo, -__int_as_float(0x7f800000));
+						if(double(fhi) < mhi[a]) fhi = nextafterf(fhi, __int_as_float(0x7f800000));
+						tlo[a] = fmaxf(tlo[a], flo);
+						thi[a] = fminf(thi[a], fhi);
+					}
+				}
+				for(int a = 0; a < 3; ++a) {
+					lo[a] = fminf(lo[a], tlo[a]);
+					hi[a] = fmaxf(hi[a], thi[a]);
+				}
+			}
 			for(int a = 0; a < 3; ++a) {
 				lo[a] = pad_down_dev(lo[a]);
 				hi[a] = pad_up_dev(hi[a]);
@@ -1214,10 +1247,11 @@ hipError_t launch_retransform(float4* tris, const float* obj, uint32_t tri_count
 	return hipGetLastError();
 }
 
-hipError_t launch_refit_level(float4* nodes, const float4* tris, const uint32_t* level_nodes, uint32_t count,
-							  hipStream_t s) {
+hipError_t launch_refit_level(float4* nodes, const float4* tris, const float* slot_box, const float* rel_xf,
+							  const uint32_t* level_nodes, uint32_t count, hipStream_t s) {
 	if(count == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(count, 128)), dim3(128), 0, s, nodes, tris, level_nodes, count);
+	hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(count, 128)), dim3(128), 0, s, nodes, tris, slot_box, rel_xf,
+					   level_nodes, count);
 	return hipGetLastError();
 }
 
