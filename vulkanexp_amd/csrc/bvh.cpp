@@ -2,9 +2,11 @@
 #include "bvh.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 namespace ddgi {
 
@@ -119,8 +121,29 @@ struct Builder {
 	std::vector<BuildNode> nodes;
 	uint32_t max_depth = 0;
 	bool spatial = true;
+	uint32_t depth0 = 0;     // depth of this builder's root in the whole tree
+	double root_area = 0.0;  // the whole tree's root surface (0: set from this builder's root)
+	size_t budget = 0;       // references this builder may reach (0: kSplitBudget x its input)
+	size_t grain = 0;        // subtrees of at most this many references go to worker threads (0: none)
 
 	explicit Builder(const std::vector<float>& w) : world(w) {}
+
+	// Subtree `sub` (built for node `at` of this tree) into this tree: its root becomes node `at`,
+	// its other nodes and its slots are appended.
+	void splice(uint32_t at, const Builder& sub) {
+		const int32_t base = int32_t(nodes.size()) - 1;  // sub node k >= 1 lands at base + k
+		const uint32_t obase = uint32_t(order.size());
+		auto mapped = [&](BuildNode n) {
+			if(n.leaf()) n.first += obase;
+			else n.left += base, n.right += base;
+			return n;
+		};
+		nodes[at] = mapped(sub.nodes[0]);
+		for(size_t k = 1; k < sub.nodes.size(); ++k) nodes.push_back(mapped(sub.nodes[k]));
+		order.insert(order.end(), sub.order.begin(), sub.order.end());
+		order_box.insert(order_box.end(), sub.order_box.begin(), sub.order_box.end());
+		max_depth = std::max(max_depth, sub.max_depth);
+	}
 
 	Poly triangle(uint32_t t) const {
 		Poly p;
@@ -142,7 +165,7 @@ struct Builder {
 
 	void build(std::vector<Ref> all) {
 		const size_t n = all.size();
-		const size_t budget = std::min(size_t(double(n) * kSplitBudget), (size_t(1) << 28) - 1);
+		if(budget == 0) budget = std::min(size_t(double(n) * kSplitBudget), (size_t(1) << 28) - 1);
 		size_t refs = n;
 		nodes.reserve(2 * n + 8);
 		nodes.emplace_back();
@@ -150,12 +173,15 @@ struct Builder {
 			uint32_t node, depth;
 			std::vector<Ref> refs;
 		};
-		std::vector<Job> stack;
-		stack.push_back(Job{0, 0, std::move(all)});
-		double root_area = 0.0;
+		std::vector<Job> stack, deferred;
+		stack.push_back(Job{0, depth0, std::move(all)});
 		while(!stack.empty()) {
 			Job j = std::move(stack.back());
 			stack.pop_back();
+			if(grain && j.node != 0 && j.refs.size() <= grain) {
+				deferred.push_back(std::move(j));
+				continue;
+			}
 			max_depth = std::max(max_depth, j.depth);
 			std::vector<Ref>& R = j.refs;
 			Aabb box, cbox;
@@ -165,7 +191,7 @@ struct Builder {
 				cbox.grow(c);
 			}
 			nodes[j.node].box = box;
-			if(j.node == 0) root_area = std::max(box.half_area(), 1e-30);
+			if(j.node == 0 && root_area == 0.0) root_area = std::max(box.half_area(), 1e-30);
 			const uint32_t cnt = uint32_t(R.size());
 			const double leaf_cost = kCostTriangle * cnt;
 			const double parent_area = std::max(box.half_area(), 1e-30);
@@ -231,9 +257,12 @@ struct Builder {
 				const bool overlap = ov.lo[0] <= ov.hi[0] && ov.lo[1] <= ov.hi[1] && ov.lo[2] <= ov.hi[2];
 				try_spatial = overlap && ov.half_area() > kSplitAlpha * root_area;
 			}
-			for(int ax = 0; ax < 3 && try_spatial; ++ax) {
+			// per axis its cheapest plane; the three axes of a large node on threads of their own (the
+			// results are combined in axis order, as one thread would)
+			double axis_cost[3] = {best_s, best_s, best_s}, axis_plane[3] = {0.0, 0.0, 0.0};
+			auto spatial_axis = [&](int ax) {
 				const double lo = box.lo[ax], ext = double(box.hi[ax]) - lo;
-				if(!(ext > 0.0)) continue;
+				if(!(ext > 0.0)) return;
 				const double w = ext / kBins;
 				auto plane_at = [&](int b) { return lo + w * b; };  // low plane of bin b
 				Aabb bb[kBins];
@@ -277,12 +306,27 @@ struct Builder {
 					if(lc == 0 || rc[b + 1] == 0) continue;
 					const double cost =
 						kCostTraverse + kCostTriangle * (lacc.half_area() * lc + racc[b + 1].half_area() * rc[b + 1]) / parent_area;
-					if(cost < best_s) {
-						best_s = cost;
-						s_axis = ax;
-						s_plane = plane_at(b + 1);
+					if(cost < axis_cost[ax]) {
+						axis_cost[ax] = cost;
+						axis_plane[ax] = plane_at(b + 1);
 					}
 				}
+			};
+			if(try_spatial) {
+				if(cnt >= 16384u) {
+					std::thread t1(spatial_axis, 1), t2(spatial_axis, 2);
+					spatial_axis(0);
+					t1.join();
+					t2.join();
+				} else {
+					for(int ax = 0; ax < 3; ++ax) spatial_axis(ax);
+				}
+				for(int ax = 0; ax < 3; ++ax)
+					if(axis_cost[ax] < best_s) {
+						best_s = axis_cost[ax];
+						s_axis = ax;
+						s_plane = axis_plane[ax];
+					}
 			}
 			const bool can_leaf = cnt <= kMaxLeafSize;
 			if(cnt == 1 || (can_leaf && leaf_cost <= std::min(best, best_s)) || (best_axis < 0 && s_axis < 0 && can_leaf)) {
@@ -336,6 +380,30 @@ struct Builder {
 			stack.push_back(Job{uint32_t(l + 1), j.depth + 1, std::move(H)});
 			stack.push_back(Job{uint32_t(l), j.depth + 1, std::move(L)});
 		}
+		if(deferred.empty()) return;
+		// the deferred subtrees, each by its own builder on a pool of threads (the work each does is
+		// fixed by its input, and the subtrees are spliced in a fixed order: the tree does not
+		// depend on the scheduling)
+		std::vector<Builder> subs;
+		subs.reserve(deferred.size());
+		for(Job& j : deferred) {
+			subs.emplace_back(world);
+			Builder& b = subs.back();
+			b.spatial = spatial;
+			b.depth0 = j.depth;
+			b.root_area = root_area;
+			b.budget = std::min(size_t(double(j.refs.size()) * kSplitBudget), (size_t(1) << 28) - 1);
+		}
+		std::atomic<size_t> next{0};
+		auto work = [&]() {
+			for(size_t i = next++; i < subs.size(); i = next++) subs[i].build(std::move(deferred[i].refs));
+		};
+		const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+		std::vector<std::thread> pool;
+		for(unsigned t = 1; t < std::min<size_t>(hw, subs.size()); ++t) pool.emplace_back(work);
+		work();
+		for(std::thread& t : pool) t.join();
+		for(size_t i = 0; i < subs.size(); ++i) splice(deferred[i].node, subs[i]);
 	}
 };
 
@@ -388,6 +456,7 @@ void emit_tree(const std::vector<float>& world, const std::vector<float>& object
 	}
 	Builder b(world);
 	b.spatial = spatial;
+	b.grain = std::max<size_t>(total / 16, 4096);  // subtrees below this size are built in parallel
 	b.build(std::move(refs));
 	out.max_depth = b.max_depth;
 	const size_t slots = b.order.size();
@@ -582,9 +651,10 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 		out.node_count = 0;
 		return true;
 	}
-	// ---- build: with spatial splits, unless that tree is deeper than the traversal stack allows ----
+	// ---- build: with spatial splits, unless that tree is deeper than the traversal stack allows or
+	// its slots overflow the leaf codes' 28 bits ----
 	emit_tree(world, object, inst_of, meta, true, out);
-	if(out.wide_depth > kMaxWideDepth) emit_tree(world, object, inst_of, meta, false, out);
+	if(out.wide_depth > kMaxWideDepth || out.tri_count >= (1u << 28)) emit_tree(world, object, inst_of, meta, false, out);
 	return true;
 }
 
