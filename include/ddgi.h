@@ -247,7 +247,8 @@ ddgi_status ddgi_set_scene(void* handle, const ddgi_vertex* vertices, size_t ver
  * values: ddgi_init_probes re-classifies them for the moved geometry (the "Update State" action,
  * src/UI.cpp:991-993). */
 ddgi_status ddgi_update_instances(void* handle, uint32_t first, uint32_t count, const float* transforms, void* stream);
-/* Node / triangle counts and bytes of the uploaded BVH. */
+/* Node count, triangle slots and bytes of the uploaded BVH (the builder's spatial splits give a
+ * triangle cut by a split plane one slot per piece, so the slots can outnumber the triangles). */
 ddgi_status ddgi_get_bvh_info(void* handle, uint64_t* node_count, uint64_t* triangle_count, uint64_t* bytes);
 /* For every BVH triangle slot (count = triangle_count of ddgi_get_bvh_info; host arrays): the TLAS
  * instance and the primitive index within its mesh (gl_InstanceID / gl_PrimitiveID of a hit on it,

@@ -104,6 +104,8 @@ class IrradianceProbes:
               self.handle)
 
     def bvh_info(self) -> dict:
+        """nodes, triangle slots ("triangles": a triangle cut by a spatial split has one slot per
+        piece) and bytes of the uploaded BVH."""
         n, t, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(self._lib.ddgi_get_bvh_info(self.handle, C.byref(n), C.byref(t), C.byref(b)), self.handle)
         return {"nodes": n.value, "triangles": t.value, "bytes": b.value}
