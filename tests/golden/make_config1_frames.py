@@ -1,12 +1,13 @@
-"""Build-owned regression fixture of the small plain-DDGI workload: the room + 64 boxes (780
+"""Regression fixture of the small plain-DDGI workload: the room + 64 boxes (780
 triangles), 8x4x8 probes x 128 rays, hysteresis 0.97, probe init then 10 frames of the FP64 oracle
 from zeroed atlases, every probe updated every frame.
 
 Writes config1_frames.json beside this script: per frame the SHA-256 of the hit records (count x n
 x 6 uint32: t bits, instance, primitive, back face, u bits, v bits) and of the probe states, and
 for frames 0 and 9 the atlas sums and 64 fixed irradiance and 64 fixed depth values.  These pin the
-oracle (and, through tests/test_gpu_golden.py, the GPU path) against this build's own history: a
-regression pin, not a reference pin (the reference holds no golden vectors for this path).
+oracle (and, through tests/test_gpu_golden.py, the GPU path) against the repository's own earlier
+results: a regression pin, not a reference pin (the reference holds no golden vectors for this
+path).
 
 Run: python tests/golden/make_config1_frames.py"""
 import hashlib
