@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <new>
+#include <system_error>
 #include <thread>
 
 namespace ddgi {
@@ -314,10 +316,19 @@ struct Builder {
 			};
 			if(try_spatial) {
 				if(cnt >= 16384u) {
-					std::thread t1(spatial_axis, 1), t2(spatial_axis, 2);
+					// an axis whose thread cannot start is binned here
+					std::thread t[2];
+					for(int k = 0; k < 2; ++k) {
+						try {
+							t[k] = std::thread(spatial_axis, k + 1);
+						} catch(const std::system_error&) {
+						}
+					}
 					spatial_axis(0);
-					t1.join();
-					t2.join();
+					for(int k = 0; k < 2; ++k) {
+						if(t[k].joinable()) t[k].join();
+						else spatial_axis(k + 1);
+					}
 				} else {
 					for(int ax = 0; ax < 3; ++ax) spatial_axis(ax);
 				}
@@ -395,14 +406,25 @@ struct Builder {
 			b.budget = std::min(size_t(double(j.refs.size()) * kSplitBudget), (size_t(1) << 28) - 1);
 		}
 		std::atomic<size_t> next{0};
+		std::atomic<bool> out_of_memory{false};
 		auto work = [&]() {
-			for(size_t i = next++; i < subs.size(); i = next++) subs[i].build(std::move(deferred[i].refs));
+			for(size_t i = next++; i < subs.size(); i = next++) {
+				try {
+					subs[i].build(std::move(deferred[i].refs));
+				} catch(const std::bad_alloc&) {
+					out_of_memory = true;
+				}
+			}
 		};
 		const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
 		std::vector<std::thread> pool;
-		for(unsigned t = 1; t < std::min<size_t>(hw, subs.size()); ++t) pool.emplace_back(work);
+		try {
+			for(unsigned t = 1; t < std::min<size_t>(hw, subs.size()); ++t) pool.emplace_back(work);
+		} catch(const std::system_error&) {  // fewer threads share the subtrees
+		}
 		work();
 		for(std::thread& t : pool) t.join();
+		if(out_of_memory) throw std::bad_alloc();
 		for(size_t i = 0; i < subs.size(); ++i) splice(deferred[i].node, subs[i]);
 	}
 };
@@ -446,7 +468,7 @@ void transform_point(const float* M, const float* p, float out[3]) {
 
 // Build the tree over the flattened triangles and emit the device layout into `out` (its node,
 // triangle and level fields).
-void emit_tree(const std::vector<float>& world, const std::vector<float>& object, const std::vector<uint32_t>& inst_of,
+static void emit_tree(const std::vector<float>& world, const std::vector<float>& object, const std::vector<uint32_t>& inst_of,
 			   const std::vector<uint32_t>& meta, bool spatial, BvhHost& out) {
 	const size_t total = inst_of.size();
 	std::vector<Ref> refs(total);
@@ -589,7 +611,7 @@ void emit_tree(const std::vector<float>& world, const std::vector<float>& object
 	out.sah_cost = cost;
 }
 
-bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices, size_t index_count,
+static bool build_bvh_unguarded(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices, size_t index_count,
 			   const ddgi_offset_entry* offsets, size_t offset_count, const ddgi_instance* instances,
 			   size_t instance_count, BvhHost& out, const char** err) {
 	out = BvhHost{};
@@ -656,6 +678,19 @@ bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t*
 	emit_tree(world, object, inst_of, meta, true, out);
 	if(out.wide_depth > kMaxWideDepth || out.tri_count >= (1u << 28)) emit_tree(world, object, inst_of, meta, false, out);
 	return true;
+}
+
+bool build_bvh(const ddgi_vertex* vertices, size_t vertex_count, const uint32_t* indices, size_t index_count,
+			   const ddgi_offset_entry* offsets, size_t offset_count, const ddgi_instance* instances,
+			   size_t instance_count, BvhHost& out, const char** err) {
+	try {
+		return build_bvh_unguarded(vertices, vertex_count, indices, index_count, offsets, offset_count, instances,
+								   instance_count, out, err);
+	} catch(const std::bad_alloc&) {
+		out = BvhHost{};
+		*err = "out of host memory building the BVH";
+		return false;
+	}
 }
 
 }  // namespace ddgi
