@@ -113,14 +113,18 @@ __device__ __forceinline__ bool tri_intersect(const RayPrep& r, float4 a, float4
 }
 
 // Conservative slab test of one child box: [tn, tf] clipped to [tmin, tmax], far end widened by
-// a few ulps (robust traversal, Ize 2013); boxes are also inflated at build time.
-__device__ __forceinline__ bool slab(const RayPrep& r, float lox, float hix, float loy, float hiy, float loz, float hiz,
+// a few ulps (robust traversal, Ize 2013); boxes are also inflated at build time.  The planes come
+// as the near and far plane of each axis for this ray (lo where 1/d is positive, hi where it is
+// negative): (p - o) / d rounds monotonically in p, so the near plane's distance is the smaller of
+// the two and no min / max pair per axis is needed.  With d = +-0 the distances are infinite and
+// the test keeps the box whenever the origin lies within its slab.
+__device__ __forceinline__ bool slab(const RayPrep& r, float nx, float fx, float ny, float fy, float nz, float fz,
 									 float tmin, float tmax, float& tn) {
-	const float x0 = (lox - r.o[0]) * r.inv[0], x1 = (hix - r.o[0]) * r.inv[0];
-	const float y0 = (loy - r.o[1]) * r.inv[1], y1 = (hiy - r.o[1]) * r.inv[1];
-	const float z0 = (loz - r.o[2]) * r.inv[2], z1 = (hiz - r.o[2]) * r.inv[2];
-	tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-	const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+	const float xn = (nx - r.o[0]) * r.inv[0], xf = (fx - r.o[0]) * r.inv[0];
+	const float yn = (ny - r.o[1]) * r.inv[1], yf = (fy - r.o[1]) * r.inv[1];
+	const float zn = (nz - r.o[2]) * r.inv[2], zf = (fz - r.o[2]) * r.inv[2];
+	tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, tmin));
+	const float tf = fminf(fminf(xf, yf), fminf(zf, tmax));
 	return tn <= tf * 1.000001f;
 }
 
@@ -236,34 +240,48 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 	uint32_t leaf = kNone;
 	uint32_t n_nodes = 0, n_tris = 0;
 	const float kInf = __int_as_float(0x7f800000);
+	// byte offset in a node of this ray's near plane row per axis (rows lo.x, hi.x, lo.y, hi.y, lo.z,
+	// hi.z): the near plane is lo where 1/d is positive (sign bit clear), hi where not; the far row
+	// is the other one of the pair, its offset the near one's with bit kRowBytes flipped (a node
+	// starts at a multiple of 4 rows)
+	constexpr uint32_t kRowBytes = 16u * (kWide / 4u);
+	uint32_t near_off[3];
+#pragma unroll
+	for(int a = 0; a < 3; ++a) near_off[a] = kRowBytes * (2u * uint32_t(a) + (__float_as_uint(r.inv[a]) >> 31));
+	const char* __restrict__ node_bytes = reinterpret_cast<const char*>(nodes);
 	for(;;) {
 		// descend through inner nodes
 		while(!(node & kLeafFlag)) {
-			const float4* nr = nodes + size_t(node) * (kNodeFloats / 4);
+			// 32-bit offsets from the array's base (the node array stays far below 4 GiB)
+			const uint32_t node_off = node * uint32_t(kNodeFloats * 4);
 			constexpr int G = int(kWide) / 4;  // float4s per row
-			float bx[6][kWide];
+			float bx[6][kWide];  // near x, far x, near y, far y, near z, far z
 			uint32_t code[kWide];
 #pragma unroll
-			for(int row = 0; row < 6; ++row)
+			for(int row = 0; row < 6; ++row) {
+				const uint32_t near = node_off + near_off[row >> 1];
+				const uint32_t off = (row & 1) ? near ^ kRowBytes : near;
 #pragma unroll
 				for(int g = 0; g < G; ++g) {
-					const float4 v = nr[row * G + g];
+					const float4 v = *reinterpret_cast<const float4*>(node_bytes + (off + 16u * uint32_t(g)));
 					bx[row][4 * g] = v.x;
 					bx[row][4 * g + 1] = v.y;
 					bx[row][4 * g + 2] = v.z;
 					bx[row][4 * g + 3] = v.w;
 				}
+			}
 #pragma unroll
 			for(int g = 0; g < G; ++g) {
-				const float4 v = nr[6 * G + g];
+				const float4 v = *reinterpret_cast<const float4*>(node_bytes + (node_off + kRowBytes * 6u + 16u * uint32_t(g)));
 				code[4 * g] = __float_as_uint(v.x);
 				code[4 * g + 1] = __float_as_uint(v.y);
 				code[4 * g + 2] = __float_as_uint(v.z);
 				code[4 * g + 3] = __float_as_uint(v.w);
 			}
 			++n_nodes;
+			// entry distance of each child the ray enters, infinity for the others (slab distances are
+			// finite or clipped to [tmin, tmax], so a finite key is an entered child)
 			float key[kWide];
-			int nh = 0;
 #pragma unroll
 			for(int k = 0; k < int(kWide); ++k) {
 				float tn;
@@ -271,11 +289,9 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 						  code[k] != kEmptyChild)
 							 ? tn
 							 : kInf;
-				nh += int(key[k] != kInf);
 			}
-			if(nh == 0) {
-				DDGI_STACK_POP(node, node = kNone)
-			} else {
+			// sorted by key, the entered children come first
+			{
 				if constexpr(kWide == 4) {
 					DDGI_CSWAP(0, 1)
 					DDGI_CSWAP(2, 3)
@@ -303,10 +319,14 @@ __device__ __forceinline__ HitResult traverse(const float4* __restrict__ nodes, 
 					DDGI_CSWAP(3, 4)
 					DDGI_CSWAP(5, 6)
 				}
+			}
+			if(key[0] == kInf) {
+				DDGI_STACK_POP(node, node = kNone)
+			} else {
 				node = code[0];
 #pragma unroll
 				for(int k = int(kWide) - 1; k >= 1; --k)
-					if(nh > k) DDGI_STACK_PUSH(code[k])
+					if(key[k] != kInf) DDGI_STACK_PUSH(code[k])
 			}
 			if(leaf == kNone && node != kNone && (node & kLeafFlag)) {  // postpone the leaf
 				leaf = node;
