@@ -90,6 +90,51 @@ def test_shadow_rays_any_hit(small):
     probes.destroy()
 
 
+def _pad(v, down):
+    """The build's box padding of a coordinate: the float just below v - (|v| 1e-6 + 1e-6) (or just
+    above v + ...), computed in double."""
+    v = np.asarray(v, np.float64)
+    d = v - (np.abs(v) * 1e-6 + 1e-6) if down else v + (np.abs(v) * 1e-6 + 1e-6)
+    f = d.astype(np.float32)
+    step = np.float32(-np.inf) if down else np.float32(np.inf)
+    off = (f.astype(np.float64) > d) if down else (f.astype(np.float64) < d)
+    return np.where(off, np.nextafter(f, step), f).astype(np.float32)
+
+
+def test_axis_aligned_rays_bit_exact(small):
+    """Rays along the coordinate axes, their other direction components +0 or -0 (1/d infinite on
+    two axes), a third of them from origins that lie on the planes of triangles (the room and the
+    boxes are axis-aligned) or on the padded planes of BVH boxes, where a slab distance is 0 x inf:
+    the traversal must keep every box such a ray meets, so its closest hits equal the brute-force
+    scan's bit for bit and its any-hit decisions agree."""
+    scene, grid, osc = small
+    rng = np.random.Generator(np.random.PCG64(17))
+    n = 6000
+    origin = rng.uniform([0.2, 0.2, 0.2], [19.8, 9.8, 19.8], (n, 3)).astype(np.float32)
+    coords = [np.unique(scene.vertices["pos"][:, k]) for k in range(3)]
+    for i in range(0, n, 3):
+        for k in rng.choice(3, size=int(rng.integers(1, 3)), replace=False):
+            v = coords[k][rng.integers(len(coords[k]))]
+            origin[i, k] = [v, _pad(v, True), _pad(v, False)][int(rng.integers(3))]
+    axis = rng.integers(0, 3, n)
+    sign = np.where(rng.integers(0, 2, n) == 0, 1.0, -1.0).astype(np.float32)
+    d = np.where(rng.integers(0, 2, (n, 3)) == 0, np.float32(0.0), np.float32(-0.0)).astype(np.float32)
+    d[np.arange(n), axis] = sign
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = origin
+    rays[:, 3] = 0.01
+    rays[:, 4:7] = d
+    rays[:, 7] = 1000.0
+    assert np.signbit(d).sum() > n // 2 and (d == 0).sum() == 2 * n  # -0 and +0 components both present
+    probes = _make(scene, grid)
+    want, _ = osc.trace(rays, mask=0xFF, brute=True)
+    _compare_hits(_gpu_trace(probes, rays, 0xFF), want, "axis-aligned closest")
+    want_any, _ = osc.trace(rays, mask=0xFF, any_hit=True, brute=True)
+    got_any = _gpu_trace(probes, rays, 0xFF, any_hit=True)
+    assert np.array_equal(want_any[:, 1] != 0xFFFFFFFF, got_any[:, 1] != 0xFFFFFFFF)
+    probes.destroy()
+
+
 def test_edge_fallback_and_ties_bit_exact():
     """The two decisions of the intersection that the random scenes never reach: a ray whose FP32
     edge function is exactly 0 but whose exact one is negative (the FP64 recomputation makes it a
