@@ -195,7 +195,8 @@ typedef struct ddgi_device_view {
 	void* rays;              /* float4[P*max_rays] (rgb, depth) of the last update, slot-major, entry
 	                            k of a slot holds ray ray_order[k] */
 	void* hits;              /* internal hit records of the last update, same order as rays */
-	void* ray_order;         /* uint16[max_rays]: ray index stored at each position of a slot */
+	void* ray_order;         /* uint16[max_rays]: ray index stored at each position of a slot (set by
+	                            each update: its rays sorted by the world elevation of their direction) */
 	void* probe_indices;     /* uint32[P] probes traced by the last update */
 	void* ray_directions;    /* float4[max_rays] world directions of the last update */
 	uint64_t probe_count;

@@ -117,8 +117,9 @@ uint32_t morton2(uint32_t x, uint32_t y) {
 	return r;
 }
 
-// Upload fib(i, n) for the frame rays, the 512 init rays, and a thread->ray order that groups
-// directions by octahedral cell, so a wavefront traces a compact cone of directions.
+// Upload fib(i, n) for the frame rays, the 512 init rays, and the thread->ray order in force until
+// the first update (directions grouped by octahedral cell, so a wavefront traces a compact cone of
+// directions); each update replaces it by its frame's order (rays by world elevation).
 ddgi_status ensure_fib(Handle* h) {
 	const uint32_t n = h->params.rays_per_probe;
 	if(h->fib_n == n) return DDGI_OK;
@@ -889,6 +890,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	uint32_t key_shift = 0, key_bits = 0;  // shade-list ordering keys
 	shade_key_layout(g, &key_shift, &key_bits);
 	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s, h->shade_count));
+	HIP_TRY(h, launch_frame_order(h->dirs, g.n, h->perm, s));
 	TraceFrameArgs a{};
 	a.scene = h->sd;
 	a.grid = g;

@@ -217,6 +217,9 @@ struct SortJobs {
 uint32_t sort_fine_bits(uint32_t key_bits);
 
 // ---- launchers (return hipGetLastError of the launch) ----
+// The frame's thread -> ray order (kMaxRays entries; the first n sorted by the world elevation of
+// dirs, ties by ray index, the rest unchanged).
+hipError_t launch_frame_order(const float4* dirs, uint32_t n, uint16_t* perm, hipStream_t s);
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);
@@ -284,7 +287,7 @@ struct BatchVolume {
 	Rot R;                 // orientation of this update
 	const float4* fib;     // fib(i, n)
 	float4* dirs;          // world directions, written by k_batch_begin
-	const uint16_t* perm;  // ray order of the traversal threads
+	uint16_t* perm;        // ray order of the traversal threads, written by k_batch_begin
 	float* wt;
 	float* inv_sum;
 	void* w_hi;

@@ -52,6 +52,31 @@ __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __re
 	dirs[i] = frame_dir(R, fib[i]);
 }
 
+// Thread -> ray order of a frame, in one workgroup of kMaxRays threads (thread i holds ray i's world
+// direction y): rays sorted by the elevation of their world direction, ties by ray index, so that
+// the wavefronts of a probe's rays take one band of elevations each.  A band's rays have more
+// alike lengths than a lune from pole to pole has where the scene is wider than it is tall (the
+// rays into the floor and the ceiling end early); the sums of the blend follow the order through
+// its weight rows, and the ray records are stored in it.
+__device__ __forceinline__ void frame_order_block(float y, uint32_t n, float* s_key, uint16_t* perm) {
+	const uint32_t i = threadIdx.x;
+	if(i < n) s_key[i] = y;
+	__syncthreads();
+	if(i < n) {
+		uint32_t rank = 0;
+		for(uint32_t j = 0; j < n; ++j) {
+			const float kj = s_key[j];  // the same address in every lane: one broadcast read
+			rank += (kj < y || (kj == y && j < i)) ? 1u : 0u;
+		}
+		perm[rank] = uint16_t(i);
+	}
+}
+
+__global__ __launch_bounds__(kMaxRays) void k_frame_order(const float4* __restrict__ dirs, uint32_t n, uint16_t* perm) {
+	__shared__ float s_key[kMaxRays];
+	frame_order_block(threadIdx.x < n ? dirs[threadIdx.x].y : 0.0f, n, s_key, perm);
+}
+
 // Ray records of rays that need no shading: misses.rmiss (sky radiance, depth -1) and back faces
 // (closesthit.glsl:137-141: black, depth 0.8 t); with `factors` also their (constant) factor rows.
 __device__ __forceinline__ void write_unshaded(const TraceFrameArgs& a, size_t ri, bool miss, float t) {
@@ -530,12 +555,20 @@ __global__ __launch_bounds__(256) void k_sky(TraceFrameArgs a) {
 }
 
 // ---- batched update of many volumes ----
-// Start of a batched update: block v writes volume v's frame directions; block 0 also resets the
-// work-queue heads and completion stamps and records the start time.
-__global__ __launch_bounds__(512) void k_batch_begin(BatchLaunch b) {
+// Start of a batched update: block v writes volume v's frame directions and thread -> ray order;
+// block 0 also resets the work-queue heads and completion stamps and records the start time.
+__global__ __launch_bounds__(kMaxRays) void k_batch_begin(BatchLaunch b) {
+	static_assert(kMaxRays == 512, "one thread per ray of a volume");
+	__shared__ float s_key[kMaxRays];
 	const BatchVolume& v = b.vols[blockIdx.x];
 	const uint32_t n = v.trace.grid.n;
-	for(uint32_t i = threadIdx.x; i < n; i += 512) v.dirs[i] = frame_dir(v.R, v.fib[i]);
+	float y = 0.0f;
+	if(threadIdx.x < n) {
+		const float4 d = frame_dir(v.R, v.fib[threadIdx.x]);
+		v.dirs[threadIdx.x] = d;
+		y = d.y;
+	}
+	frame_order_block(y, n, s_key, v.perm);
 	if(threadIdx.x == 0) v.trace.shade_count[0] = v.trace.shade_count[kMissCountSlot] = 0u;
 	if(blockIdx.x == 0) {
 		if(threadIdx.x < 4) b.ctl->next[threadIdx.x] = 0u;
@@ -1145,6 +1178,12 @@ __global__ void k_sample(GridConst g, const uint32_t* __restrict__ states, const
 inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
 
 }  // namespace
+
+hipError_t launch_frame_order(const float4* dirs, uint32_t n, uint16_t* perm, hipStream_t s) {
+	if(n == 0 || n > kMaxRays) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_frame_order, dim3(1), dim3(kMaxRays), 0, s, dirs, n, perm);
+	return hipGetLastError();
+}
 
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero) {
