@@ -58,22 +58,30 @@ __global__ void k_frame_dirs(Rot R, const float4* __restrict__ fib, float4* __re
 // alike lengths than a lune from pole to pole has where the scene is wider than it is tall (the
 // rays into the floor and the ceiling end early); the sums of the blend follow the order through
 // its weight rows, and the ray records are stored in it.
+// s_key: kMaxRays floats, 16-byte aligned; entries past n hold +infinity, which no finite key
+// counts.  Reads go four keys at a time to the same address in every lane (broadcast).
 __device__ __forceinline__ void frame_order_block(float y, uint32_t n, float* s_key, uint16_t* perm) {
 	const uint32_t i = threadIdx.x;
-	if(i < n) s_key[i] = y;
+	s_key[i] = i < n ? y : __int_as_float(0x7f800000);
 	__syncthreads();
 	if(i < n) {
 		uint32_t rank = 0;
-		for(uint32_t j = 0; j < n; ++j) {
-			const float kj = s_key[j];  // the same address in every lane: one broadcast read
-			rank += (kj < y || (kj == y && j < i)) ? 1u : 0u;
+		const float4* k4 = reinterpret_cast<const float4*>(s_key);
+#pragma unroll 4
+		for(uint32_t j = 0; j < (n + 3u) / 4u; ++j) {
+			const float4 k = k4[j];
+			const uint32_t j0 = 4u * j;
+			rank += (k.x < y || (k.x == y && j0 < i)) ? 1u : 0u;
+			rank += (k.y < y || (k.y == y && j0 + 1u < i)) ? 1u : 0u;
+			rank += (k.z < y || (k.z == y && j0 + 2u < i)) ? 1u : 0u;
+			rank += (k.w < y || (k.w == y && j0 + 3u < i)) ? 1u : 0u;
 		}
 		perm[rank] = uint16_t(i);
 	}
 }
 
 __global__ __launch_bounds__(kMaxRays) void k_frame_order(const float4* __restrict__ dirs, uint32_t n, uint16_t* perm) {
-	__shared__ float s_key[kMaxRays];
+	__shared__ __attribute__((aligned(16))) float s_key[kMaxRays];
 	frame_order_block(threadIdx.x < n ? dirs[threadIdx.x].y : 0.0f, n, s_key, perm);
 }
 
@@ -559,7 +567,7 @@ __global__ __launch_bounds__(256) void k_sky(TraceFrameArgs a) {
 // block 0 also resets the work-queue heads and completion stamps and records the start time.
 __global__ __launch_bounds__(kMaxRays) void k_batch_begin(BatchLaunch b) {
 	static_assert(kMaxRays == 512, "one thread per ray of a volume");
-	__shared__ float s_key[kMaxRays];
+	__shared__ __attribute__((aligned(16))) float s_key[kMaxRays];
 	const BatchVolume& v = b.vols[blockIdx.x];
 	const uint32_t n = v.trace.grid.n;
 	float y = 0.0f;
