@@ -311,7 +311,7 @@ ddgi_status ddgi_init_probes(void* handle, const float orientation[9], void* str
 ddgi_status ddgi_update(void* handle, const float orientation[9], uint32_t flags, void* stream);
 
 /* Blend only: feed an externally produced ray buffer `rays` (device float4[count*n], slot-major
- * (rgb, depth), n = rays_per_probe) for probes `probe_indices` (device uint32[count]) and the
+ * (rgb, depth) in ray-index order, n = rays_per_probe) for probes `probe_indices` (device uint32[count]) and the
  * frame orientation, then run blend + border + state update + publish like ddgi_update does.
  * Probes not listed keep their tiles: the next atlas starts as a copy of the exposed one.  With
  * DDGI_UPDATE_ALL_PROBES in `flags` and count = the probe count, the caller states that
