@@ -889,8 +889,7 @@ static ddgi_status run_update(Handle* h, const Rot& R, uint32_t flags, hipStream
 	const GridConst g = make_grid_const(*h);
 	uint32_t key_shift = 0, key_bits = 0;  // shade-list ordering keys
 	shade_key_layout(g, &key_shift, &key_bits);
-	HIP_TRY(h, launch_frame_dirs(R, h->fib, h->dirs, g.n, s, h->shade_count));
-	HIP_TRY(h, launch_frame_order(h->dirs, g.n, h->perm, s));
+	HIP_TRY(h, launch_frame_dirs_order(R, h->fib, h->dirs, g.n, h->perm, s, h->shade_count));
 	TraceFrameArgs a{};
 	a.scene = h->sd;
 	a.grid = g;

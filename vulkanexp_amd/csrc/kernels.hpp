@@ -217,9 +217,10 @@ struct SortJobs {
 uint32_t sort_fine_bits(uint32_t key_bits);
 
 // ---- launchers (return hipGetLastError of the launch) ----
-// The frame's thread -> ray order (kMaxRays entries; the first n sorted by the world elevation of
-// dirs, ties by ray index, the rest unchanged).
-hipError_t launch_frame_order(const float4* dirs, uint32_t n, uint16_t* perm, hipStream_t s);
+// The frame's directions (as launch_frame_dirs) and thread -> ray order (kMaxRays entries; the first
+// n sorted by the world elevation of the directions, ties by ray index, the rest unchanged).
+hipError_t launch_frame_dirs_order(const Rot& R, const float4* fib, float4* dirs, uint32_t n, uint16_t* perm,
+								   hipStream_t s, uint32_t* zero = nullptr);
 // zero (optional): a counter cleared by the same launch
 hipError_t launch_frame_dirs(const Rot& R, const float4* fib, float4* dirs, uint32_t count, hipStream_t s,
 							 uint32_t* zero = nullptr);

@@ -80,9 +80,22 @@ __device__ __forceinline__ void frame_order_block(float y, uint32_t n, float* s_
 	}
 }
 
-__global__ __launch_bounds__(kMaxRays) void k_frame_order(const float4* __restrict__ dirs, uint32_t n, uint16_t* perm) {
+// The update's frame directions and thread -> ray order in one workgroup of kMaxRays threads
+// (thread i: ray i); `zero` as in k_frame_dirs.
+__global__ __launch_bounds__(kMaxRays) void k_frame_dirs_order(Rot R, const float4* __restrict__ fib,
+															   float4* __restrict__ dirs, uint32_t n, uint16_t* perm,
+															   uint32_t* zero) {
 	__shared__ __attribute__((aligned(16))) float s_key[kMaxRays];
-	frame_order_block(threadIdx.x < n ? dirs[threadIdx.x].y : 0.0f, n, s_key, perm);
+	const uint32_t i = threadIdx.x;
+	if(zero && i < kShadeSegments) zero[kCounterStride * i] = 0u;
+	if(zero && i == 0) zero[kMissCountSlot] = 0u;
+	float y = 0.0f;
+	if(i < n) {
+		const float4 d = frame_dir(R, fib[i]);
+		dirs[i] = d;
+		y = d.y;
+	}
+	frame_order_block(y, n, s_key, perm);
 }
 
 // Ray records of rays that need no shading: misses.rmiss (sky radiance, depth -1) and back faces
@@ -1187,9 +1200,11 @@ inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1)
 
 }  // namespace
 
-hipError_t launch_frame_order(const float4* dirs, uint32_t n, uint16_t* perm, hipStream_t s) {
+hipError_t launch_frame_dirs_order(const Rot& R, const float4* fib, float4* dirs, uint32_t n, uint16_t* perm,
+								   hipStream_t s, uint32_t* zero) {
+	static_assert(kShadeSegments <= kMaxRays, "one thread clears each shade-list counter");
 	if(n == 0 || n > kMaxRays) return hipErrorInvalidValue;
-	hipLaunchKernelGGL(k_frame_order, dim3(1), dim3(kMaxRays), 0, s, dirs, n, perm);
+	hipLaunchKernelGGL(k_frame_dirs_order, dim3(1), dim3(kMaxRays), 0, s, R, fib, dirs, n, perm, zero);
 	return hipGetLastError();
 }
 
