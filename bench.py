@@ -590,6 +590,8 @@ def main():
     if dom == "trace_primary":
         roofline["limiter"] = ("latency of dependent node fetches under lane divergence: a wavefront runs until its "
                                "longest ray ends (VALU lane utilisation in the profile summary), not HBM bandwidth")
+        roofline["frac_note"] = ("achieved counts every lane's node and triangle fetches, most of them served by L1, L2 "
+                                 "and the Infinity Cache, so frac can pass 1; traffic_frac_of_peak is the HBM share")
     if peaks:
         roofline["cache_ceiling"] = peaks["cache_ceiling_GB/s"]
         roofline["frac_of_cache_ceiling"] = round(achieved / peaks["cache_ceiling_GB/s"], 4)
