@@ -256,6 +256,30 @@ def test_update_frames_small_parity(small):
     _check(_frame_compare(scene, grid, osc, 4, seed=21), "all probes")
 
 
+def test_ray_order_follows_each_frame(small):
+    """Each update's thread -> ray order (`ray_order` of the device view) lists its rays by the
+    world elevation (y) of their directions, ties by ray index, and changes with the rotation; the
+    entries past the ray count stay the identity."""
+    import torch
+    from vulkanexp_amd.probes import _device_copy
+    scene, grid, _ = small
+    p = _make(scene, grid)
+    n = int(grid["rays_per_probe"])
+    orders = []
+    for R in scenes.orientations(23, 2):
+        p.update(R, _lib.UPDATE_ALL_PROBES)
+        p.synchronize()
+        v = p.device_view()
+        dirs = _device_copy(v.ray_directions, n * 4, torch.float32).numpy().reshape(n, 4)
+        m = int(v.max_rays_per_probe)
+        order = _device_copy(v.ray_order, m, torch.int16).numpy().view(np.uint16).astype(np.int64)
+        assert np.array_equal(order[:n], np.argsort(dirs[:, 1], kind="stable")), "not sorted by world y"
+        assert np.array_equal(order[n:], np.arange(n, m))
+        orders.append(order[:n])
+    p.destroy()
+    assert not np.array_equal(orders[0], orders[1])
+
+
 def test_update_reference_schedule_parity(small):
     """The reference's per-frame path: update() with the probes selectProbesToUpdate picks
     (src/IrradianceProbes.cpp:396-424) from random states, without and with a ProbesPerUpdate cap
